@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of the VBD time step on one MI355X.
+
+Workload (default): the 128^3-cell noisy cube -- 129^3 = 2,146,689 vertices, 12,582,912
+Kuhn tets, mu = 1e5, lambda = 4e5, rho = 1000, bottom face pinned, free vertices jittered by
+seeded uniform +-0.3 cell, h = 1/60, 20 VBD iterations per step, Chebyshev rho = 0.95.
+A "step" is one whole time step (predict, 20 x (8 color launches + blend), velocity update)
+replayed from one captured hipGraph with all state resident in HBM.
+
+Prints one JSON line.  value = vertex-iterations per second (millions): all vertices
+(pinned included) x iterations x steps / seconds.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "M vertex-iters/s + %HBM roofline, 1 MI355X; batched & adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+H = 1.0 / 60.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(name: str):
+    """HBM bytes per color launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    as committed in profiles/<name>.txt on a line 'traffic_bytes_per_launch: <float>'."""
+    path = os.path.join(ROOT, "profiles", name + ".txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(m, iterations_sample_seconds: float = 8.0):
+    """Reference-cost CPU restatement (oracle mode 2: full 12x12 element Hessian per
+    (vertex, tet) call and a per-call std::vector, as VBD.cpp:205-256 / Energy.cpp:90-158)
+    timed on a bounded sample of the same mesh: one sweep over the first n free vertices of
+    every color, all host threads inside each color (race-free; result independent of the
+    thread count)."""
+    from oracle import oracle as O
+    prep = O.prepare(m)
+    y = m.x0.copy()
+    y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(cores, 16))
+    p = O.make_params(dt=H, iterations=1, rho=0.95, mu=m.mu, lam=m.lam, gravity=m.gravity,
+                      mode=O.MODE_FAITHFUL_COST, threads=cores)
+
+    def sample(n):
+        out = []
+        for c in range(prep.n_colors):
+            vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
+            vs = vs[m.fixed[vs] == 0][:n]
+            out.append(vs)
+        return out
+
+    def run(n):
+        secs, cnt = 0.0, 0
+        for vs in sample(n):
+            _, s = O.solve_subset(prep, p, m.x0, y, vs.astype(np.int32))
+            secs += s
+            cnt += vs.size
+        return secs, cnt
+
+    s0, c0 = run(64 * cores)
+    rate = c0 / max(s0, 1e-9)
+    n = int(max(64, min(rate * iterations_sample_seconds / prep.n_colors, 250000)))
+    secs, cnt = run(n)
+    rate = cnt / secs
+    return {"value": rate / 1e6, "unit": "M vertex-iters/s", "cores": cores, "kind": "port",
+            "sample": f"one sweep over the first {n} free vertices of each of the {prep.n_colors} colors of the "
+                      f"same 128^3 cube ({cnt} vertex solves, {secs:.1f} s), reference cost structure "
+                      f"(full 12x12 per (vertex, tet), per-call std::vector), {cores} threads inside each color"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128, help="cube cells per side")
+    ap.add_argument("--iterations", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--block", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (multi-GPU sharding "
+                                   "is design-only, see DESIGN.md)"}))
+        sys.exit(2)
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if ws > 1:
+        print("one GPU only: WORLD_SIZE > 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from vbd_tetmesh_amd import VBDParams, meshgen
+    from vbd_tetmesh_amd.solver import upload
+
+    t0 = time.time()
+    m = meshgen.noisy_cube(args.n, seed=0)
+    params = VBDParams(dt=H, numIterations=args.iterations, substeps=1, gravity=m.gravity, useAcceleration=True,
+                       accelerationRho=0.95, mu=m.mu, lambda_=m.lam)
+    d = upload(m, params=params)
+    info = d.info()
+    log(f"[bench] cube{args.n}: nV={info.num_verts} nT={info.num_tets} colors={info.num_colors} "
+        f"pairs={info.num_pairs} device={info.device_bytes / 2**30:.2f} GiB setup {time.time() - t0:.1f}s")
+    tune = None
+    if args.lanes:
+        d.set_launch_config(args.lanes, args.block)
+    elif not args.no_autotune:
+        d.set_positions(m.x0)
+        tune = d.autotune(3)
+        d.set_positions(m.x0)
+        log(f"[bench] autotune -> lanes={tune[0]} block={tune[1]} ({tune[2]:.3f} ms/sweep)")
+    info = d.info()
+
+    # warmup
+    d.step(args.warmup)
+    d.synchronize()
+    # timed region
+    d.synchronize()
+    t1 = time.perf_counter()
+    d.step(args.steps)
+    d.synchronize()
+    t2 = time.perf_counter()
+    secs = t2 - t1
+    vi = float(info.num_verts) * args.iterations * args.steps
+    value = vi / secs / 1e6
+    log(f"[bench] {args.steps} steps in {secs * 1e3:.1f} ms -> {value:.1f} M vertex-iters/s")
+    x = d.positions()
+    if not np.all(np.isfinite(x)):
+        raise SystemExit("non-finite positions after the timed steps")
+
+    # dominant kernel (color solve): live HIP-event timing on the solver's stream
+    cms, cn, bms, bn = d.time_iterations(args.iterations)
+    cbytes, bbytes = d.algorithmic_bytes()
+    per_launch_bytes = cbytes / info.num_colors
+    per_launch_s = cms / cn / 1e3
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    traffic = read_traffic(f"traffic_color_solve_cube{args.n}")
+    b_alg_per_vi = (cbytes + bbytes) / info.num_verts
+    overall = vi / secs * b_alg_per_vi / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "M vertex-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded noisy cube, generated in-process)",
+        "config": {"workload": f"cube{args.n} noisy Kuhn-split cube, {info.num_verts} verts, {info.num_tets} tets, "
+                               f"{args.iterations} iters/step, Chebyshev 0.95, h=1/60, bottom face pinned",
+                   "iterations_per_step": args.iterations, "colors": info.num_colors,
+                   "lanes_per_vertex": info.lanes_per_vertex, "block_size": info.block_size},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_color_solve", "alg_bytes_per_launch": round(per_launch_bytes),
+                     "avg_launch_ms": round(per_launch_s * 1e3, 4), "launches_timed": cn},
+        "roofline_step": {"alg_bytes_per_vertex_iter": round(b_alg_per_vi, 1), "achieved": round(overall, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(overall / HBM_PEAK_GBS, 4),
+                          "blend_avg_launch_ms": round(bms / max(bn, 1), 4)},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(m)
+        out["cpu_baseline"] = cb
+        log(f"[bench] cpu baseline {cb['value'] * 1e6:.0f} vertex-iters/s on {cb['cores']} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * vbd_capi.h -- C ABI of the MI355X (gfx950) Vertex Block Descent solver.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.  Every entry
+ * point returns an int status (VBD_OK = 0); vbd_last_error() returns the message of the
+ * last failure on a handle.  A handle is not thread-safe; distinct handles may be used from
+ * distinct host threads.  All device work of a handle is issued on that handle's own HIP
+ * stream; host-buffer entry points (upload/download) synchronise that stream.
+ *
+ * Vertex data crossing the ABI uses the reference TetMesh layout: 3 floats per vertex, xyz
+ * contiguous (TVerticesMat, Types.h:53), in the caller's vertex numbering.  Tets are 4 int32
+ * per tet (TetVIdsArr, Types.h:58); DmInv is 9 floats per tet, row-major, as
+ * VBD_GPU_Solver::initFromCPU flattens it (VBD_GPU_Wrapper.cu:277-282).
+ *
+ * Reference interfaces replaced (file:line in extinguish7/VBD_TetMesh):
+ *   vbd_create / vbd_destroy      VBD_GPU_Solver ctor / dtor   VBD_GPU_Wrapper.h:9-12, .cu:373-382
+ *   vbd_init                      VBD_GPU_Solver::initFromCPU  VBD_GPU_Wrapper.h:15-16, .cu:246-315
+ *   vbd_solve_iteration           VBD_GPU_Solver::solveIteration  VBD_GPU_Wrapper.h:19, .cu:317-329
+ *   vbd_solve                     VBD_GPU_Solver::solveVBD     VBD_GPU_Wrapper.h:28-29, .cu:332-362
+ *   vbd_download_positions        VBD_GPU_Solver::copyPositionsToCPU  VBD_GPU_Wrapper.h:22, .cu:364-366
+ *   vbd_upload_inertia            VBD_GPU_Solver::updateInertiaFromCPU VBD_GPU_Wrapper.h:25, .cu:369-371
+ *   vbd_set_params                VBDsolver::setParams          VBD.h:41, VBD.cpp:19-21
+ *   vbd_step                      VBDsolver::step               VBD.h:58, VBD.cpp:60-101
+ *   vbd_forward_step              VBDsolver::forwardStep        VBD.h:99, VBD.cpp:103-112
+ *   vbd_update_velocity           TetMesh::updateVelocity       TetMesh.h:305-311
+ *   vbd_mesh_initialize           TetMesh::initialize (mass, adjacency)  TetMesh.h:213-275
+ *   vbd_color_vertices            GraphColoring::colorVertices  GraphColoring.h:25-29, .cpp:28-66
+ *   vbd_precompute_tet_data       VBDsolver::precomputeTetData  VBD.h:51, VBD.cpp:36-58
+ *   vbd_accelerator_omega         VBDsolver::getAcceleratorOmega  VBD.h:139, VBD.cpp:302-311
+ *   vbd_total_energy              VBDsolver::computeTotalEnergy   VBD.h:69, VBD.cpp:258-288
+ * Entry points without a reference counterpart say so in their comment.
+ */
+#ifndef VBD_CAPI_H
+#define VBD_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VBD_CAPI_VERSION 1
+
+enum vbd_status {
+    VBD_OK = 0,
+    VBD_ERR_ARG = 1,     /* invalid argument (null pointer, size, index out of range) */
+    VBD_ERR_HIP = 2,     /* a HIP runtime call failed; message names it */
+    VBD_ERR_STATE = 3,   /* call order violated (e.g. solve before init) */
+    VBD_ERR_ALLOC = 4,   /* device or host allocation failed */
+    VBD_ERR_NUMERIC = 5  /* non-finite input detected */
+};
+
+typedef struct vbd_solver vbd_solver;
+
+/* Mesh as handed to vbd_init.  All arrays are borrowed for the duration of the call. */
+typedef struct vbd_mesh_desc {
+    int32_t num_verts;
+    int32_t num_tets;
+    const float* positions;     /* 3*num_verts, current positions (TetMesh::mVertPos) */
+    const float* inertia;       /* 3*num_verts or NULL (TetMesh::mInertia); NULL -> positions */
+    const float* velocity;      /* 3*num_verts or NULL (TetMesh::mVelocity); NULL -> zero */
+    const float* mass;          /* num_verts (TetMesh::vertexMass) */
+    const uint8_t* is_fixed;    /* num_verts, nonzero = pinned (TetMesh::isFixed) */
+    const int32_t* tets;        /* 4*num_tets (TetMesh::tets) */
+    const float* dminv;         /* 9*num_tets row-major (VBDsolver::tetDmInv) */
+    const float* rest_volume;   /* num_tets (VBDsolver::tetRestVolume) */
+    const float* tet_mu;        /* num_tets or NULL -> scalar mu of each solve call / params */
+    const float* tet_lambda;    /* num_tets or NULL (must be NULL iff tet_mu is NULL) */
+    int32_t num_colors;
+    const int32_t* color_offsets; /* num_colors+1 offsets into color_verts */
+    const int32_t* color_verts;   /* every vertex exactly once (VBDsolver::vertexColors) */
+} vbd_mesh_desc;
+
+/* Per-step parameters, VBDParams (VBD.h:14-25) plus build-defined extensions. */
+typedef struct vbd_step_params {
+    float dt;                   /* frame time step (VBDParams::dt) */
+    int32_t num_iterations;     /* VBD iterations per substep */
+    int32_t substeps;
+    float gravity[3];
+    int32_t use_acceleration;   /* Chebyshev acceleration on/off */
+    float acceleration_rho;
+    float mu;                   /* used when the mesh has no per-tet materials */
+    float lambda;
+    float velocity_damping;     /* 0.999 in the reference (TetMesh.h:308) */
+    int32_t start_from_inertia; /* 0: iterate from x_t (VBD.cpp); 1: from y (VBD_GPU_Wrapper.cu:334) */
+    int32_t contact_on;         /* extension: penalty plane x[axis] >= plane, E = k/2 max(0, plane-x)^2 */
+    int32_t contact_axis;
+    float contact_k;
+    float contact_plane;
+} vbd_step_params;
+
+typedef struct vbd_info {
+    int32_t num_verts;
+    int32_t num_tets;
+    int32_t num_free;           /* vertices that are solved (not pinned) */
+    int32_t num_colors;
+    int64_t num_pairs;          /* (free vertex, incident tet) records streamed per iteration */
+    int32_t max_valence;
+    int32_t per_tet_material;
+    int32_t lanes_per_vertex;   /* current launch configuration of the color kernel */
+    int32_t block_size;
+    int64_t device_bytes;       /* bytes of device memory the handle holds */
+} vbd_info;
+
+/* ---- lifecycle -------------------------------------------------------------------- */
+int vbd_create(vbd_solver** out, int32_t device);
+void vbd_destroy(vbd_solver* h);
+const char* vbd_last_error(const vbd_solver* h);
+/* Message of the last vbd_create failure or of a host helper (no handle). */
+const char* vbd_global_last_error(void);
+int32_t vbd_capi_version(void);
+
+/* ---- upload / transfer ------------------------------------------------------------ */
+/* Builds on the device: the vertex->tet CSR (entries tet*4 + corner, ascending tet id per
+ * vertex), the per-color vertex lists, a color-major renumbering of the free vertices and
+ * the per-(vertex, tet) records streamed by the color kernel. */
+int vbd_init(vbd_solver* h, const vbd_mesh_desc* mesh);
+/* Batched upload (no reference counterpart): the disjoint union of num_meshes meshes, each
+ * with its own coloring, tets in its own local numbering; colors are merged by index, so one
+ * color launch covers every mesh.  Vertex arrays on later transfers are the concatenation in
+ * mesh order. */
+int vbd_init_batch(vbd_solver* h, int32_t num_meshes, const vbd_mesh_desc* meshes);
+int vbd_upload_positions(vbd_solver* h, const float* positions);
+int vbd_upload_inertia(vbd_solver* h, const float* inertia);
+int vbd_upload_velocities(vbd_solver* h, const float* velocity);
+int vbd_download_positions(vbd_solver* h, float* positions);
+int vbd_download_velocities(vbd_solver* h, float* velocity);
+int vbd_download_inertia(vbd_solver* h, float* inertia);
+int vbd_get_info(const vbd_solver* h, vbd_info* info);
+/* GPU-built adjacency in the caller's numbering: offsets (num_verts+1), entries (4*num_tets)
+ * = tet*4 + local corner. */
+int vbd_get_adjacency(vbd_solver* h, int32_t* offsets, int32_t* entries);
+/* GPU-built per-color vertex lists: offsets (num_colors+1), vertices (num_verts). */
+int vbd_get_color_lists(vbd_solver* h, int32_t* offsets, int32_t* verts);
+
+/* ---- solve (VBD_GPU_Solver surface) ------------------------------------------------ */
+int vbd_solve_iteration(vbd_solver* h, float dt_sqr_reciprocal, float mu, float lambda);
+int vbd_solve(vbd_solver* h, int32_t num_iterations, float dt_sqr_reciprocal, float mu, float lambda,
+              int32_t use_acceleration, float rho, int32_t start_from_inertia);
+
+/* ---- whole time step on the device (VBDsolver surface) ----------------------------- */
+int vbd_set_params(vbd_solver* h, const vbd_step_params* params);
+int vbd_forward_step(vbd_solver* h);
+int vbd_update_velocity(vbd_solver* h);
+/* num_frames calls of VBDsolver::step; the step is captured once in a hipGraph and
+ * replayed.  Asynchronous: returns after enqueueing; transfers or vbd_synchronize wait. */
+int vbd_step(vbd_solver* h, int32_t num_frames);
+int vbd_synchronize(vbd_solver* h);
+/* Enables (1, default) or disables (0) hipGraph capture of vbd_step. */
+int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
+
+/* ---- launch configuration, autotuning and timing (no reference counterpart) ------- */
+int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size);
+/* Times each candidate (lanes_per_vertex x block_size) over `iterations` sweeps on the
+ * uploaded mesh, restores the positions, keeps the fastest; writes it to the outputs. */
+int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_block, float* best_ms);
+/* Runs `iterations` plain sweeps (+ blends if use_acceleration in params) with HIP events
+ * around every launch on the handle's stream.  Outputs: total color-kernel milliseconds,
+ * number of color launches, total blend milliseconds, number of blend launches. */
+int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int32_t* color_launches,
+                        double* blend_ms, int32_t* blend_launches);
+/* Algorithmic bytes of one sweep's color launches and blend launches (see DESIGN). */
+int vbd_algorithmic_bytes(const vbd_solver* h, double* color_bytes_per_sweep, double* blend_bytes_per_sweep);
+
+/* ---- host helpers (CPU; no device work) ------------------------------------------- */
+/* TetMesh::initialize: per-tet rest volume, lumped mass rho*V/4 per corner and vertex->tet
+ * adjacency in ascending tet order (adj_offsets num_verts+1, adj_tets 4*num_tets). */
+int vbd_mesh_initialize(int32_t num_verts, const float* rest, int32_t num_tets, const int32_t* tets, float density,
+                        float* mass, float* tet_rest_volume, int32_t* adj_offsets, int32_t* adj_tets);
+/* Greedy first-fit coloring in ascending vertex order.  color_offsets must hold
+ * num_verts+1 entries; returns the color count in *num_colors. */
+int vbd_color_vertices(int32_t num_verts, const int32_t* tets, const int32_t* adj_offsets, const int32_t* adj_tets,
+                       int32_t* color_of, int32_t* color_offsets, int32_t* color_verts, int32_t* num_colors);
+int vbd_precompute_tet_data(const float* rest, int32_t num_tets, const int32_t* tets, float* dminv,
+                            float* rest_volume);
+float vbd_accelerator_omega(int32_t order, float rho, float prev_omega);
+/* Device reduction of the diagnostic log-Neo-Hookean energy (|det F| for the singular
+ * value product) over the current positions. */
+int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VBD_CAPI_H */

@@ -1,0 +1,291 @@
+"""GPU parity: the gfx950 path through the C ABI against the CPU oracle.
+
+Tolerance (stated per test): positions per frame within 1e-5 relative max-norm,
+max|x_gpu - x_oracle| / max|x_oracle|, against the FP32 oracle (VBD.cpp semantics).
+Integer/index work (adjacency, color lists) must match bit-exactly.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vbd_tetmesh_amd import VBDParams, meshgen
+from vbd_tetmesh_amd.solver import DeviceMesh, TetMesh, VBD_GPU_Solver, prepare_host, upload
+
+pytestmark = pytest.mark.gpu
+
+H = 1.0 / 60.0
+TOL = 1e-5
+
+
+def relmax(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def params_for(m, iterations, accel=True, rho=0.95, contact=None, start_from_inertia=False):
+    return VBDParams(dt=H, numIterations=iterations, substeps=1, gravity=m.gravity, useAcceleration=accel,
+                     accelerationRho=rho, mu=m.mu, lambda_=m.lam, contact=contact,
+                     startFromInertia=start_from_inertia)
+
+
+def oracle_params(m, iterations, accel=True, rho=0.95, contact=None, mode=O.MODE_CLOSED, init_from_inertia=False):
+    return O.make_params(dt=H, iterations=iterations, gravity=m.gravity, accel=accel, rho=rho, mu=m.mu, lam=m.lam,
+                         contact=contact, mode=mode, init_from_inertia=init_from_inertia)
+
+
+def run_gpu_frames(m, params, frames, lanes=None, block=256, graph=True):
+    d = upload(m, params=params)
+    if lanes is not None:
+        d.set_launch_config(lanes, block)
+    d.set_graph(graph)
+    out = []
+    for _ in range(frames):
+        d.step(1)
+        out.append(d.positions())
+    return np.stack(out), d
+
+
+@pytest.mark.parametrize("gen", ["beam", "cube16", "sphere"])
+def test_device_built_adjacency_and_colors_match_host(gen):
+    m = {"beam": lambda: meshgen.beam(), "cube16": lambda: meshgen.noisy_cube(16, seed=1),
+         "sphere": lambda: meshgen.voxel_sphere(20, seed=3)}[gen]()
+    print(f"[upload {gen}] nV={m.num_verts} nT={m.num_tets}", flush=True)
+    t, off, verts, dminv, vol = prepare_host(m)
+    d = upload(m)
+    aoff, aent = d.adjacency()
+    assert np.array_equal(aoff, t.adjOffsets)
+    assert np.array_equal(aent >> 2, t.adjTets)
+    corner = aent & 3
+    assert np.array_equal(m.tets[aent >> 2, corner], np.repeat(np.arange(m.num_verts), np.diff(aoff)))
+    coff, cverts = d.color_lists()
+    assert np.array_equal(coff, off) and np.array_equal(cverts, verts)
+    info = d.info()
+    assert info.num_free == int((m.fixed == 0).sum())
+    assert info.num_pairs == int(np.diff(t.adjOffsets)[m.fixed == 0].sum())
+    # round trip of the vertex state through the slot permutation
+    assert np.array_equal(d.positions(), m.x0)
+
+
+def test_one_sweep_matches_oracle_vertex_solves():
+    """One VBD_GPU_Solver::solveIteration vs the oracle's per-vertex solves, color by color."""
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    t, off, verts, dminv, vol = prepare_host(m)
+    y = m.x0.copy()
+    y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
+    g = VBD_GPU_Solver()
+    t.mInertia[:] = y
+    g.initFromCPU(t, (off, verts), dminv, vol)
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    g.solveIteration(sdt, m.mu, m.lam)
+    got = g.device_mesh.positions()
+    x = m.x0.copy()
+    p = oracle_params(m, 1, accel=False)
+    for c in range(prep.n_colors):
+        vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
+        vs = vs[m.fixed[vs] == 0]
+        xo, _ = O.solve_subset(prep, p, x, y, vs)
+        x[vs] = xo
+    disp = np.abs(x - m.x0).max()
+    assert disp > 0
+    err = np.abs(got.astype(np.float64) - x).max()
+    print(f"one sweep: max |dx| {disp:.3e}, gpu-oracle {err:.3e}", flush=True)
+    assert err <= 1e-4 * disp
+
+
+def test_beam_60_frames_matches_fp32_oracle():
+    """Cantilever beam 20x4x4 (525 verts), 100 iterations, Chebyshev 0.95, 60 frames."""
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    _, _, of32, _ = O.simulate(prep, oracle_params(m, 100), m.x0, None, 60)
+    _, _, of64, _ = O.simulate(prep, oracle_params(m, 100), m.x0, None, 60, f64=True, rest=m.rest,
+                               density=m.density)
+    got, _ = run_gpu_frames(m, params_for(m, 100), 60)
+    rel = [relmax(got[f], of32[f]) for f in range(60)]
+    rel64 = [relmax(got[f], of64[f]) for f in range(60)]
+    o3264 = [relmax(of32[f], of64[f]) for f in range(60)]
+    print(f"beam: max rel vs fp32 oracle {max(rel):.3e}, gpu vs fp64 {max(rel64):.3e}, "
+          f"fp32 oracle vs fp64 {max(o3264):.3e}", flush=True)
+    assert max(rel) <= TOL
+
+
+def test_beam_matches_faithful_oracle():
+    """Against the Energy.cpp-faithful force path (9x9 directional derivatives), 5 frames."""
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, oracle_params(m, 100, mode=O.MODE_FAITHFUL), m.x0, None, 5)
+    got, _ = run_gpu_frames(m, params_for(m, 100), 5)
+    rel = max(relmax(got[f], of[f]) for f in range(5))
+    print(f"beam faithful: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_main_cpp_hybrid_loop_with_gpu_solver():
+    """The reference main.cpp:215-232 pattern: host forwardStep, updateInertiaFromCPU,
+    solveVBD (x <- y start), copyPositionsToCPU, host updateVelocity."""
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    frames = 20
+    _, _, of, _ = O.simulate(prep, oracle_params(m, 30, rho=0.95, init_from_inertia=True), m.x0, None, frames)
+    t, off, verts, dminv, vol = prepare_host(m)
+    g = VBD_GPU_Solver()
+    t.computeInertia(H)
+    g.initFromCPU(t, (off, verts), dminv, vol)
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    rel = 0.0
+    for f in range(frames):
+        t.savePreviousPosition()
+        t.applyGravity(m.gravity, H)
+        t.computeInertia(H)
+        g.updateInertiaFromCPU(t)
+        g.solveVBD(30, sdt, m.mu, m.lam, True, 0.95)
+        g.copyPositionsToCPU(t)
+        t.updateVelocity(H)
+        rel = max(rel, relmax(t.mVertPos, of[f]))
+    print(f"hybrid loop: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_graph_replay_is_bitwise_identical_to_eager():
+    m = meshgen.beam(12, 3, 3)
+    a, _ = run_gpu_frames(m, params_for(m, 20), 6, graph=True)
+    b, _ = run_gpu_frames(m, params_for(m, 20), 6, graph=False)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("lanes,block", [(4, 64), (8, 128), (16, 256), (32, 256), (64, 256), (8, 192)])
+def test_launch_configurations_agree(lanes, block):
+    m = meshgen.noisy_cube(12, seed=3)
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, oracle_params(m, 20), m.x0, None, 3)
+    got, _ = run_gpu_frames(m, params_for(m, 20), 3, lanes=lanes, block=block)
+    rel = max(relmax(got[f], of[f]) for f in range(3))
+    print(f"cube12 lanes={lanes} block={block}: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_noisy_cube_32_matches_oracle():
+    """The 128^3 workload's setup (noise, pins, Chebyshev) at 32^3, 20 iterations, 3 frames."""
+    m = meshgen.noisy_cube(32, seed=0)
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=20, rho=0.95, threads=8), m.x0, None, 3)
+    got, _ = run_gpu_frames(m, params_for(m, 20), 3)
+    rel = max(relmax(got[f], of[f]) for f in range(3))
+    print(f"cube32: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_cube128_color_launches_match_oracle_on_sampled_vertices():
+    """Full-size 128^3 cube (2.15M verts, 12.6M tets): one device sweep, every color launch
+    checked on a seeded sample of 2000 of its vertices against the oracle's local solves from
+    the state that launch saw (earlier colors already updated by the device, later colors
+    and this color still at the start positions)."""
+    m = meshgen.noisy_cube(128, seed=0)
+    print(f"[cube128] nV={m.num_verts} nT={m.num_tets}", flush=True)
+    prep = O.prepare(m)
+    t, off, verts, dminv, vol = prepare_host(m)
+    y = m.x0.copy()
+    y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
+    d = DeviceMesh().init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, off, verts, inertia=y)
+    info = d.info()
+    assert info.num_verts == 129 ** 3 and info.num_tets == 6 * 128 ** 3
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    d.solve_iteration(sdt, m.mu, m.lam)
+    got = d.positions()
+    rng = np.random.default_rng(0)
+    p = oracle_params(m, 1, accel=False)
+    worst = 0.0
+    for c in range(prep.n_colors):
+        vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
+        vs = vs[m.fixed[vs] == 0]
+        sample = np.sort(rng.choice(vs, size=min(2000, vs.size), replace=False)).astype(np.int32)
+        xin = got.copy()
+        later = prep.color_verts[prep.color_off[c]:]
+        xin[later] = m.x0[later]
+        xo, _ = O.solve_subset(prep, p, xin, y, sample)
+        disp = np.abs(xo - m.x0[sample]).max()
+        err = np.abs(got[sample].astype(np.float64) - xo).max()
+        worst = max(worst, err / max(disp, 1e-30))
+        print(f"cube128 color {c}: |dx| {disp:.3e} err {err:.3e}", flush=True)
+    assert worst <= 1e-4
+
+
+def test_sphere_with_contact_matches_oracle():
+    """Voxel sphere (shuffled ids, irregular valence) dropped on a penalty plane, 24^3 grid."""
+    m = meshgen.voxel_sphere(24, seed=0)
+    contact = (2, -1.2, 1e4)
+    prep = O.prepare(m)
+    frames = 20  # contact starts near frame 12
+    _, _, of, _ = O.simulate(prep, oracle_params(m, 30, contact=contact), m.x0, None, frames)
+    got, _ = run_gpu_frames(m, params_for(m, 30, contact=contact), frames)
+    rel = max(relmax(got[f], of[f]) for f in range(frames))
+    assert of[-1][:, 2].min() < -1.19  # contact is active by the last frame
+    print(f"sphere24 contact: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_uniform_per_tet_material_equals_scalar_bitwise():
+    m = meshgen.beam(10, 3, 3)
+    a, _ = run_gpu_frames(m, params_for(m, 15), 3)
+    m2 = meshgen.beam(10, 3, 3)
+    m2.tet_mu = np.full(m2.num_tets, m2.mu, np.float32)
+    m2.tet_lambda = np.full(m2.num_tets, m2.lam, np.float32)
+    b, _ = run_gpu_frames(m2, params_for(m2, 15), 3)
+    assert np.array_equal(a, b)
+
+
+def test_batch_equals_per_mesh_runs_bitwise():
+    ms = meshgen.batch_beams(12, seed=5)
+    params = VBDParams(dt=H, numIterations=20, useAcceleration=True, accelerationRho=0.95, gravity=(0, -9.8, 0))
+    descs = []
+    singles = []
+    for m in ms:
+        t, off, verts, dminv, vol = prepare_host(m)
+        kw = dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                  rest_volume=vol, color_off=off, color_verts=verts, tet_mu=t.tetMu, tet_lambda=t.tetLambda)
+        descs.append(kw)
+        d = DeviceMesh().init(**kw)
+        d.set_params(params)
+        d.set_launch_config(8, 256)
+        d.step(4)
+        singles.append(d.positions())
+    b = DeviceMesh().init_batch(descs)
+    b.set_params(params)
+    b.set_launch_config(8, 256)
+    b.step(4)
+    got = b.positions()
+    ref = np.concatenate(singles)
+    assert np.array_equal(got, ref)
+    # and the batch agrees with the oracle on one of its meshes
+    m = ms[3]
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=20, rho=0.95), m.x0, None, 4)
+    o = sum(x.num_verts for x in ms[:3])
+    assert relmax(got[o:o + m.num_verts], of[-1]) <= TOL
+
+
+def test_device_energy_matches_oracle():
+    m = meshgen.noisy_cube(10, seed=1)
+    prep = O.prepare(m)
+    d = upload(m)
+    e = d.total_energy(m.mu, m.lam)
+    eo = O.total_energy(prep, O.make_params(mu=m.mu, lam=m.lam), m.x0)
+    assert np.isclose(e, eo, rtol=1e-4), (e, eo)
+
+
+def test_solve_before_init_is_a_state_error():
+    from vbd_tetmesh_amd import _capi as A
+    d = DeviceMesh()
+    with pytest.raises(A.VBDError) as ei:
+        d.solve_iteration(1.0, 1.0, 1.0)
+    assert ei.value.code == A.VBD_ERR_STATE
+
+
+def test_invalid_coloring_is_rejected():
+    from vbd_tetmesh_amd import _capi as A
+    m = meshgen.beam(3, 1, 1)
+    t, off, verts, dminv, vol = prepare_host(m)
+    bad_off = np.array([0, m.num_verts], np.int32)  # everything in one color
+    with pytest.raises(A.VBDError) as ei:
+        DeviceMesh().init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, bad_off,
+                          np.arange(m.num_verts, dtype=np.int32))
+    assert ei.value.code == A.VBD_ERR_ARG

@@ -1,0 +1,929 @@
+// C ABI of the VBD solver (declared in include/vbd_capi.h): handle lifecycle, upload,
+// VBD_GPU_Solver-style solves, the device time step captured in a hipGraph, launch
+// configuration / autotuning / timing, and the host-side mesh preparation helpers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vbd_capi.h"
+#include "vbd_internal.h"
+
+using namespace vbd;
+
+struct vbd_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool inited = false;
+    DevMesh m;
+    vbd_step_params params{};
+    bool paramsSet = false;
+    int lpv = 8;
+    int block = 256;
+    bool graphEnabled = true;
+    hipGraphExec_t graphExec = nullptr;
+    bool graphValid = false;
+    double* d_energy = nullptr;
+    std::vector<int32_t> meshVertOffsets;  // batched uploads: per-mesh vertex offsets
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(vbd_solver* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(vbd_solver* h, const char* what, hipError_t e) {
+    return fail(h, VBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_CHECK(h, expr)                                   \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail((h), #expr, _e); \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+void drop_graph(vbd_solver* h) {
+    if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
+    h->graphExec = nullptr;
+    h->graphValid = false;
+}
+
+// VBDsolver::getAcceleratorOmega (VBD.cpp:302-311): float arguments, double literals, so the
+// quotients are formed in double and rounded to float on return.
+float omega_ref(int order, float rho, float prev) {
+    switch (order) {
+        case 1: return 1.0f;
+        case 2: return (float)(2.0 / (2.0 - (double)(rho * rho)));
+        default: return (float)(4.0 / (4.0 - (double)((rho * rho) * prev)));
+    }
+}
+
+// 1 / h^2 as VBDsolver::solve forms it (VBD.cpp:116-117)
+float inv_h2_ref(float h) { return (float)(1.0 / (double)(h * h)); }
+
+ContactArgs contact_of(const vbd_step_params& p) {
+    ContactArgs c;
+    c.on = p.contact_on;
+    c.axis = p.contact_axis;
+    c.k = p.contact_k;
+    c.plane = p.contact_plane;
+    return c;
+}
+
+bool finite_array(const float* a, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+// One Gauss-Seidel sweep over all colors.
+hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const ContactArgs& ct, f4* save) {
+    for (int c = 0; c < h->m.nColors; ++c) {
+        hipError_t e = launch_color(h->m, c, h->lpv, h->block, sdt, mu, lambda, ct, save, h->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Iteration loop of VBDsolver::step (VBD.cpp:75-96) / VBD_GPU_Solver::solveVBD (.cu:338-361):
+// the color kernels save each vertex's start-of-sweep position into save[it & 1]; after the
+// sweep, omega > 1 blends against save[(it + 1) & 1], the start of the previous sweep, which
+// is the prevPrevPos of the reference (the two buffers alternate instead of being copied).
+hipError_t enqueue_iterations(vbd_solver* h, int iters, float sdt, float mu, float lambda, bool accel, float rho,
+                              const ContactArgs& ct) {
+    float omega = 1.0f;
+    for (int it = 0; it < iters; ++it) {
+        f4* save = accel ? h->m.save[it & 1] : nullptr;
+        hipError_t e = enqueue_sweep(h, sdt, mu, lambda, ct, save);
+        if (e != hipSuccess) return e;
+        if (accel) {
+            omega = omega_ref(it + 1, rho, omega);
+            if (omega > 1.0f) {
+                e = launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream);
+                if (e != hipSuccess) return e;
+            }
+        }
+    }
+    return hipSuccess;
+}
+
+// One VBDsolver::step (VBD.cpp:60-101): substeps of forwardStep, iterations, updateVelocity.
+hipError_t enqueue_step(vbd_solver* h) {
+    const vbd_step_params& p = h->params;
+    const float hstep = p.dt / (float)p.substeps;
+    const float sdt = inv_h2_ref(hstep);
+    const ContactArgs ct = contact_of(p);
+    for (int sub = 0; sub < p.substeps; ++sub) {
+        hipError_t e = launch_predict(h->m, hstep, p.gravity, h->stream);
+        if (e != hipSuccess) return e;
+        if (p.start_from_inertia) {
+            e = launch_copy_inertia(h->m, h->stream);
+            if (e != hipSuccess) return e;
+        }
+        e = enqueue_iterations(h, p.num_iterations, sdt, p.mu, p.lambda, p.use_acceleration != 0, p.acceleration_rho,
+                               ct);
+        if (e != hipSuccess) return e;
+        e = launch_velocity(h->m, hstep, p.velocity_damping, h->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int check_ready(vbd_solver* h) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (!h->inited) return fail(h, VBD_ERR_STATE, "vbd_init has not been called on this handle");
+    return VBD_OK;
+}
+
+int upload_vertex_array(vbd_solver* h, const float* src, f4* dst) {
+    const size_t n = (size_t)h->m.nV;
+    float* tmp = nullptr;
+    HIP_CHECK(h, hipMalloc(&tmp, sizeof(float) * 3 * (n ? n : 1)));
+    hipError_t e = hipMemcpyAsync(tmp, src, sizeof(float) * 3 * n, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_scatter_pos((int)n, tmp, h->m.inv, dst, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(h, "upload", e);
+    return VBD_OK;
+}
+
+int download_vertex_array(vbd_solver* h, const f4* src, float* dst) {
+    const size_t n = (size_t)h->m.nV;
+    float* tmp = nullptr;
+    HIP_CHECK(h, hipMalloc(&tmp, sizeof(float) * 3 * (n ? n : 1)));
+    hipError_t e = launch_gather_pos((int)n, src, h->m.inv, tmp, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, tmp, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(h, "download", e);
+    return VBD_OK;
+}
+
+// Valence-histogram key of the autotuner cache: free-vertex count bucket (log2), mean
+// valence, and the share of vertices above 16 tets, quantised.
+std::string tune_key(const DevMesh& m, bool contact) {
+    double sum = 0, n = 0, big = 0;
+    for (size_t d = 0; d < m.valenceHist.size(); ++d) {
+        sum += (double)d * m.valenceHist[d];
+        n += m.valenceHist[d];
+        if (d > 16) big += m.valenceHist[d];
+    }
+    const double mean = n > 0 ? sum / n : 0;
+    char buf[128];
+    snprintf(buf, sizeof(buf), "n%d_c%d_mean%d_big%d_mat%d_ct%d", (int)std::log2(std::max(1.0, n)),
+             m.nColors, (int)std::lround(mean * 2), (int)std::lround(n > 0 ? 10 * big / n : 0), (int)m.perTetMat,
+             (int)contact);
+    return buf;
+}
+
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune_cache;
+
+// Launch configuration picked from the valence histogram alone: the smallest lane count
+// that covers the mean valence in at most three strides.
+int heuristic_lpv(const DevMesh& m) {
+    double sum = 0, n = 0;
+    for (size_t d = 0; d < m.valenceHist.size(); ++d) {
+        sum += (double)d * m.valenceHist[d];
+        n += m.valenceHist[d];
+    }
+    const double mean = n > 0 ? sum / n : 8;
+    int l = 4;
+    while (l < 32 && l * 3 < mean) l *= 2;
+    return l;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vbd_capi_version(void) { return VBD_CAPI_VERSION; }
+
+const char* vbd_global_last_error(void) { return g_err.c_str(); }
+
+const char* vbd_last_error(const vbd_solver* h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+int vbd_create(vbd_solver** out, int32_t device) {
+    if (!out) return fail(nullptr, VBD_ERR_ARG, "out is null");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(nullptr, "hipGetDeviceCount", e);
+    if (device < 0 || device >= n) return fail(nullptr, VBD_ERR_ARG, "device index out of range");
+    vbd_solver* h = new (std::nothrow) vbd_solver();
+    if (!h) return fail(nullptr, VBD_ERR_ALLOC, "host allocation failed");
+    h->device = device;
+    DeviceGuard g(device);
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(nullptr, "hipStreamCreateWithFlags", e);
+    }
+    e = hipMalloc(&h->d_energy, sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return hip_fail(nullptr, "hipMalloc", e);
+    }
+    *out = h;
+    return VBD_OK;
+}
+
+void vbd_destroy(vbd_solver* h) {
+    if (!h) return;
+    DeviceGuard g(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    free_device_mesh(h->m);
+    if (h->d_energy) (void)hipFree(h->d_energy);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (!d) return fail(h, VBD_ERR_ARG, "mesh is null");
+    const int nV = d->num_verts, nT = d->num_tets, nC = d->num_colors;
+    if (nV < 0 || nT < 0 || nC < 0) return fail(h, VBD_ERR_ARG, "negative size");
+    if ((int64_t)nT * 4 >= (int64_t)1 << 31) return fail(h, VBD_ERR_ARG, "4*num_tets must fit int32");
+    if ((nV > 0 && (!d->positions || !d->mass || !d->is_fixed)) ||
+        (nT > 0 && (!d->tets || !d->dminv || !d->rest_volume)) || !d->color_offsets || (nV > 0 && !d->color_verts))
+        return fail(h, VBD_ERR_ARG, "required mesh array is null");
+    if ((d->tet_mu == nullptr) != (d->tet_lambda == nullptr))
+        return fail(h, VBD_ERR_ARG, "tet_mu and tet_lambda must both be set or both be null");
+    // host-side validation of the index arrays
+    for (int64_t i = 0; i < (int64_t)4 * nT; ++i)
+        if (d->tets[i] < 0 || d->tets[i] >= nV) return fail(h, VBD_ERR_ARG, "tet vertex index out of range");
+    for (int t = 0; t < nT; ++t) {
+        const int32_t* v = d->tets + 4 * (size_t)t;
+        if (v[0] == v[1] || v[0] == v[2] || v[0] == v[3] || v[1] == v[2] || v[1] == v[3] || v[2] == v[3])
+            return fail(h, VBD_ERR_ARG, "tet with repeated vertex");
+    }
+    if (d->color_offsets[0] != 0 || d->color_offsets[nC] != nV)
+        return fail(h, VBD_ERR_ARG, "color offsets must start at 0 and end at num_verts");
+    std::vector<int32_t> colorOf(nV > 0 ? nV : 1, -1);
+    for (int c = 0; c < nC; ++c) {
+        if (d->color_offsets[c + 1] < d->color_offsets[c]) return fail(h, VBD_ERR_ARG, "color offsets decrease");
+        for (int i = d->color_offsets[c]; i < d->color_offsets[c + 1]; ++i) {
+            const int v = d->color_verts[i];
+            if (v < 0 || v >= nV || colorOf[v] != -1)
+                return fail(h, VBD_ERR_ARG, "color lists must hold every vertex exactly once");
+            colorOf[v] = c;
+        }
+    }
+    // a valid coloring: no two corners of a tet share a color (same-color vertices must be
+    // independent for the within-color parallel solve to equal the sequential sweep)
+    for (int t = 0; t < nT; ++t) {
+        const int32_t* v = d->tets + 4 * (size_t)t;
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b)
+                if (colorOf[v[a]] == colorOf[v[b]])
+                    return fail(h, VBD_ERR_ARG, "invalid coloring: two corners of a tet share a color");
+    }
+    if (!finite_array(d->positions, 3 * (size_t)nV) || (d->inertia && !finite_array(d->inertia, 3 * (size_t)nV)) ||
+        (d->velocity && !finite_array(d->velocity, 3 * (size_t)nV)) || !finite_array(d->mass, nV) ||
+        !finite_array(d->dminv, 9 * (size_t)nT) || !finite_array(d->rest_volume, nT))
+        return fail(h, VBD_ERR_NUMERIC, "non-finite value in mesh input");
+
+    DeviceGuard g(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    free_device_mesh(h->m);
+    h->inited = false;
+
+    // stage host arrays on the device, build, free staging
+    struct Stage {
+        void* p = nullptr;
+        ~Stage() { if (p) (void)hipFree(p); }
+    };
+    Stage s_tets, s_col, s_fix, s_dm, s_vol, s_mu, s_lam, s_x, s_y, s_v, s_m;
+    auto up = [&](Stage& st, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(&st.p, bytes ? bytes : 4);
+        if (e != hipSuccess) return e;
+        if (bytes) return hipMemcpyAsync(st.p, src, bytes, hipMemcpyHostToDevice, h->stream);
+        return hipSuccess;
+    };
+    HIP_CHECK(h, up(s_tets, d->tets, sizeof(int32_t) * 4 * (size_t)nT));
+    HIP_CHECK(h, up(s_col, colorOf.data(), sizeof(int32_t) * (size_t)nV));
+    HIP_CHECK(h, up(s_fix, d->is_fixed, (size_t)nV));
+    HIP_CHECK(h, up(s_dm, d->dminv, sizeof(float) * 9 * (size_t)nT));
+    HIP_CHECK(h, up(s_vol, d->rest_volume, sizeof(float) * (size_t)nT));
+    if (d->tet_mu) {
+        HIP_CHECK(h, up(s_mu, d->tet_mu, sizeof(float) * (size_t)nT));
+        HIP_CHECK(h, up(s_lam, d->tet_lambda, sizeof(float) * (size_t)nT));
+    }
+    HIP_CHECK(h, up(s_x, d->positions, sizeof(float) * 3 * (size_t)nV));
+    if (d->inertia) HIP_CHECK(h, up(s_y, d->inertia, sizeof(float) * 3 * (size_t)nV));
+    if (d->velocity) HIP_CHECK(h, up(s_v, d->velocity, sizeof(float) * 3 * (size_t)nV));
+    HIP_CHECK(h, up(s_m, d->mass, sizeof(float) * (size_t)nV));
+    std::string err;
+    hipError_t e = build_device_mesh(h->m, nV, nT, nC, (const i4*)s_tets.p, (const int*)s_col.p,
+                                     (const uint8_t*)s_fix.p, (const float*)s_dm.p, (const float*)s_vol.p,
+                                     (const float*)s_mu.p, (const float*)s_lam.p, (const float*)s_x.p,
+                                     (const float*)s_y.p, (const float*)s_v.p, (const float*)s_m.p, h->stream, err);
+    if (e != hipSuccess) return fail(h, e == hipErrorOutOfMemory ? VBD_ERR_ALLOC : VBD_ERR_HIP, "mesh build: " + err);
+    h->lpv = heuristic_lpv(h->m);
+    h->block = 256;
+    h->meshVertOffsets.assign({0, nV});
+    h->inited = true;
+    return VBD_OK;
+}
+
+int vbd_init_batch(vbd_solver* h, int32_t num_meshes, const vbd_mesh_desc* meshes) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (num_meshes <= 0 || !meshes) return fail(h, VBD_ERR_ARG, "no meshes");
+    const bool mat = meshes[0].tet_mu != nullptr;
+    int64_t nV = 0, nT = 0;
+    int nC = 0;
+    for (int i = 0; i < num_meshes; ++i) {
+        const vbd_mesh_desc& d = meshes[i];
+        if (d.num_verts < 0 || d.num_tets < 0 || d.num_colors < 0) return fail(h, VBD_ERR_ARG, "negative size");
+        if ((d.tet_mu != nullptr) != mat || (d.tet_lambda != nullptr) != mat)
+            return fail(h, VBD_ERR_ARG, "per-tet materials must be given for all meshes or none");
+        if (!d.color_offsets) return fail(h, VBD_ERR_ARG, "color offsets null");
+        nV += d.num_verts;
+        nT += d.num_tets;
+        nC = std::max(nC, (int)d.num_colors);
+    }
+    if (nV >= ((int64_t)1 << 31) || 4 * nT >= ((int64_t)1 << 31)) return fail(h, VBD_ERR_ARG, "batch too large");
+    std::vector<float> x(3 * nV), y, v, mass(nV), dm(9 * nT), vol(nT), mu, lam;
+    std::vector<uint8_t> fixed(nV);
+    std::vector<int32_t> tets(4 * nT), coff(nC + 1, 0), cverts;
+    bool anyY = false, anyV = false;
+    for (int i = 0; i < num_meshes; ++i) {
+        anyY |= meshes[i].inertia != nullptr;
+        anyV |= meshes[i].velocity != nullptr;
+    }
+    if (anyY) y.resize(3 * nV);
+    if (anyV) v.assign(3 * nV, 0.f);
+    if (mat) {
+        mu.resize(nT);
+        lam.resize(nT);
+    }
+    std::vector<int32_t> voff(num_meshes + 1, 0);
+    int64_t vo = 0, to = 0;
+    for (int i = 0; i < num_meshes; ++i) {
+        const vbd_mesh_desc& d = meshes[i];
+        const size_t nv = d.num_verts, nt = d.num_tets;
+        if ((nv && (!d.positions || !d.mass || !d.is_fixed)) || (nt && (!d.tets || !d.dminv || !d.rest_volume)))
+            return fail(h, VBD_ERR_ARG, "required mesh array is null");
+        std::memcpy(&x[3 * vo], d.positions, sizeof(float) * 3 * nv);
+        if (anyY) std::memcpy(&y[3 * vo], d.inertia ? d.inertia : d.positions, sizeof(float) * 3 * nv);
+        if (anyV && d.velocity) std::memcpy(&v[3 * vo], d.velocity, sizeof(float) * 3 * nv);
+        std::memcpy(&mass[vo], d.mass, sizeof(float) * nv);
+        std::memcpy(&fixed[vo], d.is_fixed, nv);
+        for (size_t k = 0; k < 4 * nt; ++k) {
+            if (d.tets[k] < 0 || d.tets[k] >= (int)nv) return fail(h, VBD_ERR_ARG, "tet vertex index out of range");
+            tets[4 * to + k] = d.tets[k] + (int32_t)vo;
+        }
+        std::memcpy(&dm[9 * to], d.dminv, sizeof(float) * 9 * nt);
+        std::memcpy(&vol[to], d.rest_volume, sizeof(float) * nt);
+        if (mat) {
+            std::memcpy(&mu[to], d.tet_mu, sizeof(float) * nt);
+            std::memcpy(&lam[to], d.tet_lambda, sizeof(float) * nt);
+        }
+        if (d.color_offsets[d.num_colors] != (int)nv) return fail(h, VBD_ERR_ARG, "color offsets do not end at num_verts");
+        vo += nv;
+        to += nt;
+        voff[i + 1] = (int32_t)vo;
+    }
+    // colors merged by index: merged color c = union over meshes of each mesh's color c
+    cverts.reserve(nV);
+    for (int c = 0; c < nC; ++c) {
+        for (int i = 0; i < num_meshes; ++i) {
+            const vbd_mesh_desc& d = meshes[i];
+            if (c >= d.num_colors) continue;
+            for (int k = d.color_offsets[c]; k < d.color_offsets[c + 1]; ++k) cverts.push_back(d.color_verts[k] + voff[i]);
+        }
+        coff[c + 1] = (int32_t)cverts.size();
+    }
+    vbd_mesh_desc u{};
+    u.num_verts = (int32_t)nV;
+    u.num_tets = (int32_t)nT;
+    u.positions = x.data();
+    u.inertia = anyY ? y.data() : nullptr;
+    u.velocity = anyV ? v.data() : nullptr;
+    u.mass = mass.data();
+    u.is_fixed = fixed.data();
+    u.tets = tets.data();
+    u.dminv = dm.data();
+    u.rest_volume = vol.data();
+    u.tet_mu = mat ? mu.data() : nullptr;
+    u.tet_lambda = mat ? lam.data() : nullptr;
+    u.num_colors = nC;
+    u.color_offsets = coff.data();
+    u.color_verts = cverts.data();
+    int rc = vbd_init(h, &u);
+    if (rc == VBD_OK) h->meshVertOffsets = voff;
+    return rc;
+}
+
+int vbd_get_info(const vbd_solver* h, vbd_info* info) {
+    if (!h || !info) return fail(nullptr, VBD_ERR_ARG, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    if (!h->inited) return VBD_OK;
+    info->num_verts = h->m.nV;
+    info->num_tets = h->m.nT;
+    info->num_free = h->m.nFree;
+    info->num_colors = h->m.nColors;
+    info->num_pairs = h->m.nPairs;
+    info->max_valence = h->m.maxValence;
+    info->per_tet_material = h->m.perTetMat;
+    info->lanes_per_vertex = h->lpv;
+    info->block_size = h->block;
+    info->device_bytes = h->m.bytes;
+    return VBD_OK;
+}
+
+int vbd_upload_positions(vbd_solver* h, const float* x) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!x && h->m.nV) return fail(h, VBD_ERR_ARG, "null positions");
+    DeviceGuard g(h->device);
+    return upload_vertex_array(h, x, h->m.pos);
+}
+
+int vbd_upload_inertia(vbd_solver* h, const float* y) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!y && h->m.nV) return fail(h, VBD_ERR_ARG, "null inertia");
+    DeviceGuard g(h->device);
+    return upload_vertex_array(h, y, h->m.inert);
+}
+
+int vbd_upload_velocities(vbd_solver* h, const float* v) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!v && h->m.nV) return fail(h, VBD_ERR_ARG, "null velocities");
+    DeviceGuard g(h->device);
+    return upload_vertex_array(h, v, h->m.vel);
+}
+
+int vbd_download_positions(vbd_solver* h, float* x) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!x && h->m.nV) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    return download_vertex_array(h, h->m.pos, x);
+}
+
+int vbd_download_velocities(vbd_solver* h, float* v) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!v && h->m.nV) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    return download_vertex_array(h, h->m.vel, v);
+}
+
+int vbd_download_inertia(vbd_solver* h, float* y) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!y && h->m.nV) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    return download_vertex_array(h, h->m.inert, y);
+}
+
+int vbd_get_adjacency(vbd_solver* h, int32_t* offsets, int32_t* entries) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!offsets || (!entries && h->m.nT)) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, hipMemcpyAsync(offsets, h->m.adjOff, sizeof(int32_t) * (h->m.nV + 1), hipMemcpyDeviceToHost, h->stream));
+    if (h->m.nT)
+        HIP_CHECK(h, hipMemcpyAsync(entries, h->m.adjEntry, sizeof(int32_t) * 4 * (size_t)h->m.nT,
+                                    hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_get_color_lists(vbd_solver* h, int32_t* offsets, int32_t* verts) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!offsets || (!verts && h->m.nV)) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    for (int c = 0; c <= h->m.nColors; ++c) offsets[c] = h->m.colorFullOff[c];
+    if (h->m.nV)
+        HIP_CHECK(h, hipMemcpyAsync(verts, h->m.colorList, sizeof(int32_t) * h->m.nV, hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_solve_iteration(vbd_solver* h, float dt_sqr_reciprocal, float mu, float lambda) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    ContactArgs ct{};
+    if (h->paramsSet) ct = contact_of(h->params);
+    HIP_CHECK(h, enqueue_sweep(h, dt_sqr_reciprocal, mu, lambda, ct, nullptr));
+    return VBD_OK;
+}
+
+int vbd_solve(vbd_solver* h, int32_t num_iterations, float dt_sqr_reciprocal, float mu, float lambda,
+              int32_t use_acceleration, float rho, int32_t start_from_inertia) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (num_iterations < 0) return fail(h, VBD_ERR_ARG, "negative iteration count");
+    DeviceGuard g(h->device);
+    ContactArgs ct{};
+    if (h->paramsSet) ct = contact_of(h->params);
+    if (start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
+    HIP_CHECK(h, enqueue_iterations(h, num_iterations, dt_sqr_reciprocal, mu, lambda, use_acceleration != 0, rho, ct));
+    return VBD_OK;
+}
+
+int vbd_set_params(vbd_solver* h, const vbd_step_params* p) {
+    if (!h || !p) return fail(h, VBD_ERR_ARG, "null argument");
+    if (!(p->dt > 0.f) || p->substeps <= 0 || p->num_iterations < 0)
+        return fail(h, VBD_ERR_ARG, "dt must be > 0, substeps > 0, num_iterations >= 0");
+    if (p->contact_on && (p->contact_axis < 0 || p->contact_axis > 2))
+        return fail(h, VBD_ERR_ARG, "contact axis must be 0, 1 or 2");
+    h->params = *p;
+    h->paramsSet = true;
+    drop_graph(h);
+    return VBD_OK;
+}
+
+int vbd_forward_step(vbd_solver* h) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, launch_predict(h->m, h->params.dt / (float)h->params.substeps, h->params.gravity, h->stream));
+    return VBD_OK;
+}
+
+int vbd_update_velocity(vbd_solver* h) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, launch_velocity(h->m, h->params.dt / (float)h->params.substeps, h->params.velocity_damping, h->stream));
+    return VBD_OK;
+}
+
+int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    h->graphEnabled = enabled != 0;
+    DeviceGuard g(h->device);
+    drop_graph(h);
+    return VBD_OK;
+}
+
+int vbd_step(vbd_solver* h, int32_t num_frames) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    if (num_frames < 0) return fail(h, VBD_ERR_ARG, "negative frame count");
+    DeviceGuard g(h->device);
+    if (!h->graphEnabled) {
+        for (int f = 0; f < num_frames; ++f) HIP_CHECK(h, enqueue_step(h));
+        return VBD_OK;
+    }
+    if (!h->graphValid) {
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_step(h);
+        hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+        if (e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return hip_fail(h, "capture of the time step", e);
+        }
+        if (e2 != hipSuccess) return hip_fail(h, "hipStreamEndCapture", e2);
+        e = hipGraphInstantiate(&h->graphExec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            h->graphExec = nullptr;
+            return hip_fail(h, "hipGraphInstantiate", e);
+        }
+        h->graphValid = true;
+    }
+    for (int f = 0; f < num_frames; ++f) HIP_CHECK(h, hipGraphLaunch(h->graphExec, h->stream));
+    return VBD_OK;
+}
+
+int vbd_synchronize(vbd_solver* h) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    const int l = lanes_per_vertex;
+    if (l != 4 && l != 8 && l != 16 && l != 32 && l != 64) return fail(h, VBD_ERR_ARG, "lanes_per_vertex must be 4..64, power of 2");
+    if (block_size != 64 && block_size != 128 && block_size != 192 && block_size != 256)
+        return fail(h, VBD_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+    if (block_size % l) return fail(h, VBD_ERR_ARG, "block_size must be a multiple of lanes_per_vertex");
+    h->lpv = l;
+    h->block = block_size;
+    DeviceGuard g(h->device);
+    drop_graph(h);
+    return VBD_OK;
+}
+
+int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int32_t* color_launches,
+                        double* blend_ms, int32_t* blend_launches) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (iterations <= 0) return fail(h, VBD_ERR_ARG, "iterations must be positive");
+    DeviceGuard g(h->device);
+    const vbd_step_params p = h->params;
+    const bool accel = h->paramsSet && p.use_acceleration;
+    const float hstep = h->paramsSet ? p.dt / (float)p.substeps : 1.0f / 60.0f;
+    const float sdt = inv_h2_ref(hstep);
+    const float mu = h->paramsSet ? p.mu : 1e5f, lam = h->paramsSet ? p.lambda : 4e5f;
+    const ContactArgs ct = h->paramsSet ? contact_of(p) : ContactArgs{};
+    struct Ev {
+        hipEvent_t a, b;
+        bool blend;
+    };
+    std::vector<Ev> evs;
+    evs.reserve((size_t)iterations * (h->m.nColors + 1));
+    auto mk = [&](bool blend) -> hipError_t {
+        Ev e{};
+        e.blend = blend;
+        hipError_t r = hipEventCreate(&e.a);
+        if (r != hipSuccess) return r;
+        r = hipEventCreate(&e.b);
+        if (r != hipSuccess) { (void)hipEventDestroy(e.a); return r; }
+        evs.push_back(e);
+        return hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    float omega = 1.0f;
+    for (int it = 0; it < iterations && e == hipSuccess; ++it) {
+        f4* save = accel ? h->m.save[it & 1] : nullptr;
+        for (int c = 0; c < h->m.nColors && e == hipSuccess; ++c) {
+            if (h->m.colorSlotOff[c + 1] == h->m.colorSlotOff[c]) continue;
+            if ((e = mk(false)) != hipSuccess) break;
+            (void)hipEventRecord(evs.back().a, h->stream);
+            e = launch_color(h->m, c, h->lpv, h->block, sdt, mu, lam, ct, save, h->stream);
+            (void)hipEventRecord(evs.back().b, h->stream);
+        }
+        if (accel && e == hipSuccess) {
+            omega = omega_ref(it + 1, p.acceleration_rho, omega);
+            if (omega > 1.0f) {
+                if ((e = mk(true)) != hipSuccess) break;
+                (void)hipEventRecord(evs.back().a, h->stream);
+                e = launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream);
+                (void)hipEventRecord(evs.back().b, h->stream);
+            }
+        }
+    }
+    hipError_t es = hipStreamSynchronize(h->stream);
+    double cms = 0, bms = 0;
+    int cn = 0, bn = 0;
+    for (auto& ev : evs) {
+        float ms = 0.f;
+        if (e == hipSuccess && es == hipSuccess) (void)hipEventElapsedTime(&ms, ev.a, ev.b);
+        if (ev.blend) { bms += ms; ++bn; } else { cms += ms; ++cn; }
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    if (e != hipSuccess) return hip_fail(h, "timed launch", e);
+    if (es != hipSuccess) return hip_fail(h, "hipStreamSynchronize", es);
+    if (color_ms) *color_ms = cms;
+    if (color_launches) *color_launches = cn;
+    if (blend_ms) *blend_ms = bms;
+    if (blend_launches) *blend_launches = bn;
+    return VBD_OK;
+}
+
+int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_block, float* best_ms) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (iterations <= 0) return fail(h, VBD_ERR_ARG, "iterations must be positive");
+    DeviceGuard g(h->device);
+    const std::string key = tune_key(h->m, h->paramsSet && h->params.contact_on);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            h->lpv = it->second.first;
+            h->block = it->second.second;
+            drop_graph(h);
+            if (best_lanes) *best_lanes = h->lpv;
+            if (best_block) *best_block = h->block;
+            if (best_ms) *best_ms = -1.f;
+            return VBD_OK;
+        }
+    }
+    // keep the state: timing sweeps move the positions and the Chebyshev buffers
+    f4* keep = nullptr;
+    HIP_CHECK(h, hipMalloc(&keep, sizeof(f4) * (h->m.nV ? h->m.nV : 1)));
+    hipError_t e = launch_copy_f4(h->m.nV, h->m.pos, keep, h->stream);
+    const int cand_l[] = {4, 8, 16, 32};
+    const int cand_b[] = {64, 128, 256};
+    int bl = h->lpv, bb = h->block;
+    double best = 1e30;
+    const int ol = h->lpv, ob = h->block;
+    for (int l : cand_l) {
+        for (int b : cand_b) {
+            if (e != hipSuccess) break;
+            if (b % l) continue;
+            h->lpv = l;
+            h->block = b;
+            double cms = 0, bms = 0;
+            int cn = 0, bn = 0;
+            // one untimed warm sweep, then the measurement
+            rc = vbd_time_iterations(h, 1, &cms, &cn, &bms, &bn);
+            if (rc == VBD_OK) rc = vbd_time_iterations(h, iterations, &cms, &cn, &bms, &bn);
+            if (rc != VBD_OK) {
+                h->lpv = ol;
+                h->block = ob;
+                (void)launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
+                (void)hipStreamSynchronize(h->stream);
+                (void)hipFree(keep);
+                return rc;
+            }
+            if (cms < best) {
+                best = cms;
+                bl = l;
+                bb = b;
+            }
+            e = launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(keep);
+    if (e != hipSuccess) return hip_fail(h, "autotune", e);
+    h->lpv = bl;
+    h->block = bb;
+    drop_graph(h);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = {bl, bb};
+    }
+    if (best_lanes) *best_lanes = bl;
+    if (best_block) *best_block = bb;
+    if (best_ms) *best_ms = (float)(best / iterations);
+    return VBD_OK;
+}
+
+// Algorithmic bytes (fixed yardstick; neighbour-position gathers excluded):
+//   color launches, per solved vertex: x read 12 + x write 12 + y 12 + mass 4 + CSR row 4 +
+//   fixed flag 1 = 45; per (vertex, incident tet): CSR entry 4 + tet 16 + DmInv 36 + V0 4 = 60,
+//   + 8 (mu, lambda) with per-tet materials.  Chebyshev adds 36 per vertex (read x_{n-2},
+//   write x_{n-1}, write blended x), attributed to the blend launches.
+int vbd_algorithmic_bytes(const vbd_solver* h, double* color_bytes, double* blend_bytes) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (!h->inited) return fail(const_cast<vbd_solver*>(h), VBD_ERR_STATE, "not initialised");
+    const double per_pair = 60.0 + (h->m.perTetMat ? 8.0 : 0.0);
+    if (color_bytes) *color_bytes = 45.0 * h->m.nFree + per_pair * (double)h->m.nPairs;
+    if (blend_bytes) *blend_bytes = 36.0 * h->m.nFree;
+    return VBD_OK;
+}
+
+int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!energy) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, launch_energy(h->m, mu, lambda, h->d_energy, h->stream));
+    HIP_CHECK(h, hipMemcpyAsync(energy, h->d_energy, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// host helpers (evaluated without fused multiply-adds, like the reference's scalar code)
+// ---------------------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+// TetMesh::initialize (TetMesh.h:213-275) and computeRestVolume (TetMesh.h:291-298):
+// V = |e0 . (e1 x e2)| / 6, mass += rho V / 4 per corner, adjacency appended in tet order.
+int vbd_mesh_initialize(int32_t nV, const float* rest, int32_t nT, const int32_t* tets, float density, float* mass,
+                        float* tet_rest_volume, int32_t* adj_offsets, int32_t* adj_tets) {
+    if (nV < 0 || nT < 0 || (nV && (!rest || !mass || !adj_offsets)) || (nT && (!tets || !tet_rest_volume || !adj_tets)))
+        return fail(nullptr, VBD_ERR_ARG, "vbd_mesh_initialize: bad argument");
+    if (!adj_offsets) return fail(nullptr, VBD_ERR_ARG, "vbd_mesh_initialize: adj_offsets null");
+    std::vector<int32_t> deg(nV, 0);
+    for (int i = 0; i < nV; ++i) mass[i] = 0.0f;
+    for (int t = 0; t < nT; ++t) {
+        const int32_t* v = tets + 4 * (size_t)t;
+        for (int j = 0; j < 4; ++j)
+            if (v[j] < 0 || v[j] >= nV) return fail(nullptr, VBD_ERR_ARG, "vbd_mesh_initialize: index out of range");
+        float e0[3], e1[3], e2[3];
+        for (int c = 0; c < 3; ++c) {
+            e0[c] = rest[3 * (size_t)v[1] + c] - rest[3 * (size_t)v[0] + c];
+            e1[c] = rest[3 * (size_t)v[2] + c] - rest[3 * (size_t)v[0] + c];
+            e2[c] = rest[3 * (size_t)v[3] + c] - rest[3 * (size_t)v[0] + c];
+        }
+        const float cx = e1[1] * e2[2] - e1[2] * e2[1];
+        const float cy = e1[2] * e2[0] - e1[0] * e2[2];
+        const float cz = e1[0] * e2[1] - e1[1] * e2[0];
+        const float vol = std::fabs(e0[0] * cx + (e0[1] * cy + e0[2] * cz)) / 6.0f;
+        tet_rest_volume[t] = vol;
+        const float tm = vol * density;
+        for (int j = 0; j < 4; ++j) {
+            deg[v[j]]++;
+            mass[v[j]] += tm / 4.0f;
+        }
+    }
+    adj_offsets[0] = 0;
+    for (int i = 0; i < nV; ++i) adj_offsets[i + 1] = adj_offsets[i] + deg[i];
+    std::vector<int32_t> cur(adj_offsets, adj_offsets + nV);
+    for (int t = 0; t < nT; ++t)
+        for (int j = 0; j < 4; ++j) adj_tets[cur[tets[4 * (size_t)t + j]]++] = t;
+    return VBD_OK;
+}
+
+// GraphColoring::colorVertices (GraphColoring.cpp:28-66): greedy first-fit in ascending
+// vertex order over the graph of shared tets; groups list vertices in ascending order.  A
+// per-color stamp array replaces the reference's per-vertex unordered_sets (same result).
+int vbd_color_vertices(int32_t nV, const int32_t* tets, const int32_t* adj_offsets, const int32_t* adj_tets,
+                       int32_t* color_of, int32_t* color_offsets, int32_t* color_verts, int32_t* num_colors) {
+    if (nV < 0 || (nV && (!tets || !adj_offsets || !adj_tets || !color_of || !color_verts)) || !color_offsets ||
+        !num_colors)
+        return fail(nullptr, VBD_ERR_ARG, "vbd_color_vertices: bad argument");
+    std::vector<int32_t> stamp;
+    std::vector<int32_t> count;
+    for (int v = 0; v < nV; ++v) color_of[v] = -1;
+    for (int v = 0; v < nV; ++v) {
+        for (int e = adj_offsets[v]; e < adj_offsets[v + 1]; ++e) {
+            const int32_t* tet = tets + 4 * (size_t)adj_tets[e];
+            for (int i = 0; i < 4; ++i) {
+                const int u = tet[i];
+                if (u == v) continue;
+                const int c = color_of[u];
+                if (c >= 0) stamp[c] = v;
+            }
+        }
+        int c = 0;
+        while (c < (int)stamp.size() && stamp[c] == v) ++c;
+        if (c == (int)stamp.size()) {
+            stamp.push_back(-1);
+            count.push_back(0);
+        }
+        color_of[v] = c;
+        count[c]++;
+    }
+    const int nc = (int)count.size();
+    color_offsets[0] = 0;
+    for (int c = 0; c < nc; ++c) color_offsets[c + 1] = color_offsets[c] + count[c];
+    std::vector<int32_t> cur(color_offsets, color_offsets + nc);
+    for (int v = 0; v < nV; ++v) color_verts[cur[color_of[v]]++] = v;
+    *num_colors = nc;
+    return VBD_OK;
+}
+
+// VBDsolver::precomputeTetData (VBD.cpp:36-58): Dm = [x1-x0, x2-x0, x3-x0] at rest,
+// DmInv = adj(Dm)/det(Dm) (cofactor column 0 dotted with column 0 for det), V0 =
+// |d3 . (d1 x d2)| / 6 (Energy.cpp:24-32).  DmInv is written row-major.
+int vbd_precompute_tet_data(const float* rest, int32_t nT, const int32_t* tets, float* dminv, float* rest_volume) {
+    if (nT < 0 || (nT && (!rest || !tets || !dminv || !rest_volume)))
+        return fail(nullptr, VBD_ERR_ARG, "vbd_precompute_tet_data: bad argument");
+    for (int t = 0; t < nT; ++t) {
+        const int32_t* v = tets + 4 * (size_t)t;
+        float A[3][3];
+        for (int j = 0; j < 3; ++j)
+            for (int c = 0; c < 3; ++c) A[c][j] = rest[3 * (size_t)v[j + 1] + c] - rest[3 * (size_t)v[0] + c];
+        auto cof = [&](int i, int j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            return A[i1][j1] * A[i2][j2] - A[i1][j2] * A[i2][j1];
+        };
+        const float c00 = cof(0, 0), c10 = cof(1, 0), c20 = cof(2, 0);
+        const float det = c00 * A[0][0] + (c10 * A[1][0] + c20 * A[2][0]);
+        const float inv = 1.0f / det;
+        float* D = dminv + 9 * (size_t)t;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) D[3 * j + i] = cof(i, j) * inv;
+        const float d1[3] = {A[0][0], A[1][0], A[2][0]}, d2[3] = {A[0][1], A[1][1], A[2][1]},
+                    d3[3] = {A[0][2], A[1][2], A[2][2]};
+        const float cx = d1[1] * d2[2] - d1[2] * d2[1];
+        const float cy = d1[2] * d2[0] - d1[0] * d2[2];
+        const float cz = d1[0] * d2[1] - d1[1] * d2[0];
+        rest_volume[t] = std::fabs(d3[0] * cx + (d3[1] * cy + d3[2] * cz)) / 6.0f;
+    }
+    return VBD_OK;
+}
+
+float vbd_accelerator_omega(int32_t order, float rho, float prev_omega) { return omega_ref(order, rho, prev_omega); }
+
+}  // extern "C"

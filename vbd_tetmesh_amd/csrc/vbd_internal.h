@@ -1,0 +1,87 @@
+// Internal declarations shared by the device code (vbd_kernels.hip) and the C ABI
+// implementation (vbd_capi.hip).  Not installed; callers use include/vbd_capi.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "vbd_capi.h"
+
+namespace vbd {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+// Device image of one uploaded mesh (or batch union).  Vertex arrays are in "slot" order:
+// free vertices color-major (ascending caller id inside a color), then pinned vertices.
+// Per-(vertex, tet) "pair" records are laid out in the same order, so one color launch
+// streams one contiguous range of every pair array.
+struct DevMesh {
+    int nV = 0, nT = 0, nFree = 0, nColors = 0;
+    int64_t nPairs = 0;
+    int maxValence = 0;
+    bool perTetMat = false;
+    std::vector<int> colorSlotOff;   // host copy, nColors+1: slot range of each color (free only)
+    std::vector<int> colorFullOff;   // host copy, nColors+1: offsets of the full color lists
+    std::vector<int> valenceHist;    // host copy: number of free vertices per valence
+
+    f4* pos = nullptr;      // nV   current positions (w unused)
+    f4* inert = nullptr;    // nV   inertia target y
+    f4* vel = nullptr;      // nV   velocity
+    f4* xprev = nullptr;    // nV   position at the start of the step
+    f4* save[2] = {nullptr, nullptr};  // nV  x at the start of an iteration (Chebyshev ping-pong)
+    float* mass = nullptr;  // nV
+    int* row = nullptr;     // nFree+1 pair offsets
+    int* perm = nullptr;    // nV slot -> caller vertex id
+    int* inv = nullptr;     // nV caller vertex id -> slot
+    i4* ptet = nullptr;     // nPairs  the tet's 4 corner slots in tet corner order
+    f4* pa = nullptr;       // nPairs  DmInv(0,0) (0,1) (0,2) (1,0)
+    f4* pb = nullptr;       // nPairs  DmInv(1,1) (1,2) (2,0) (2,1)
+    f2* pc = nullptr;       // nPairs  DmInv(2,2), V0
+    f2* pmat = nullptr;     // nPairs  mu, lambda (per-tet materials only)
+    // GPU-built structures kept in the caller's numbering (exposed for verification)
+    int* adjOff = nullptr;    // nV+1
+    int* adjEntry = nullptr;  // 4*nT, tet*4 + corner
+    int* colorList = nullptr; // nV
+    // per-tet arrays kept for the energy reduction
+    i4* tets = nullptr;     // nT (caller numbering)
+    float* dminv = nullptr; // 9*nT
+    float* vol = nullptr;   // nT
+    float* tmu = nullptr;   // nT or null
+    float* tlam = nullptr;  // nT or null
+    int64_t bytes = 0;
+};
+
+struct ContactArgs {
+    int on;
+    int axis;
+    float k;
+    float plane;
+};
+
+// ---- launchers (vbd_kernels.hip); all return hipError_t of the launch -----------------
+hipError_t launch_color(const DevMesh& m, int color, int lpv, int block, float sdt, float mu, float lambda,
+                        const ContactArgs& ct, f4* save, hipStream_t s);
+hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
+hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s);
+hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t s);
+hipError_t launch_copy_inertia(const DevMesh& m, hipStream_t s);
+hipError_t launch_scatter_pos(int n, const float* src3, const int* inv, f4* dst, hipStream_t s);
+hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, hipStream_t s);
+hipError_t launch_energy(const DevMesh& m, float mu, float lambda, double* out, hipStream_t s);
+hipError_t launch_copy_f4(int n, const f4* src, f4* dst, hipStream_t s);
+
+// Device mesh build.  Host inputs are already on the device (d_* pointers, caller numbering).
+// colorOf: color per vertex (host coloring); fixed: pinned flags.  Allocates every DevMesh
+// array; returns a HIP error or hipSuccess, message in err.
+hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* d_tets, const int* d_colorOf,
+                             const uint8_t* d_fixed, const float* d_dminv, const float* d_vol, const float* d_mu,
+                             const float* d_lam, const float* d_x3, const float* d_y3, const float* d_v3,
+                             const float* d_mass, hipStream_t s, std::string& err);
+void free_device_mesh(DevMesh& m);
+
+}  // namespace vbd

@@ -1,0 +1,819 @@
+// gfx950 device code of the VBD solver: mesh build (CSR, color lists, color-major
+// renumbering, pair records), the per-color vertex solve, Chebyshev blend, time-step
+// kernels and the diagnostic energy reduction.
+//
+// Algorithm references (extinguish7/VBD_TetMesh):
+//   per-vertex solve ............ VBD.cpp:160-203 (inertia + incident tets), :141-153 (3x3 solve)
+//   per-(tet, corner) block ..... Energy.cpp:90-158 in closed form (see block comment below)
+//   3x3 solve ................... Types.h:85-112
+//   Chebyshev blend ............. VBD.cpp:290-300
+//   predict / velocity .......... TetMesh.h:305-348
+//   adjacency / colors .......... TetMesh.h:254-266, GraphColoring.cpp:28-66, VBD_GPU_Wrapper.cu:286-308
+#include "vbd_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace vbd {
+
+// ---------------------------------------------------------------------------------------
+// Wave-level sums with DPP.  gsum<L> sums each aligned group of L lanes of a wave64; the
+// total is valid in the group's last lane (lane % L == L-1).  row_shr steps build suffix
+// sums inside a 16-lane row; row_bcast15 / row_bcast31 carry row totals across rows.
+// Lanes whose DPP source is outside the row (or whose row is masked off) receive 0.
+// ---------------------------------------------------------------------------------------
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_take(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+}
+
+template <int L>
+__device__ __forceinline__ float gsum(float v) {
+    if constexpr (L >= 2) v += dpp_take<0x111, 0xF>(v);  // row_shr:1
+    if constexpr (L >= 4) v += dpp_take<0x112, 0xF>(v);  // row_shr:2
+    if constexpr (L >= 8) v += dpp_take<0x114, 0xF>(v);  // row_shr:4
+    if constexpr (L >= 16) v += dpp_take<0x118, 0xF>(v); // row_shr:8
+    if constexpr (L >= 32) v += dpp_take<0x142, 0xA>(v); // row_bcast:15 into rows 1, 3
+    if constexpr (L >= 64) v += dpp_take<0x143, 0xC>(v); // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// Blocks of a launch are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8).  The
+// remap hands each XCD one contiguous run of logical blocks, so neighbouring vertices (and
+// the neighbour positions they gather) share one XCD's L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-color vertex solve.
+//
+// One group of LPV lanes per free vertex of the color.  Lanes stride over the vertex's
+// incident-tet records; each record gives the tet's four corner slots (corner order of the
+// tet), DmInv (row-major) and V0.  Per record, with q = shape vector of the vertex's corner
+// (q_0 = -(row0+row1+row2) of DmInv, q_k = row k-1, i.e. row k of Q = Ps*DmInv,
+// Energy.cpp:101-106):
+//   F = Ds*DmInv, J = det F, C = cof F, P = mu(F - C) + lambda(J-1) C   (Energy.cpp:93-97)
+//   g = V P q,  H = V [mu |q|^2 I + lambda (C q)(C q)^T]
+// H is exactly the (k,k) 3x3 block of the 12x12 Hessian of Energy.cpp:115-154: the
+// (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.  The group sums the
+// 6 distinct H entries and 3 force entries, the last lane adds the inertia term
+// m/h^2 (VBD.cpp:170-173), optional penalty contact, and applies the adjugate 3x3 solve
+// when |det H| > 1e-8 (VBD.cpp:141-153, Types.h:85-112).  With SAVE the vertex's position
+// before its update (its position at the start of the sweep) is written to `save` for the
+// Chebyshev blend.
+// ---------------------------------------------------------------------------------------
+template <int LPV, bool MAT, bool CONTACT, bool SAVE>
+__global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* __restrict__ pos,
+                                                     const f4* __restrict__ inert, const float* __restrict__ mass,
+                                                     const int* __restrict__ row, const i4* __restrict__ ptet,
+                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                                     const f2* __restrict__ pc, const f2* __restrict__ pmat,
+                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
+                                                     ContactArgs ct) {
+    const int gpb = blockDim.x / LPV;
+    const int g = xcd_block() * gpb + (int)threadIdx.x / LPV;
+    const int lane = (int)threadIdx.x % LPV;
+    const bool valid = g < count;
+    const int s = slot0 + g;
+
+    float h00 = 0.f, h01 = 0.f, h02 = 0.f, h11 = 0.f, h12 = 0.f, h22 = 0.f;
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    if (valid) {
+        const int p0 = row[s], p1 = row[s + 1];
+        for (int p = p0 + lane; p < p1; p += LPV) {
+            const i4 t = __builtin_nontemporal_load(&ptet[p]);
+            const f4 a = __builtin_nontemporal_load(&pa[p]);
+            const f4 b = __builtin_nontemporal_load(&pb[p]);
+            const f2 c = __builtin_nontemporal_load(&pc[p]);
+            float tmu = mu, tlam = lambda;
+            if constexpr (MAT) {
+                const f2 mm = __builtin_nontemporal_load(&pmat[p]);
+                tmu = mm.x;
+                tlam = mm.y;
+            }
+            const f4 x0 = pos[t.x], x1 = pos[t.y], x2 = pos[t.z], x3 = pos[t.w];
+            const float D00 = a.x, D01 = a.y, D02 = a.z, D10 = a.w;
+            const float D11 = b.x, D12 = b.y, D20 = b.z, D21 = b.w;
+            const float D22 = c.x, V = c.y;
+            // Ds columns (Energy.cpp:11-13)
+            const float e1x = x1.x - x0.x, e1y = x1.y - x0.y, e1z = x1.z - x0.z;
+            const float e2x = x2.x - x0.x, e2y = x2.y - x0.y, e2z = x2.z - x0.z;
+            const float e3x = x3.x - x0.x, e3y = x3.y - x0.y, e3z = x3.z - x0.z;
+            // F = Ds * DmInv
+            const float F00 = e1x * D00 + e2x * D10 + e3x * D20;
+            const float F01 = e1x * D01 + e2x * D11 + e3x * D21;
+            const float F02 = e1x * D02 + e2x * D12 + e3x * D22;
+            const float F10 = e1y * D00 + e2y * D10 + e3y * D20;
+            const float F11 = e1y * D01 + e2y * D11 + e3y * D21;
+            const float F12 = e1y * D02 + e2y * D12 + e3y * D22;
+            const float F20 = e1z * D00 + e2z * D10 + e3z * D20;
+            const float F21 = e1z * D01 + e2z * D11 + e3z * D21;
+            const float F22 = e1z * D02 + e2z * D12 + e3z * D22;
+            // cofactor (Energy.cpp:61-73) and J
+            const float C00 = F11 * F22 - F12 * F21, C01 = F12 * F20 - F10 * F22, C02 = F10 * F21 - F11 * F20;
+            const float C10 = F21 * F02 - F22 * F01, C11 = F22 * F00 - F20 * F02, C12 = F20 * F01 - F21 * F00;
+            const float C20 = F01 * F12 - F02 * F11, C21 = F02 * F10 - F00 * F12, C22 = F00 * F11 - F01 * F10;
+            const float J = F00 * C00 + F01 * C01 + F02 * C02;
+            const float aj = tlam * (J - 1.0f);
+            // corner of this vertex in the tet and its shape vector q
+            const int k = (t.x == s) ? 0 : (t.y == s) ? 1 : (t.z == s) ? 2 : 3;
+            float q0, q1, q2;
+            if (k == 0) {
+                q0 = -(D00 + D10 + D20);
+                q1 = -(D01 + D11 + D21);
+                q2 = -(D02 + D12 + D22);
+            } else if (k == 1) {
+                q0 = D00; q1 = D01; q2 = D02;
+            } else if (k == 2) {
+                q0 = D10; q1 = D11; q2 = D12;
+            } else {
+                q0 = D20; q1 = D21; q2 = D22;
+            }
+            // P q = mu (F q - C q) + aj C q
+            const float Cq0 = C00 * q0 + C01 * q1 + C02 * q2;
+            const float Cq1 = C10 * q0 + C11 * q1 + C12 * q2;
+            const float Cq2 = C20 * q0 + C21 * q1 + C22 * q2;
+            const float Fq0 = F00 * q0 + F01 * q1 + F02 * q2;
+            const float Fq1 = F10 * q0 + F11 * q1 + F12 * q2;
+            const float Fq2 = F20 * q0 + F21 * q1 + F22 * q2;
+            const float g0 = tmu * (Fq0 - Cq0) + aj * Cq0;
+            const float g1 = tmu * (Fq1 - Cq1) + aj * Cq1;
+            const float g2 = tmu * (Fq2 - Cq2) + aj * Cq2;
+            const float qq = q0 * q0 + q1 * q1 + q2 * q2;
+            const float vm = V * tmu * qq;
+            const float vl = V * tlam;
+            const float vc0 = vl * Cq0, vc1 = vl * Cq1, vc2 = vl * Cq2;
+            h00 += vm + vc0 * Cq0;
+            h01 += vc0 * Cq1;
+            h02 += vc0 * Cq2;
+            h11 += vm + vc1 * Cq1;
+            h12 += vc1 * Cq2;
+            h22 += vm + vc2 * Cq2;
+            r0 -= V * g0;
+            r1 -= V * g1;
+            r2 -= V * g2;
+        }
+    }
+    h00 = gsum<LPV>(h00); h01 = gsum<LPV>(h01); h02 = gsum<LPV>(h02);
+    h11 = gsum<LPV>(h11); h12 = gsum<LPV>(h12); h22 = gsum<LPV>(h22);
+    r0 = gsum<LPV>(r0); r1 = gsum<LPV>(r1); r2 = gsum<LPV>(r2);
+    if (!valid || lane != LPV - 1) return;
+
+    const float m = mass[s];
+    const f4 xs = pos[s];
+    const f4 ys = inert[s];
+    const float ms = m * sdt;
+    h00 += ms; h11 += ms; h22 += ms;
+    r0 += (m * (ys.x - xs.x)) * sdt;
+    r1 += (m * (ys.y - xs.y)) * sdt;
+    r2 += (m * (ys.z - xs.z)) * sdt;
+    if constexpr (CONTACT) {
+        const float xa = ct.axis == 0 ? xs.x : (ct.axis == 1 ? xs.y : xs.z);
+        const float d = ct.plane - xa;
+        if (d > 0.f) {
+            if (ct.axis == 0) { h00 += ct.k; r0 += ct.k * d; }
+            else if (ct.axis == 1) { h11 += ct.k; r1 += ct.k * d; }
+            else { h22 += ct.k; r2 += ct.k * d; }
+        }
+    }
+    const float det = h00 * (h11 * h22 - h12 * h12) - h01 * (h01 * h22 - h12 * h02) + h02 * (h01 * h12 - h11 * h02);
+    f4 xn = xs;
+    if (fabsf(det) > 1e-8f) {
+        const float inv = 1.0f / det;
+        const float a00 = h11 * h22 - h12 * h12, a01 = h02 * h12 - h01 * h22, a02 = h01 * h12 - h02 * h11;
+        const float a11 = h00 * h22 - h02 * h02, a12 = h02 * h01 - h00 * h12, a22 = h00 * h11 - h01 * h01;
+        xn.x = xs.x + inv * (a00 * r0 + a01 * r1 + a02 * r2);
+        xn.y = xs.y + inv * (a01 * r0 + a11 * r1 + a12 * r2);
+        xn.z = xs.z + inv * (a02 * r0 + a12 * r1 + a22 * r2);
+    }
+    if constexpr (SAVE) save[s] = xs;
+    pos[s] = xn;
+}
+
+template <int LPV, bool MAT, bool CONTACT, bool SAVE>
+static hipError_t launch_color_t(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+    const int slot0 = m.colorSlotOff[color];
+    const int count = m.colorSlotOff[color + 1] - slot0;
+    if (count <= 0) return hipSuccess;
+    const int gpb = block / LPV;
+    int nb = (count + gpb - 1) / gpb;
+    nb = (nb + 7) & ~7;
+    hipLaunchKernelGGL((k_color_solve<LPV, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.pos,
+                       m.inert, m.mass, m.row, m.ptet, m.pa, m.pb, m.pc, m.pmat, save, mu, lambda, sdt, ct);
+    return hipGetLastError();
+}
+
+template <int LPV>
+static hipError_t launch_color_l(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+    const bool mat = m.perTetMat, con = ct.on != 0, sv = save != nullptr;
+#define VBD_DISPATCH(MT, CT, SV) \
+    if (mat == MT && con == CT && sv == SV) return launch_color_t<LPV, MT, CT, SV>(m, color, block, sdt, mu, lambda, ct, save, s);
+    VBD_DISPATCH(false, false, false) VBD_DISPATCH(false, false, true)
+    VBD_DISPATCH(false, true, false) VBD_DISPATCH(false, true, true)
+    VBD_DISPATCH(true, false, false) VBD_DISPATCH(true, false, true)
+    VBD_DISPATCH(true, true, false) VBD_DISPATCH(true, true, true)
+#undef VBD_DISPATCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_color(const DevMesh& m, int color, int lpv, int block, float sdt, float mu, float lambda,
+                        const ContactArgs& ct, f4* save, hipStream_t s) {
+    if (block < 64 || block > 256 || (block % 64) != 0) return hipErrorInvalidValue;
+    switch (lpv) {
+        case 4: return launch_color_l<4>(m, color, block, sdt, mu, lambda, ct, save, s);
+        case 8: return launch_color_l<8>(m, color, block, sdt, mu, lambda, ct, save, s);
+        case 16: return launch_color_l<16>(m, color, block, sdt, mu, lambda, ct, save, s);
+        case 32: return launch_color_l<32>(m, color, block, sdt, mu, lambda, ct, save, s);
+        case 64: return launch_color_l<64>(m, color, block, sdt, mu, lambda, ct, save, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Chebyshev blend over the free slots: x = omega (x - x_pp) + x_pp   (VBD.cpp:290-300)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_blend(int n, f4* __restrict__ pos, const f4* __restrict__ pp, float omega) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f4 x = pos[i];
+    const f4 q = pp[i];
+    f4 r = x;
+    r.x = omega * (x.x - q.x) + q.x;
+    r.y = omega * (x.y - q.y) + q.y;
+    r.z = omega * (x.z - q.z) + q.z;
+    pos[i] = r;
+}
+
+hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s) {
+    if (m.nFree == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_blend, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.pos, pp, omega);
+    return hipGetLastError();
+}
+
+// forwardStep (VBD.cpp:103-112): x_prev = x; free: v += h g, y = x + h v; pinned: y = x.
+__global__ __launch_bounds__(256) void k_predict(int n, int nFree, const f4* __restrict__ pos, f4* __restrict__ vel,
+                                                 f4* __restrict__ xprev, f4* __restrict__ inert, float h, float gx,
+                                                 float gy, float gz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f4 x = pos[i];
+    xprev[i] = x;
+    f4 y = x;
+    if (i < nFree) {
+        f4 v = vel[i];
+        v.x += h * gx;
+        v.y += h * gy;
+        v.z += h * gz;
+        vel[i] = v;
+        y.x = x.x + h * v.x;
+        y.y = x.y + h * v.y;
+        y.z = x.z + h * v.z;
+    }
+    inert[i] = y;
+}
+
+hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s) {
+    if (m.nV == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict, dim3((m.nV + 255) / 256), dim3(256), 0, s, m.nV, m.nFree, m.pos, m.vel, m.xprev,
+                       m.inert, h, g[0], g[1], g[2]);
+    return hipGetLastError();
+}
+
+// updateVelocity (TetMesh.h:305-311): free: v = (x - x_prev) / h * damping.
+__global__ __launch_bounds__(256) void k_velocity(int nFree, const f4* __restrict__ pos,
+                                                  const f4* __restrict__ xprev, f4* __restrict__ vel, float h,
+                                                  float damp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nFree) return;
+    const f4 x = pos[i], xp = xprev[i];
+    f4 v = vel[i];
+    v.x = ((x.x - xp.x) / h) * damp;
+    v.y = ((x.y - xp.y) / h) * damp;
+    v.z = ((x.z - xp.z) / h) * damp;
+    vel[i] = v;
+}
+
+hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t s) {
+    if (m.nFree == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_velocity, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.pos, m.xprev, m.vel, h,
+                       damping);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_copy_f4(int n, const f4* __restrict__ src, f4* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+hipError_t launch_copy_inertia(const DevMesh& m, hipStream_t s) {
+    if (m.nFree == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_f4, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.inert, m.pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_f4(int n, const f4* src, f4* dst, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_f4, dim3((n + 255) / 256), dim3(256), 0, s, n, src, dst);
+    return hipGetLastError();
+}
+
+// caller layout (3 floats per vertex, caller numbering) <-> slot layout (f4 per slot)
+__global__ __launch_bounds__(256) void k_scatter_pos(int n, const float* __restrict__ src3, const int* __restrict__ inv,
+                                                     f4* __restrict__ dst) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    f4 r;
+    r.x = src3[3 * (size_t)v];
+    r.y = src3[3 * (size_t)v + 1];
+    r.z = src3[3 * (size_t)v + 2];
+    r.w = 0.f;
+    dst[inv[v]] = r;
+}
+
+__global__ __launch_bounds__(256) void k_gather_pos(int n, const f4* __restrict__ src, const int* __restrict__ inv,
+                                                    float* __restrict__ dst3) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const f4 r = src[inv[v]];
+    dst3[3 * (size_t)v] = r.x;
+    dst3[3 * (size_t)v + 1] = r.y;
+    dst3[3 * (size_t)v + 2] = r.z;
+}
+
+hipError_t launch_scatter_pos(int n, const float* src3, const int* inv, f4* dst, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_pos, dim3((n + 255) / 256), dim3(256), 0, s, n, src3, inv, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_pos, dim3((n + 255) / 256), dim3(256), 0, s, n, src, inv, dst3);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Diagnostic energy (VBD.cpp:258-288 with NeoHookean.cpp:6-20): sum over tets of
+// V0 [mu/2 (|F|^2 - 3) - mu ln J + lambda/2 (ln J)^2], J = |det F| (the product of the
+// non-negative singular values).  Accumulated in double.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_energy(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
+                                                const f4* __restrict__ pos, const float* __restrict__ dminv,
+                                                const float* __restrict__ vol, const float* __restrict__ tmu,
+                                                const float* __restrict__ tlam, float mu, float lambda,
+                                                double* __restrict__ out) {
+    __shared__ double red[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (t < nT) {
+        const i4 tt = tets[t];
+        const f4 x0 = pos[inv[tt.x]], x1 = pos[inv[tt.y]], x2 = pos[inv[tt.z]], x3 = pos[inv[tt.w]];
+        const float* D = dminv + 9 * (size_t)t;
+        const float ds[3][3] = {{x1.x - x0.x, x2.x - x0.x, x3.x - x0.x},
+                                {x1.y - x0.y, x2.y - x0.y, x3.y - x0.y},
+                                {x1.z - x0.z, x2.z - x0.z, x3.z - x0.z}};
+        float F[3][3];
+        float tr = 0.f;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                F[r][c] = ds[r][0] * D[c] + ds[r][1] * D[3 + c] + ds[r][2] * D[6 + c];
+                tr += F[r][c] * F[r][c];
+            }
+        const float det = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                          F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                          F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+        const float J = fabsf(det);
+        const float m_ = tmu ? tmu[t] : mu, l_ = tlam ? tlam[t] : lambda;
+        const double lj = log((double)J);
+        e = (0.5 * m_ * (tr - 3.0) - m_ * lj + 0.5 * l_ * lj * lj) * vol[t];
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+hipError_t launch_energy(const DevMesh& m, float mu, float lambda, double* out, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(out, 0, sizeof(double), s);
+    if (e != hipSuccess || m.nT == 0) return e;
+    hipLaunchKernelGGL(k_energy, dim3((m.nT + 255) / 256), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos, m.dminv,
+                       m.vol, m.tmu, m.tlam, mu, lambda, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Mesh build
+// ---------------------------------------------------------------------------------------
+constexpr int SCAN_B = 256, SCAN_I = 8, SCAN_CHUNK = SCAN_B * SCAN_I;
+
+// exclusive scan of one chunk per block; chunk totals to sums[]
+__global__ __launch_bounds__(SCAN_B) void k_scan_chunk(const int* __restrict__ in, int* __restrict__ out,
+                                                       int* __restrict__ sums, int n) {
+    __shared__ int sh[SCAN_B];
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_I;
+    int v[SCAN_I];
+    int tot = 0;
+    for (int i = 0; i < SCAN_I; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        tot += v[i];
+    }
+    sh[threadIdx.x] = tot;
+    __syncthreads();
+    for (int off = 1; off < SCAN_B; off <<= 1) {
+        const int t = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int ex = sh[threadIdx.x] - tot;
+    for (int i = 0; i < SCAN_I; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == SCAN_B - 1) sums[blockIdx.x] = sh[SCAN_B - 1];
+}
+
+__global__ __launch_bounds__(SCAN_B) void k_scan_add(int* __restrict__ out, const int* __restrict__ offs, int n) {
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_I;
+    const int o = offs[blockIdx.x];
+    for (int i = 0; i < SCAN_I; ++i)
+        if (base + i < n) out[base + i] += o;
+}
+
+__global__ void k_scan_total(const int* __restrict__ in, int* __restrict__ out, int n) {
+    out[n] = n > 0 ? out[n - 1] + in[n - 1] : 0;
+}
+
+// out[0..n] = exclusive scan of in[0..n), out[n] = total.  out must hold n+1 ints.
+static hipError_t scan_exclusive(const int* in, int* out, int n, hipStream_t s) {
+    hipError_t e;
+    if (n > 0) {
+        const int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        int* sums = nullptr;
+        int* soff = nullptr;
+        if ((e = hipMalloc(&sums, sizeof(int) * nb)) != hipSuccess) return e;
+        if ((e = hipMalloc(&soff, sizeof(int) * (nb + 1))) != hipSuccess) { (void)hipFree(sums); return e; }
+        hipLaunchKernelGGL(k_scan_chunk, dim3(nb), dim3(SCAN_B), 0, s, in, out, sums, n);
+        e = hipGetLastError();
+        if (e == hipSuccess && nb > 1) {
+            e = scan_exclusive(sums, soff, nb, s);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_B), 0, s, out, soff, n);
+                e = hipGetLastError();
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(sums);
+        (void)hipFree(soff);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_degree(int nT, const i4* __restrict__ tets, int* __restrict__ deg) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const i4 tt = tets[t];
+    atomicAdd(&deg[tt.x], 1);
+    atomicAdd(&deg[tt.y], 1);
+    atomicAdd(&deg[tt.z], 1);
+    atomicAdd(&deg[tt.w], 1);
+}
+
+__global__ __launch_bounds__(256) void k_fill_adj(int nT, const i4* __restrict__ tets, const int* __restrict__ off,
+                                                  int* __restrict__ cursor, int* __restrict__ entry) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const i4 tt = tets[t];
+    const int v[4] = {tt.x, tt.y, tt.z, tt.w};
+    for (int j = 0; j < 4; ++j) {
+        const int slot = off[v[j]] + atomicAdd(&cursor[v[j]], 1);
+        entry[slot] = t * 4 + j;
+    }
+}
+
+// Atomic fill leaves each vertex's entries in arbitrary order; sorting by tet*4+corner
+// restores ascending tet id, the order TetMesh::initialize appends them (TetMesh.h:254-266).
+__global__ __launch_bounds__(256) void k_sort_segments(int nV, const int* __restrict__ off, int* __restrict__ entry) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int b = off[v], e = off[v + 1];
+    for (int i = b + 1; i < e; ++i) {
+        const int key = entry[i];
+        int j = i - 1;
+        while (j >= b && entry[j] > key) {
+            entry[j + 1] = entry[j];
+            --j;
+        }
+        entry[j + 1] = key;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_key_flag(int n, const int* __restrict__ key, int k, int* __restrict__ flag) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) flag[v] = key[v] == k ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_key_place(int n, const int* __restrict__ key, int k,
+                                                   const int* __restrict__ rank, int base, int* __restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && key[v] == k) out[base + rank[v]] = v;
+}
+
+__global__ __launch_bounds__(256) void k_slot_key(int n, const int* __restrict__ colorOf,
+                                                  const uint8_t* __restrict__ fixed, int nColors,
+                                                  int* __restrict__ key) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) key[v] = fixed[v] ? nColors : colorOf[v];
+}
+
+__global__ __launch_bounds__(256) void k_invert(int n, const int* __restrict__ perm, int* __restrict__ inv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) inv[perm[s]] = s;
+}
+
+__global__ __launch_bounds__(256) void k_slot_valence(int nFree, const int* __restrict__ perm,
+                                                      const int* __restrict__ off, int* __restrict__ cnt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nFree) cnt[s] = off[perm[s] + 1] - off[perm[s]];
+}
+
+__global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __restrict__ perm,
+                                                     const int* __restrict__ inv, const int* __restrict__ off,
+                                                     const int* __restrict__ entry, const i4* __restrict__ tets,
+                                                     const float* __restrict__ dminv, const float* __restrict__ vol,
+                                                     const float* __restrict__ tmu, const float* __restrict__ tlam,
+                                                     const int* __restrict__ row, i4* __restrict__ ptet,
+                                                     f4* __restrict__ pa, f4* __restrict__ pb, f2* __restrict__ pc,
+                                                     f2* __restrict__ pmat) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nFree) return;
+    const int v = perm[s];
+    int p = row[s];
+    for (int e = off[v]; e < off[v + 1]; ++e, ++p) {
+        const int t = entry[e] >> 2;
+        const i4 tt = tets[t];
+        i4 r;
+        r.x = inv[tt.x];
+        r.y = inv[tt.y];
+        r.z = inv[tt.z];
+        r.w = inv[tt.w];
+        ptet[p] = r;
+        const float* D = dminv + 9 * (size_t)t;
+        f4 a, b;
+        f2 c;
+        a.x = D[0]; a.y = D[1]; a.z = D[2]; a.w = D[3];
+        b.x = D[4]; b.y = D[5]; b.z = D[6]; b.w = D[7];
+        c.x = D[8]; c.y = vol[t];
+        pa[p] = a;
+        pb[p] = b;
+        pc[p] = c;
+        if (pmat) {
+            f2 mm;
+            mm.x = tmu[t];
+            mm.y = tlam[t];
+            pmat[p] = mm;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_scalar(int n, const int* __restrict__ perm,
+                                                       const float* __restrict__ src, float* __restrict__ dst) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) dst[s] = src[perm[s]];
+}
+
+template <class T>
+static hipError_t dalloc(DevMesh& m, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e == hipSuccess) m.bytes += (int64_t)(sizeof(T) * count);
+    return e;
+}
+
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+#define VBD_TRY(expr)                                   \
+    do {                                                \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) {                         \
+            rc = _e;                                    \
+            err = std::string(#expr) + ": " + hipGetErrorString(_e); \
+            goto fail;                                  \
+        }                                               \
+    } while (0)
+
+hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* d_tets, const int* d_colorOf,
+                             const uint8_t* d_fixed, const float* d_dminv, const float* d_vol, const float* d_mu,
+                             const float* d_lam, const float* d_x3, const float* d_y3, const float* d_v3,
+                             const float* d_mass, hipStream_t s, std::string& err) {
+    hipError_t rc = hipSuccess;
+    int* deg = nullptr;
+    int* cursor = nullptr;
+    int* key = nullptr;
+    int* flag = nullptr;
+    int* rank = nullptr;
+    int* cnt = nullptr;
+    std::vector<int> hrow_last(1, 0);
+    m.nV = nV;
+    m.nT = nT;
+    m.nColors = nColors;
+    m.perTetMat = d_mu != nullptr;
+    m.colorSlotOff.assign(nColors + 1, 0);
+    m.colorFullOff.assign(nColors + 1, 0);
+
+    // ---- vertex -> tet CSR in ascending tet order ----
+    VBD_TRY(dalloc(m, &m.adjOff, (size_t)nV + 1));
+    VBD_TRY(dalloc(m, &m.adjEntry, (size_t)4 * nT));
+    VBD_TRY(hipMalloc(&deg, sizeof(int) * (nV + 1)));
+    VBD_TRY(hipMalloc(&cursor, sizeof(int) * (nV + 1)));
+    VBD_TRY(hipMemsetAsync(deg, 0, sizeof(int) * (nV + 1), s));
+    VBD_TRY(hipMemsetAsync(cursor, 0, sizeof(int) * (nV + 1), s));
+    if (nT > 0) {
+        hipLaunchKernelGGL(k_degree, grid_for(nT), dim3(256), 0, s, nT, d_tets, deg);
+        VBD_TRY(hipGetLastError());
+    }
+    VBD_TRY(scan_exclusive(deg, m.adjOff, nV, s));
+    if (nT > 0) {
+        hipLaunchKernelGGL(k_fill_adj, grid_for(nT), dim3(256), 0, s, nT, d_tets, m.adjOff, cursor, m.adjEntry);
+        VBD_TRY(hipGetLastError());
+    }
+    if (nV > 0) {
+        hipLaunchKernelGGL(k_sort_segments, grid_for(nV), dim3(256), 0, s, nV, m.adjOff, m.adjEntry);
+        VBD_TRY(hipGetLastError());
+    }
+
+    // ---- per-color lists (every vertex) and color-major slot order (free vertices) ----
+    VBD_TRY(dalloc(m, &m.colorList, (size_t)nV));
+    VBD_TRY(dalloc(m, &m.perm, (size_t)nV));
+    VBD_TRY(dalloc(m, &m.inv, (size_t)nV));
+    VBD_TRY(hipMalloc(&key, sizeof(int) * (nV + 1)));
+    VBD_TRY(hipMalloc(&flag, sizeof(int) * (nV + 1)));
+    VBD_TRY(hipMalloc(&rank, sizeof(int) * (nV + 1)));
+    {
+        int base = 0;
+        for (int c = 0; c < nColors; ++c) {
+            int total = 0;
+            if (nV > 0) {
+                hipLaunchKernelGGL(k_key_flag, grid_for(nV), dim3(256), 0, s, nV, d_colorOf, c, flag);
+                VBD_TRY(hipGetLastError());
+            }
+            VBD_TRY(scan_exclusive(flag, rank, nV, s));
+            VBD_TRY(hipMemcpyAsync(&total, rank + nV, sizeof(int), hipMemcpyDeviceToHost, s));
+            VBD_TRY(hipStreamSynchronize(s));
+            if (total > 0) {
+                hipLaunchKernelGGL(k_key_place, grid_for(nV), dim3(256), 0, s, nV, d_colorOf, c, rank, base,
+                                   m.colorList);
+                VBD_TRY(hipGetLastError());
+            }
+            base += total;
+            m.colorFullOff[c + 1] = base;
+        }
+        if (base != nV) {
+            err = "color lists do not cover every vertex exactly once";
+            rc = hipErrorInvalidValue;
+            goto fail;
+        }
+        if (nV > 0) {
+            hipLaunchKernelGGL(k_slot_key, grid_for(nV), dim3(256), 0, s, nV, d_colorOf, d_fixed, nColors, key);
+            VBD_TRY(hipGetLastError());
+        }
+        base = 0;
+        for (int c = 0; c <= nColors; ++c) {
+            int total = 0;
+            if (nV > 0) {
+                hipLaunchKernelGGL(k_key_flag, grid_for(nV), dim3(256), 0, s, nV, key, c, flag);
+                VBD_TRY(hipGetLastError());
+            }
+            VBD_TRY(scan_exclusive(flag, rank, nV, s));
+            VBD_TRY(hipMemcpyAsync(&total, rank + nV, sizeof(int), hipMemcpyDeviceToHost, s));
+            VBD_TRY(hipStreamSynchronize(s));
+            if (total > 0) {
+                hipLaunchKernelGGL(k_key_place, grid_for(nV), dim3(256), 0, s, nV, key, c, rank, base, m.perm);
+                VBD_TRY(hipGetLastError());
+            }
+            base += total;
+            if (c < nColors) m.colorSlotOff[c + 1] = base;
+        }
+        m.nFree = m.colorSlotOff[nColors];
+        if (nV > 0) {
+            hipLaunchKernelGGL(k_invert, grid_for(nV), dim3(256), 0, s, nV, m.perm, m.inv);
+            VBD_TRY(hipGetLastError());
+        }
+    }
+
+    // ---- pair records in slot order ----
+    VBD_TRY(dalloc(m, &m.row, (size_t)m.nFree + 1));
+    VBD_TRY(hipMalloc(&cnt, sizeof(int) * (m.nFree + 1)));
+    if (m.nFree > 0) {
+        hipLaunchKernelGGL(k_slot_valence, grid_for(m.nFree), dim3(256), 0, s, m.nFree, m.perm, m.adjOff, cnt);
+        VBD_TRY(hipGetLastError());
+    }
+    VBD_TRY(scan_exclusive(cnt, m.row, m.nFree, s));
+    {
+        std::vector<int> hcnt(m.nFree > 0 ? m.nFree : 1, 0);
+        if (m.nFree > 0) VBD_TRY(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * m.nFree, hipMemcpyDeviceToHost, s));
+        VBD_TRY(hipMemcpyAsync(hrow_last.data(), m.row + m.nFree, sizeof(int), hipMemcpyDeviceToHost, s));
+        VBD_TRY(hipStreamSynchronize(s));
+        m.nPairs = hrow_last[0];
+        m.maxValence = 0;
+        m.valenceHist.assign(1, 0);
+        for (int i = 0; i < m.nFree; ++i) {
+            const int d = hcnt[i];
+            if (d > m.maxValence) m.maxValence = d;
+            if ((int)m.valenceHist.size() <= d) m.valenceHist.resize(d + 1, 0);
+            m.valenceHist[d]++;
+        }
+    }
+    VBD_TRY(dalloc(m, &m.ptet, (size_t)m.nPairs));
+    VBD_TRY(dalloc(m, &m.pa, (size_t)m.nPairs));
+    VBD_TRY(dalloc(m, &m.pb, (size_t)m.nPairs));
+    VBD_TRY(dalloc(m, &m.pc, (size_t)m.nPairs));
+    if (m.perTetMat) VBD_TRY(dalloc(m, &m.pmat, (size_t)m.nPairs));
+    if (m.nFree > 0) {
+        hipLaunchKernelGGL(k_build_pairs, grid_for(m.nFree), dim3(256), 0, s, m.nFree, m.perm, m.inv, m.adjOff,
+                           m.adjEntry, d_tets, d_dminv, d_vol, d_mu, d_lam, m.row, m.ptet, m.pa, m.pb, m.pc,
+                           m.perTetMat ? m.pmat : nullptr);
+        VBD_TRY(hipGetLastError());
+    }
+
+    // ---- per-tet arrays kept for the energy reduction ----
+    VBD_TRY(dalloc(m, &m.tets, (size_t)nT));
+    VBD_TRY(dalloc(m, &m.dminv, (size_t)9 * nT));
+    VBD_TRY(dalloc(m, &m.vol, (size_t)nT));
+    if (nT > 0) {
+        VBD_TRY(hipMemcpyAsync(m.tets, d_tets, sizeof(i4) * nT, hipMemcpyDeviceToDevice, s));
+        VBD_TRY(hipMemcpyAsync(m.dminv, d_dminv, sizeof(float) * 9 * nT, hipMemcpyDeviceToDevice, s));
+        VBD_TRY(hipMemcpyAsync(m.vol, d_vol, sizeof(float) * nT, hipMemcpyDeviceToDevice, s));
+    }
+    if (m.perTetMat) {
+        VBD_TRY(dalloc(m, &m.tmu, (size_t)nT));
+        VBD_TRY(dalloc(m, &m.tlam, (size_t)nT));
+        if (nT > 0) {
+            VBD_TRY(hipMemcpyAsync(m.tmu, d_mu, sizeof(float) * nT, hipMemcpyDeviceToDevice, s));
+            VBD_TRY(hipMemcpyAsync(m.tlam, d_lam, sizeof(float) * nT, hipMemcpyDeviceToDevice, s));
+        }
+    }
+
+    // ---- vertex state in slot order ----
+    VBD_TRY(dalloc(m, &m.pos, (size_t)nV));
+    VBD_TRY(dalloc(m, &m.inert, (size_t)nV));
+    VBD_TRY(dalloc(m, &m.vel, (size_t)nV));
+    VBD_TRY(dalloc(m, &m.xprev, (size_t)nV));
+    VBD_TRY(dalloc(m, &m.save[0], (size_t)nV));
+    VBD_TRY(dalloc(m, &m.save[1], (size_t)nV));
+    VBD_TRY(dalloc(m, &m.mass, (size_t)nV));
+    VBD_TRY(hipMemsetAsync(m.save[0], 0, sizeof(f4) * (nV > 0 ? nV : 1), s));
+    VBD_TRY(hipMemsetAsync(m.save[1], 0, sizeof(f4) * (nV > 0 ? nV : 1), s));
+    VBD_TRY(hipMemsetAsync(m.xprev, 0, sizeof(f4) * (nV > 0 ? nV : 1), s));
+    VBD_TRY(launch_scatter_pos(nV, d_x3, m.inv, m.pos, s));
+    VBD_TRY(launch_scatter_pos(nV, d_y3 ? d_y3 : d_x3, m.inv, m.inert, s));
+    if (d_v3) {
+        VBD_TRY(launch_scatter_pos(nV, d_v3, m.inv, m.vel, s));
+    } else {
+        VBD_TRY(hipMemsetAsync(m.vel, 0, sizeof(f4) * (nV > 0 ? nV : 1), s));
+    }
+    if (nV > 0) {
+        hipLaunchKernelGGL(k_gather_scalar, grid_for(nV), dim3(256), 0, s, nV, m.perm, d_mass, m.mass);
+        VBD_TRY(hipGetLastError());
+    }
+    VBD_TRY(hipStreamSynchronize(s));
+    (void)hipFree(deg);
+    (void)hipFree(cursor);
+    (void)hipFree(key);
+    (void)hipFree(flag);
+    (void)hipFree(rank);
+    (void)hipFree(cnt);
+    return hipSuccess;
+fail:
+    (void)hipStreamSynchronize(s);
+    if (deg) (void)hipFree(deg);
+    if (cursor) (void)hipFree(cursor);
+    if (key) (void)hipFree(key);
+    if (flag) (void)hipFree(flag);
+    if (rank) (void)hipFree(rank);
+    if (cnt) (void)hipFree(cnt);
+    free_device_mesh(m);
+    return rc == hipSuccess ? hipErrorUnknown : rc;
+}
+#undef VBD_TRY
+
+void free_device_mesh(DevMesh& m) {
+    void* ptrs[] = {m.pos, m.inert, m.vel, m.xprev, m.save[0], m.save[1], m.mass, m.row, m.perm, m.inv,
+                    m.ptet, m.pa, m.pb, m.pc, m.pmat, m.adjOff, m.adjEntry, m.colorList, m.tets, m.dminv,
+                    m.vol, m.tmu, m.tlam};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    m = DevMesh();
+}
+
+}  // namespace vbd

@@ -1,0 +1,505 @@
+"""Host-side mirror of the reference VBD interfaces over the gfx950 C ABI.
+
+Names, argument meaning and call order follow the reference so code written against it
+reads the same (paths are relative to the reference repository):
+
+  * ``VBDParams``      -- VBD.h:14-25
+  * ``TetMesh``        -- TetMesh.h:19-349 (state arrays, initialize, time-step helpers)
+  * ``VBD_GPU_Solver`` -- VBD_GPU_Wrapper.h:7-45 (initFromCPU, solveIteration, solveVBD,
+                          copyPositionsToCPU, updateInertiaFromCPU)
+  * ``VBDsolver``      -- VBD.h:33-140 (setParams, initializeColoring, precomputeTetData,
+                          step, simulate, computeTotalEnergy, getters), with the whole
+                          step running on the device.
+
+Vertex arrays are (N, 3) float32 numpy arrays (rows = vertices), the transpose view of the
+reference's 3 x N column-major TVerticesMat, i.e. the same memory layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _capi as A
+from ._capi import check, lib, ptr
+
+
+@dataclass
+class VBDParams:
+    """VBDParams (VBD.h:14-25) plus the build's extensions."""
+
+    dt: float = 0.01666666
+    numIterations: int = 50
+    substeps: int = 1
+    gravity: tuple = (0.0, -9.8, 0.0)
+    useAcceleration: bool = False
+    accelerationRho: float = 0.75
+    numThreads: int = 4          # accepted for source compatibility; the device path ignores it
+    mu: float = 1e6
+    lambda_: float = 1e6
+    density: float = 1000.0
+    # extensions (no reference counterpart)
+    velocityDamping: float = 0.999   # TetMesh.h:308 hard-codes 0.999
+    startFromInertia: bool = False   # True reproduces VBD_GPU_Wrapper.cu:334 (x <- y)
+    contact: Optional[tuple] = None  # (axis, plane, stiffness) penalty half-space x[axis] >= plane
+
+    def to_c(self) -> A.StepParams:
+        p = A.StepParams()
+        p.dt = self.dt
+        p.num_iterations = int(self.numIterations)
+        p.substeps = int(self.substeps)
+        p.gravity[:] = [float(g) for g in self.gravity]
+        p.use_acceleration = int(bool(self.useAcceleration))
+        p.acceleration_rho = self.accelerationRho
+        p.mu = self.mu
+        p.lambda_ = self.lambda_
+        p.velocity_damping = self.velocityDamping
+        p.start_from_inertia = int(bool(self.startFromInertia))
+        if self.contact is not None:
+            axis, plane, k = self.contact
+            p.contact_on, p.contact_axis, p.contact_plane, p.contact_k = 1, int(axis), float(plane), float(k)
+        return p
+
+
+class TetMesh:
+    """TetMesh (TetMesh.h:19-349).  Adjacency is held as CSR (vertAdjacentTets)."""
+
+    def __init__(self):
+        self.numVerts = 0
+        self.numTets = 0
+
+    def initialize(self, rest_positions, current_positions, density, tetIndices, mu, lambda_, fixedVerts=()):
+        """TetMesh::initialize (TetMesh.h:213-275)."""
+        rest = np.ascontiguousarray(rest_positions, np.float32).reshape(-1, 3)
+        cur = np.ascontiguousarray(current_positions, np.float32).reshape(-1, 3)
+        if rest.shape != cur.shape:
+            raise ValueError("rest and current positions differ in size")
+        self.numVerts = rest.shape[0]
+        self.tets = np.ascontiguousarray(tetIndices, np.int32).reshape(-1, 4)
+        self.numTets = self.tets.shape[0]
+        self.mRestPos = rest.copy()
+        self.mVertPos = cur.copy()
+        self.mVertPrevPos = cur.copy()
+        self.mVelocity = np.zeros_like(cur)
+        self.mInertia = np.zeros_like(cur)
+        mass, vol, off, adj = A.mesh_initialize(rest, self.tets, density)
+        self.vertexMass = mass
+        self.tetRestVol = vol
+        self.tetMu = np.full(self.numTets, mu, np.float32)
+        self.tetLambda = np.full(self.numTets, lambda_, np.float32)
+        self.adjOffsets = off
+        self.adjTets = adj
+        self.isFixed = np.zeros(self.numVerts, np.uint8)
+        fv = np.asarray(list(fixedVerts), dtype=np.int64)
+        fv = fv[(fv >= 0) & (fv < self.numVerts)]
+        self.isFixed[fv] = 1
+        return self
+
+    @property
+    def vertAdjacentTets(self) -> List[np.ndarray]:
+        return [self.adjTets[self.adjOffsets[i]: self.adjOffsets[i + 1]] for i in range(self.numVerts)]
+
+    # time-step helpers (host), TetMesh.h:305-348
+    def updateVelocity(self, dt):
+        free = self.isFixed == 0
+        self.mVelocity[free] = ((self.mVertPos[free] - self.mVertPrevPos[free]) / np.float32(dt)) * np.float32(0.999)
+
+    def applyGravity(self, gravity, dt):
+        free = self.isFixed == 0
+        self.mVelocity[free] += np.float32(dt) * np.asarray(gravity, np.float32)
+
+    def computeInertia(self, dt):
+        free = (self.isFixed == 0)[:, None]
+        self.mInertia[:] = np.where(free, self.mVertPos + np.float32(dt) * self.mVelocity, self.mVertPos)
+
+    def savePreviousPosition(self):
+        self.mVertPrevPos[:] = self.mVertPos
+
+
+def mesh_from_generator(m, mu=None, lam=None) -> TetMesh:
+    """TetMesh from a vbd_tetmesh_amd.meshgen.Mesh (per-tet materials carried over)."""
+    t = TetMesh().initialize(m.rest, m.x0, m.density, m.tets, m.mu if mu is None else mu, m.lam if lam is None else lam,
+                             np.nonzero(m.fixed)[0])
+    if m.tet_mu is not None:
+        t.tetMu = np.ascontiguousarray(m.tet_mu, np.float32)
+        t.tetLambda = np.ascontiguousarray(m.tet_lambda, np.float32)
+    if m.v0 is not None:
+        t.mVelocity[:] = m.v0
+    return t
+
+
+def _colors_to_csr(colors) -> tuple:
+    if isinstance(colors, tuple) and len(colors) == 2:
+        off, verts = colors
+        return np.ascontiguousarray(off, np.int32), np.ascontiguousarray(verts, np.int32)
+    off = np.zeros(len(colors) + 1, np.int32)
+    for i, c in enumerate(colors):
+        off[i + 1] = off[i] + len(c)
+    verts = np.concatenate([np.asarray(c, np.int32) for c in colors]) if len(colors) else np.zeros(0, np.int32)
+    return off, np.ascontiguousarray(verts, np.int32)
+
+
+class _Handle:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib.vbd_create(C.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.vbd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMesh(_Handle):
+    """One uploaded mesh (or batch) on the device; thin object wrapper of the C ABI."""
+
+    def __init__(self, device: int = 0):
+        super().__init__(device)
+        self.num_verts = 0
+        self._keep = None
+
+    # -- upload
+    def init(self, positions, mass, is_fixed, tets, dminv, rest_volume, color_off, color_verts, inertia=None,
+             velocity=None, tet_mu=None, tet_lambda=None):
+        arrs = dict(
+            positions=np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+            mass=np.ascontiguousarray(mass, np.float32),
+            is_fixed=np.ascontiguousarray(is_fixed, np.uint8),
+            tets=np.ascontiguousarray(tets, np.int32).reshape(-1, 4),
+            dminv=np.ascontiguousarray(dminv, np.float32).reshape(-1, 9),
+            rest_volume=np.ascontiguousarray(rest_volume, np.float32),
+            color_off=np.ascontiguousarray(color_off, np.int32),
+            color_verts=np.ascontiguousarray(color_verts, np.int32),
+        )
+        if inertia is not None:
+            arrs["inertia"] = np.ascontiguousarray(inertia, np.float32).reshape(-1, 3)
+        if velocity is not None:
+            arrs["velocity"] = np.ascontiguousarray(velocity, np.float32).reshape(-1, 3)
+        if tet_mu is not None:
+            arrs["tet_mu"] = np.ascontiguousarray(tet_mu, np.float32)
+            arrs["tet_lambda"] = np.ascontiguousarray(tet_lambda, np.float32)
+        d = A.MeshDesc()
+        d.num_verts = arrs["positions"].shape[0]
+        d.num_tets = arrs["tets"].shape[0]
+        d.positions = ptr(arrs["positions"], C.c_float)
+        d.inertia = ptr(arrs.get("inertia"), C.c_float)
+        d.velocity = ptr(arrs.get("velocity"), C.c_float)
+        d.mass = ptr(arrs["mass"], C.c_float)
+        d.is_fixed = ptr(arrs["is_fixed"], C.c_uint8)
+        d.tets = ptr(arrs["tets"], C.c_int32)
+        d.dminv = ptr(arrs["dminv"], C.c_float)
+        d.rest_volume = ptr(arrs["rest_volume"], C.c_float)
+        d.tet_mu = ptr(arrs.get("tet_mu"), C.c_float)
+        d.tet_lambda = ptr(arrs.get("tet_lambda"), C.c_float)
+        d.num_colors = arrs["color_off"].size - 1
+        d.color_offsets = ptr(arrs["color_off"], C.c_int32)
+        d.color_verts = ptr(arrs["color_verts"], C.c_int32)
+        check(lib.vbd_init(self.h, C.byref(d)), self.h)
+        self.num_verts = d.num_verts
+        return self
+
+    def init_batch(self, descs: Sequence[dict]):
+        """descs: per-mesh dicts with the keyword arguments of init() (local numbering)."""
+        keep = []
+        arr = (A.MeshDesc * len(descs))()
+        total = 0
+        for i, kw in enumerate(descs):
+            a = {k: (np.ascontiguousarray(v) if v is not None else None) for k, v in kw.items()}
+            pos = np.ascontiguousarray(a["positions"], np.float32).reshape(-1, 3)
+            tets = np.ascontiguousarray(a["tets"], np.int32).reshape(-1, 4)
+            fields = dict(positions=pos, mass=np.ascontiguousarray(a["mass"], np.float32),
+                          is_fixed=np.ascontiguousarray(a["is_fixed"], np.uint8), tets=tets,
+                          dminv=np.ascontiguousarray(a["dminv"], np.float32).reshape(-1, 9),
+                          rest_volume=np.ascontiguousarray(a["rest_volume"], np.float32),
+                          color_off=np.ascontiguousarray(a["color_off"], np.int32),
+                          color_verts=np.ascontiguousarray(a["color_verts"], np.int32))
+            for opt in ("inertia", "velocity"):
+                if a.get(opt) is not None:
+                    fields[opt] = np.ascontiguousarray(a[opt], np.float32).reshape(-1, 3)
+            for opt in ("tet_mu", "tet_lambda"):
+                if a.get(opt) is not None:
+                    fields[opt] = np.ascontiguousarray(a[opt], np.float32)
+            keep.append(fields)
+            d = arr[i]
+            d.num_verts = pos.shape[0]
+            d.num_tets = tets.shape[0]
+            d.positions = ptr(fields["positions"], C.c_float)
+            d.inertia = ptr(fields.get("inertia"), C.c_float)
+            d.velocity = ptr(fields.get("velocity"), C.c_float)
+            d.mass = ptr(fields["mass"], C.c_float)
+            d.is_fixed = ptr(fields["is_fixed"], C.c_uint8)
+            d.tets = ptr(fields["tets"], C.c_int32)
+            d.dminv = ptr(fields["dminv"], C.c_float)
+            d.rest_volume = ptr(fields["rest_volume"], C.c_float)
+            d.tet_mu = ptr(fields.get("tet_mu"), C.c_float)
+            d.tet_lambda = ptr(fields.get("tet_lambda"), C.c_float)
+            d.num_colors = fields["color_off"].size - 1
+            d.color_offsets = ptr(fields["color_off"], C.c_int32)
+            d.color_verts = ptr(fields["color_verts"], C.c_int32)
+            total += pos.shape[0]
+        check(lib.vbd_init_batch(self.h, len(descs), arr), self.h)
+        self.num_verts = total
+        return self
+
+    def info(self) -> A.Info:
+        i = A.Info()
+        check(lib.vbd_get_info(self.h, C.byref(i)), self.h)
+        return i
+
+    # -- transfers
+    def _down(self, fn) -> np.ndarray:
+        out = np.zeros((self.num_verts, 3), np.float32)
+        check(fn(self.h, ptr(out, C.c_float)), self.h)
+        return out
+
+    def _up(self, fn, a):
+        a = np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+        if a.shape[0] != self.num_verts:
+            raise ValueError("vertex count mismatch")
+        check(fn(self.h, ptr(a, C.c_float)), self.h)
+
+    def positions(self) -> np.ndarray:
+        return self._down(lib.vbd_download_positions)
+
+    def velocities(self) -> np.ndarray:
+        return self._down(lib.vbd_download_velocities)
+
+    def inertia(self) -> np.ndarray:
+        return self._down(lib.vbd_download_inertia)
+
+    def set_positions(self, x):
+        self._up(lib.vbd_upload_positions, x)
+
+    def set_inertia(self, y):
+        self._up(lib.vbd_upload_inertia, y)
+
+    def set_velocities(self, v):
+        self._up(lib.vbd_upload_velocities, v)
+
+    def adjacency(self):
+        info = self.info()
+        off = np.zeros(info.num_verts + 1, np.int32)
+        ent = np.zeros(max(4 * info.num_tets, 1), np.int32)
+        check(lib.vbd_get_adjacency(self.h, ptr(off, C.c_int32), ptr(ent, C.c_int32)), self.h)
+        return off, ent[: 4 * info.num_tets]
+
+    def color_lists(self):
+        info = self.info()
+        off = np.zeros(info.num_colors + 1, np.int32)
+        verts = np.zeros(max(info.num_verts, 1), np.int32)
+        check(lib.vbd_get_color_lists(self.h, ptr(off, C.c_int32), ptr(verts, C.c_int32)), self.h)
+        return off, verts[: info.num_verts]
+
+    # -- solve
+    def solve_iteration(self, dt_sqr_recip, mu, lam):
+        check(lib.vbd_solve_iteration(self.h, dt_sqr_recip, mu, lam), self.h)
+
+    def solve(self, iters, dt_sqr_recip, mu, lam, accel=True, rho=0.99, start_from_inertia=True):
+        check(lib.vbd_solve(self.h, iters, dt_sqr_recip, mu, lam, int(bool(accel)), rho, int(bool(start_from_inertia))),
+              self.h)
+
+    def set_params(self, p: VBDParams):
+        cp = p.to_c()
+        check(lib.vbd_set_params(self.h, C.byref(cp)), self.h)
+
+    def step(self, frames: int = 1):
+        check(lib.vbd_step(self.h, frames), self.h)
+
+    def forward_step(self):
+        check(lib.vbd_forward_step(self.h), self.h)
+
+    def update_velocity(self):
+        check(lib.vbd_update_velocity(self.h), self.h)
+
+    def synchronize(self):
+        check(lib.vbd_synchronize(self.h), self.h)
+
+    def set_graph(self, enabled: bool):
+        check(lib.vbd_set_graph_enabled(self.h, int(bool(enabled))), self.h)
+
+    def set_launch_config(self, lanes: int, block: int):
+        check(lib.vbd_set_launch_config(self.h, lanes, block), self.h)
+
+    def autotune(self, iterations: int = 3):
+        l, b, ms = C.c_int32(), C.c_int32(), C.c_float()
+        check(lib.vbd_autotune(self.h, iterations, C.byref(l), C.byref(b), C.byref(ms)), self.h)
+        return l.value, b.value, ms.value
+
+    def time_iterations(self, iterations: int):
+        cms, bms = C.c_double(), C.c_double()
+        cn, bn = C.c_int32(), C.c_int32()
+        check(lib.vbd_time_iterations(self.h, iterations, C.byref(cms), C.byref(cn), C.byref(bms), C.byref(bn)), self.h)
+        return cms.value, cn.value, bms.value, bn.value
+
+    def algorithmic_bytes(self):
+        cb, bb = C.c_double(), C.c_double()
+        check(lib.vbd_algorithmic_bytes(self.h, C.byref(cb), C.byref(bb)), self.h)
+        return cb.value, bb.value
+
+    def total_energy(self, mu: float, lam: float) -> float:
+        e = C.c_double()
+        check(lib.vbd_total_energy(self.h, mu, lam, C.byref(e)), self.h)
+        return e.value
+
+
+class VBD_GPU_Solver:
+    """VBD_GPU_Solver (VBD_GPU_Wrapper.h:7-45) on the gfx950 C ABI."""
+
+    def __init__(self, device: int = 0):
+        self._dev = DeviceMesh(device)
+
+    @property
+    def device_mesh(self) -> DeviceMesh:
+        return self._dev
+
+    def initFromCPU(self, mesh: TetMesh, vertexColors, tetDmInv, tetRestVol):
+        """VBD_GPU_Wrapper.cu:246-315.  Per-tet materials of `mesh` are not uploaded (the
+        reference solve takes scalar mu, lambda per call)."""
+        off, verts = _colors_to_csr(vertexColors)
+        self._dev.init(mesh.mVertPos, mesh.vertexMass, mesh.isFixed, mesh.tets, tetDmInv, tetRestVol, off, verts,
+                       inertia=mesh.mInertia)
+
+    def solveIteration(self, dtSqrReciprocal, mu, lambda_):
+        """VBD_GPU_Wrapper.cu:317-329: one sweep over all colors."""
+        self._dev.solve_iteration(dtSqrReciprocal, mu, lambda_)
+
+    def solveVBD(self, numIterations, dtSqrReciprocal, mu, lambda_, useAcceleration=True, rho=0.99):
+        """VBD_GPU_Wrapper.cu:332-362: x <- y, then iterations with Chebyshev."""
+        self._dev.solve(numIterations, dtSqrReciprocal, mu, lambda_, useAcceleration, rho, start_from_inertia=True)
+
+    def copyPositionsToCPU(self, mesh: TetMesh):
+        """VBD_GPU_Wrapper.cu:364-366."""
+        mesh.mVertPos[:] = self._dev.positions()
+
+    def updateInertiaFromCPU(self, mesh: TetMesh):
+        """VBD_GPU_Wrapper.cu:369-371."""
+        self._dev.set_inertia(mesh.mInertia)
+
+
+class VBDsolver:
+    """VBDsolver (VBD.h:33-140) whose step runs entirely on the device.
+
+    ``step(mesh)`` uploads the mesh state on the first call (and whenever the host arrays
+    were replaced), runs forwardStep / iterations / updateVelocity as one captured hipGraph,
+    then downloads positions, velocities and the inertia target back into ``mesh`` so the
+    host view matches the reference after every call.  ``step_device(n)`` advances n frames
+    without any host transfer.
+    """
+
+    def __init__(self, device: int = 0):
+        self.params = VBDParams()
+        self.vertexColors: List[np.ndarray] = []
+        self.colorOffsets = None
+        self.colorVerts = None
+        self.tetDmInv = None
+        self.tetRestVolume = None
+        self.energyHistory: List[float] = []
+        self._device = device
+        self._dev: Optional[DeviceMesh] = None
+        self._uploaded_ids = None
+
+    def setParams(self, params: VBDParams):
+        self.params = params
+        if self._dev is not None:
+            self._dev.set_params(self._device_params())
+
+    def initializeColoring(self, mesh: TetMesh):
+        """VBD.cpp:23-34 (greedy coloring on the host, GraphColoring.cpp:28-66)."""
+        _, off, verts = A.color_vertices(mesh.numVerts, mesh.tets, mesh.adjOffsets, mesh.adjTets)
+        self.colorOffsets, self.colorVerts = off, verts
+        self.vertexColors = [verts[off[c]: off[c + 1]] for c in range(off.size - 1)]
+
+    def precomputeTetData(self, mesh: TetMesh):
+        """VBD.cpp:36-58."""
+        self.tetDmInv, self.tetRestVolume = A.precompute_tet_data(mesh.mRestPos, mesh.tets)
+
+    def getVertexColors(self):
+        return self.vertexColors
+
+    def getTetDmInv(self):
+        return self.tetDmInv
+
+    def getTetRestVol(self):
+        return self.tetRestVolume
+
+    def _device_params(self) -> VBDParams:
+        return self.params
+
+    def _ensure_device(self, mesh: TetMesh, per_tet: bool):
+        ids = (id(mesh.mVertPos), id(mesh.mVelocity), mesh.numVerts)
+        if self._dev is None:
+            self._dev = DeviceMesh(self._device)
+            mu = mesh.tetMu if per_tet else None
+            lam = mesh.tetLambda if per_tet else None
+            self._dev.init(mesh.mVertPos, mesh.vertexMass, mesh.isFixed, mesh.tets, self.tetDmInv, self.tetRestVolume,
+                           self.colorOffsets, self.colorVerts, inertia=mesh.mInertia, velocity=mesh.mVelocity,
+                           tet_mu=mu, tet_lambda=lam)
+            self._dev.set_params(self._device_params())
+            self._uploaded_ids = ids
+
+    def step(self, mesh: TetMesh, per_tet_material: bool = False):
+        """VBD.cpp:60-101."""
+        if not self.vertexColors:
+            self.initializeColoring(mesh)
+            self.precomputeTetData(mesh)
+        self._ensure_device(mesh, per_tet_material)
+        self._dev.set_positions(mesh.mVertPos)
+        self._dev.set_velocities(mesh.mVelocity)
+        mesh.mVertPrevPos[:] = mesh.mVertPos
+        self._dev.step(1)
+        mesh.mVertPos[:] = self._dev.positions()
+        mesh.mVelocity[:] = self._dev.velocities()
+        mesh.mInertia[:] = self._dev.inertia()
+
+    def step_device(self, frames: int = 1):
+        self._dev.step(frames)
+
+    def simulate(self, mesh: TetMesh, numFrames: int, callback: Optional[Callable] = None,
+                 per_tet_material: bool = False):
+        """VBD.cpp:313-330."""
+        self.energyHistory.clear()
+        for f in range(numFrames):
+            self.step(mesh, per_tet_material)
+            if callback is not None:
+                callback(f, mesh)
+
+    def computeTotalEnergy(self, mesh: TetMesh) -> float:
+        """VBD.cpp:258-288 as a device reduction (|det F| for the singular-value product)."""
+        if self._dev is None:
+            self._ensure_device(mesh, False)
+        self._dev.set_positions(mesh.mVertPos)
+        return self._dev.total_energy(self.params.mu, self.params.lambda_)
+
+    @property
+    def device_mesh(self) -> Optional[DeviceMesh]:
+        return self._dev
+
+
+def prepare_host(m, mu=None, lam=None):
+    """Mesh generator output -> (TetMesh, color offsets, color verts, DmInv, V0) using the
+    product's host C++ helpers (the same preparation VBDsolver does before its first step)."""
+    t = mesh_from_generator(m, mu, lam)
+    _, off, verts = A.color_vertices(t.numVerts, t.tets, t.adjOffsets, t.adjTets)
+    dminv, vol = A.precompute_tet_data(t.mRestPos, t.tets)
+    return t, off, verts, dminv, vol
+
+
+def upload(m, device: int = 0, params: Optional[VBDParams] = None, per_tet: Optional[bool] = None) -> DeviceMesh:
+    """Generator mesh -> DeviceMesh with params set (host prep by the product C++ helpers)."""
+    t, off, verts, dminv, vol = prepare_host(m)
+    use_mat = (m.tet_mu is not None) if per_tet is None else per_tet
+    d = DeviceMesh(device).init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, off, verts,
+                                velocity=t.mVelocity, tet_mu=t.tetMu if use_mat else None,
+                                tet_lambda=t.tetLambda if use_mat else None)
+    if params is not None:
+        d.set_params(params)
+    return d
