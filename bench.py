@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--unroll", type=int, default=1)
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only the copy probe")
     args = ap.parse_args()
     if args.copy_probe:
@@ -147,12 +148,12 @@ def main():
         f"pairs={info.num_pairs} device={info.device_bytes / 2**30:.2f} GiB setup {time.time() - t0:.1f}s")
     tune = None
     if args.lanes:
-        d.set_launch_config(args.lanes, args.block)
+        d.set_launch_config(args.lanes, args.block, args.unroll)
     elif not args.no_autotune:
         d.set_positions(m.x0)
         tune = d.autotune(3)
         d.set_positions(m.x0)
-        log(f"[bench] autotune -> lanes={tune[0]} block={tune[1]} ({tune[2]:.3f} ms/sweep)")
+        log(f"[bench] autotune -> lanes={tune[0]} pairs/lane={tune[1]} block={tune[2]} ({tune[3]:.3f} ms/sweep)")
     info = d.info()
 
     # warmup
@@ -197,7 +198,8 @@ def main():
         "config": {"workload": f"cube{args.n} noisy Kuhn-split cube, {info.num_verts} verts, {info.num_tets} tets, "
                                f"{args.iterations} iters/step, Chebyshev 0.95, h=1/60, bottom face pinned",
                    "iterations_per_step": args.iterations, "colors": info.num_colors,
-                   "lanes_per_vertex": info.lanes_per_vertex, "block_size": info.block_size},
+                   "lanes_per_vertex": info.lanes_per_vertex, "pairs_per_lane": info.pairs_per_lane,
+                   "block_size": info.block_size},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_color_solve", "alg_bytes_per_launch": round(per_launch_bytes),

@@ -98,6 +98,7 @@ typedef struct vbd_info {
     int32_t max_valence;
     int32_t per_tet_material;
     int32_t lanes_per_vertex;   /* current launch configuration of the color kernel */
+    int32_t pairs_per_lane;
     int32_t block_size;
     int64_t device_bytes;       /* bytes of device memory the handle holds */
 } vbd_info;
@@ -150,10 +151,15 @@ int vbd_synchronize(vbd_solver* h);
 int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
 
 /* ---- launch configuration, autotuning and timing (no reference counterpart) ------- */
-int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size);
-/* Times each candidate (lanes_per_vertex x block_size) over `iterations` sweeps on the
- * uploaded mesh, restores the positions, keeps the fastest; writes it to the outputs. */
-int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_block, float* best_ms);
+/* Color-kernel mapping: lanes_per_vertex lanes share one vertex; each lane issues the loads
+ * of pairs_per_lane incident-tet records per pass.  Supported: lanes 4/8/16 with 1-3 pairs
+ * per lane, lanes 32/64 with 1; block_size 64/128/192/256 (a multiple of lanes). */
+int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane);
+/* Autotuner keyed on the valence histogram: on the first mesh with a given key it times each
+ * candidate mapping over `iterations` sweeps, restores the positions and keeps the fastest;
+ * later meshes with the same key reuse the choice without timing (*best_ms = -1). */
+int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_unroll, int32_t* best_block,
+                 float* best_ms);
 /* Runs `iterations` plain sweeps (+ blends if use_acceleration in params) with HIP events
  * around every launch on the handle's stream.  Outputs: total color-kernel milliseconds,
  * number of color launches, total blend milliseconds, number of blend launches. */

@@ -32,10 +32,10 @@ def oracle_params(m, iterations, accel=True, rho=0.95, contact=None, mode=O.MODE
                          contact=contact, mode=mode, init_from_inertia=init_from_inertia)
 
 
-def run_gpu_frames(m, params, frames, lanes=None, block=256, graph=True):
+def run_gpu_frames(m, params, frames, lanes=None, block=256, graph=True, unroll=1):
     d = upload(m, params=params)
     if lanes is not None:
-        d.set_launch_config(lanes, block)
+        d.set_launch_config(lanes, block, unroll)
     d.set_graph(graph)
     out = []
     for _ in range(frames):
@@ -154,15 +154,24 @@ def test_graph_replay_is_bitwise_identical_to_eager():
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("lanes,block", [(4, 64), (8, 128), (16, 256), (32, 256), (64, 256), (8, 192)])
-def test_launch_configurations_agree(lanes, block):
+@pytest.mark.parametrize("lanes,block,unroll", [(4, 64, 1), (4, 128, 3), (8, 128, 1), (8, 256, 2), (8, 192, 3),
+                                                (16, 256, 1), (16, 64, 2), (32, 256, 1), (64, 256, 1)])
+def test_launch_configurations_agree(lanes, block, unroll):
     m = meshgen.noisy_cube(12, seed=3)
     prep = O.prepare(m)
     _, _, of, _ = O.simulate(prep, oracle_params(m, 20), m.x0, None, 3)
-    got, _ = run_gpu_frames(m, params_for(m, 20), 3, lanes=lanes, block=block)
+    got, _ = run_gpu_frames(m, params_for(m, 20), 3, lanes=lanes, block=block, unroll=unroll)
     rel = max(relmax(got[f], of[f]) for f in range(3))
-    print(f"cube12 lanes={lanes} block={block}: {rel:.3e}", flush=True)
+    print(f"cube12 lanes={lanes} block={block} unroll={unroll}: {rel:.3e}", flush=True)
     assert rel <= TOL
+
+
+def test_unsupported_launch_configuration_is_rejected():
+    from vbd_tetmesh_amd import _capi as A
+    d = upload(meshgen.beam(3, 1, 1))
+    for bad in [(3, 64, 1), (32, 256, 2), (8, 96, 1), (16, 512, 1), (8, 64, 4)]:
+        with pytest.raises(A.VBDError):
+            d.set_launch_config(*bad)
 
 
 def test_noisy_cube_32_matches_oracle():
@@ -247,12 +256,12 @@ def test_batch_equals_per_mesh_runs_bitwise():
         descs.append(kw)
         d = DeviceMesh().init(**kw)
         d.set_params(params)
-        d.set_launch_config(8, 256)
+        d.set_launch_config(8, 256, 1)
         d.step(4)
         singles.append(d.positions())
     b = DeviceMesh().init_batch(descs)
     b.set_params(params)
-    b.set_launch_config(8, 256)
+    b.set_launch_config(8, 256, 1)
     b.step(4)
     got = b.positions()
     ref = np.concatenate(singles)

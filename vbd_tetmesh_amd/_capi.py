@@ -76,6 +76,7 @@ class Info(C.Structure):
         ("max_valence", C.c_int32),
         ("per_tet_material", C.c_int32),
         ("lanes_per_vertex", C.c_int32),
+        ("pairs_per_lane", C.c_int32),
         ("block_size", C.c_int32),
         ("device_bytes", C.c_int64),
     ]
@@ -108,8 +109,8 @@ SIGNATURES = [
     ("vbd_step", C.c_int, [H, C.c_int32]),
     ("vbd_synchronize", C.c_int, [H]),
     ("vbd_set_graph_enabled", C.c_int, [H, C.c_int32]),
-    ("vbd_set_launch_config", C.c_int, [H, C.c_int32, C.c_int32]),
-    ("vbd_autotune", C.c_int, [H, C.c_int32, i32p, i32p, f32p]),
+    ("vbd_set_launch_config", C.c_int, [H, C.c_int32, C.c_int32, C.c_int32]),
+    ("vbd_autotune", C.c_int, [H, C.c_int32, i32p, i32p, i32p, f32p]),
     ("vbd_time_iterations", C.c_int, [H, C.c_int32, f64p, i32p, f64p, i32p]),
     ("vbd_algorithmic_bytes", C.c_int, [H, f64p, f64p]),
     ("vbd_mesh_initialize", C.c_int, [C.c_int32, f32p, C.c_int32, i32p, C.c_float, f32p, f32p, i32p, i32p]),

@@ -27,6 +27,7 @@ struct vbd_solver {
     vbd_step_params params{};
     bool paramsSet = false;
     int lpv = 8;
+    int unroll = 1;
     int block = 256;
     bool graphEnabled = true;
     hipGraphExec_t graphExec = nullptr;
@@ -104,7 +105,7 @@ bool finite_array(const float* a, size_t n) {
 // One Gauss-Seidel sweep over all colors.
 hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const ContactArgs& ct, f4* save) {
     for (int c = 0; c < h->m.nColors; ++c) {
-        hipError_t e = launch_color(h->m, c, h->lpv, h->block, sdt, mu, lambda, ct, save, h->stream);
+        hipError_t e = launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -201,8 +202,11 @@ std::string tune_key(const DevMesh& m, bool contact) {
     return buf;
 }
 
+struct TuneEntry {
+    int l, u, b;
+};
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<int, int>> g_tune_cache;
+std::map<std::string, TuneEntry> g_tune_cache;
 
 // Launch configuration picked from the valence histogram alone: the smallest lane count
 // that covers the mean valence in at most three strides.
@@ -271,6 +275,7 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
     const int nV = d->num_verts, nT = d->num_tets, nC = d->num_colors;
     if (nV < 0 || nT < 0 || nC < 0) return fail(h, VBD_ERR_ARG, "negative size");
     if ((int64_t)nT * 4 >= (int64_t)1 << 31) return fail(h, VBD_ERR_ARG, "4*num_tets must fit int32");
+    if (nV >= (1 << 30)) return fail(h, VBD_ERR_ARG, "num_verts must be below 2^30");
     if ((nV > 0 && (!d->positions || !d->mass || !d->is_fixed)) ||
         (nT > 0 && (!d->tets || !d->dminv || !d->rest_volume)) || !d->color_offsets || (nV > 0 && !d->color_verts))
         return fail(h, VBD_ERR_ARG, "required mesh array is null");
@@ -348,6 +353,7 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
                                      (const float*)s_y.p, (const float*)s_v.p, (const float*)s_m.p, h->stream, err);
     if (e != hipSuccess) return fail(h, e == hipErrorOutOfMemory ? VBD_ERR_ALLOC : VBD_ERR_HIP, "mesh build: " + err);
     h->lpv = heuristic_lpv(h->m);
+    h->unroll = 1;
     h->block = 256;
     h->meshVertOffsets.assign({0, nV});
     h->inited = true;
@@ -455,6 +461,7 @@ int vbd_get_info(const vbd_solver* h, vbd_info* info) {
     info->max_valence = h->m.maxValence;
     info->per_tet_material = h->m.perTetMat;
     info->lanes_per_vertex = h->lpv;
+    info->pairs_per_lane = h->unroll;
     info->block_size = h->block;
     info->device_bytes = h->m.bytes;
     return VBD_OK;
@@ -633,14 +640,14 @@ int vbd_synchronize(vbd_solver* h) {
     return VBD_OK;
 }
 
-int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size) {
+int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane) {
     if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
-    const int l = lanes_per_vertex;
-    if (l != 4 && l != 8 && l != 16 && l != 32 && l != 64) return fail(h, VBD_ERR_ARG, "lanes_per_vertex must be 4..64, power of 2");
-    if (block_size != 64 && block_size != 128 && block_size != 192 && block_size != 256)
-        return fail(h, VBD_ERR_ARG, "block_size must be 64, 128, 192 or 256");
-    if (block_size % l) return fail(h, VBD_ERR_ARG, "block_size must be a multiple of lanes_per_vertex");
-    h->lpv = l;
+    if (!launch_config_supported(lanes_per_vertex, pairs_per_lane, block_size) || block_size % lanes_per_vertex)
+        return fail(h, VBD_ERR_ARG,
+                    "unsupported launch configuration: lanes_per_vertex 4/8/16 with pairs_per_lane 1-3, or 32/64 with "
+                    "pairs_per_lane 1; block_size 64/128/192/256");
+    h->lpv = lanes_per_vertex;
+    h->unroll = pairs_per_lane;
     h->block = block_size;
     DeviceGuard g(h->device);
     drop_graph(h);
@@ -683,7 +690,7 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
             if (h->m.colorSlotOff[c + 1] == h->m.colorSlotOff[c]) continue;
             if ((e = mk(false)) != hipSuccess) break;
             (void)hipEventRecord(evs.back().a, h->stream);
-            e = launch_color(h->m, c, h->lpv, h->block, sdt, mu, lam, ct, save, h->stream);
+            e = launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lam, ct, save, h->stream);
             (void)hipEventRecord(evs.back().b, h->stream);
         }
         if (accel && e == hipSuccess) {
@@ -715,7 +722,8 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
     return VBD_OK;
 }
 
-int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_block, float* best_ms) {
+int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_unroll, int32_t* best_block,
+                 float* best_ms) {
     int rc = check_ready(h);
     if (rc) return rc;
     if (iterations <= 0) return fail(h, VBD_ERR_ARG, "iterations must be positive");
@@ -725,10 +733,12 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            h->lpv = it->second.first;
-            h->block = it->second.second;
+            h->lpv = it->second.l;
+            h->unroll = it->second.u;
+            h->block = it->second.b;
             drop_graph(h);
             if (best_lanes) *best_lanes = h->lpv;
+            if (best_unroll) *best_unroll = h->unroll;
             if (best_block) *best_block = h->block;
             if (best_ms) *best_ms = -1.f;
             return VBD_OK;
@@ -738,16 +748,20 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     f4* keep = nullptr;
     HIP_CHECK(h, hipMalloc(&keep, sizeof(f4) * (h->m.nV ? h->m.nV : 1)));
     hipError_t e = launch_copy_f4(h->m.nV, h->m.pos, keep, h->stream);
-    const int cand_l[] = {4, 8, 16, 32};
+    struct Cand {
+        int l, u;
+    };
+    const Cand cands[] = {{4, 1}, {4, 2}, {4, 3}, {8, 1}, {8, 2}, {8, 3}, {16, 1}, {16, 2}, {32, 1}};
     const int cand_b[] = {64, 128, 256};
-    int bl = h->lpv, bb = h->block;
+    int bl = h->lpv, bu = h->unroll, bb = h->block;
     double best = 1e30;
-    const int ol = h->lpv, ob = h->block;
-    for (int l : cand_l) {
+    const int ol = h->lpv, ou = h->unroll, ob = h->block;
+    for (const Cand& cd : cands) {
         for (int b : cand_b) {
             if (e != hipSuccess) break;
-            if (b % l) continue;
-            h->lpv = l;
+            if (b % cd.l) continue;
+            h->lpv = cd.l;
+            h->unroll = cd.u;
             h->block = b;
             double cms = 0, bms = 0;
             int cn = 0, bn = 0;
@@ -756,6 +770,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
             if (rc == VBD_OK) rc = vbd_time_iterations(h, iterations, &cms, &cn, &bms, &bn);
             if (rc != VBD_OK) {
                 h->lpv = ol;
+                h->unroll = ou;
                 h->block = ob;
                 (void)launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
                 (void)hipStreamSynchronize(h->stream);
@@ -764,7 +779,8 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
             }
             if (cms < best) {
                 best = cms;
-                bl = l;
+                bl = cd.l;
+                bu = cd.u;
                 bb = b;
             }
             e = launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
@@ -774,13 +790,15 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     (void)hipFree(keep);
     if (e != hipSuccess) return hip_fail(h, "autotune", e);
     h->lpv = bl;
+    h->unroll = bu;
     h->block = bb;
     drop_graph(h);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[key] = {bl, bb};
+        g_tune_cache[key] = {bl, bu, bb};
     }
     if (best_lanes) *best_lanes = bl;
+    if (best_unroll) *best_unroll = bu;
     if (best_block) *best_block = bb;
     if (best_ms) *best_ms = (float)(best / iterations);
     return VBD_OK;

@@ -38,10 +38,11 @@ struct DevMesh {
     int* row = nullptr;     // nFree+1 pair offsets
     int* perm = nullptr;    // nV slot -> caller vertex id
     int* inv = nullptr;     // nV caller vertex id -> slot
-    i4* ptet = nullptr;     // nPairs  the tet's 4 corner slots in tet corner order
+    i4* ptet = nullptr;     // nPairs  other 3 corner slots in tet order (corner k of the vertex in the
+                            //         top 2 bits of .x), V0 bits in .w
     f4* pa = nullptr;       // nPairs  DmInv(0,0) (0,1) (0,2) (1,0)
     f4* pb = nullptr;       // nPairs  DmInv(1,1) (1,2) (2,0) (2,1)
-    f2* pc = nullptr;       // nPairs  DmInv(2,2), V0
+    float* pc = nullptr;    // nPairs  DmInv(2,2)
     f2* pmat = nullptr;     // nPairs  mu, lambda (per-tet materials only)
     // GPU-built structures kept in the caller's numbering (exposed for verification)
     int* adjOff = nullptr;    // nV+1
@@ -64,7 +65,8 @@ struct ContactArgs {
 };
 
 // ---- launchers (vbd_kernels.hip); all return hipError_t of the launch -----------------
-hipError_t launch_color(const DevMesh& m, int color, int lpv, int block, float sdt, float mu, float lambda,
+bool launch_config_supported(int lpv, int unroll, int block);
+hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int block, float sdt, float mu, float lambda,
                         const ContactArgs& ct, f4* save, hipStream_t s);
 hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
 hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s);

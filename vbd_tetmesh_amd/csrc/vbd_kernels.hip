@@ -47,29 +47,110 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 // ---------------------------------------------------------------------------------------
-// Per-color vertex solve.
+// Per-(vertex, tet) contribution.
 //
-// One group of LPV lanes per free vertex of the color.  Lanes stride over the vertex's
-// incident-tet records; each record gives the tet's four corner slots (corner order of the
-// tet), DmInv (row-major) and V0.  Per record, with q = shape vector of the vertex's corner
-// (q_0 = -(row0+row1+row2) of DmInv, q_k = row k-1, i.e. row k of Q = Ps*DmInv,
-// Energy.cpp:101-106):
+// A pair record holds the tet's three other corner slots, the vertex's own corner index k
+// (top two bits of the first slot), V0, and DmInv row-major.  The vertex's own position xs
+// is loaded once per vertex.  With q = shape vector of corner k (q_0 = -(row0+row1+row2)
+// of DmInv, q_k = row k-1, i.e. row k of Q = Ps*DmInv, Energy.cpp:101-106):
 //   F = Ds*DmInv, J = det F, C = cof F, P = mu(F - C) + lambda(J-1) C   (Energy.cpp:93-97)
 //   g = V P q,  H = V [mu |q|^2 I + lambda (C q)(C q)^T]
 // H is exactly the (k,k) 3x3 block of the 12x12 Hessian of Energy.cpp:115-154: the
-// (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.  The group sums the
-// 6 distinct H entries and 3 force entries, the last lane adds the inertia term
-// m/h^2 (VBD.cpp:170-173), optional penalty contact, and applies the adjugate 3x3 solve
+// (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.
+// ---------------------------------------------------------------------------------------
+struct Acc {
+    float h00, h01, h02, h11, h12, h22, r0, r1, r2;
+};
+
+constexpr int SLOT_MASK = 0x3FFFFFFF;
+
+__device__ __forceinline__ void pair_accumulate(const i4 t, const f4 a, const f4 b, const float D22, const float tmu,
+                                                const float tlam, const f4 xs, const f4 ya, const f4 yb,
+                                                const f4 yc, Acc& A) {
+    const int k = (int)((unsigned)t.x >> 30);
+    const float V = __int_as_float(t.w);
+    // corners in tet order: the vertex itself at k, the other three in order
+    const f4 x0 = k == 0 ? xs : ya;
+    const f4 x1 = k == 0 ? ya : (k == 1 ? xs : yb);
+    const f4 x2 = k <= 1 ? yb : (k == 2 ? xs : yc);
+    const f4 x3 = k == 3 ? xs : yc;
+    const float D00 = a.x, D01 = a.y, D02 = a.z, D10 = a.w;
+    const float D11 = b.x, D12 = b.y, D20 = b.z, D21 = b.w;
+    // Ds columns (Energy.cpp:11-13)
+    const float e1x = x1.x - x0.x, e1y = x1.y - x0.y, e1z = x1.z - x0.z;
+    const float e2x = x2.x - x0.x, e2y = x2.y - x0.y, e2z = x2.z - x0.z;
+    const float e3x = x3.x - x0.x, e3y = x3.y - x0.y, e3z = x3.z - x0.z;
+    // F = Ds * DmInv
+    const float F00 = e1x * D00 + e2x * D10 + e3x * D20;
+    const float F01 = e1x * D01 + e2x * D11 + e3x * D21;
+    const float F02 = e1x * D02 + e2x * D12 + e3x * D22;
+    const float F10 = e1y * D00 + e2y * D10 + e3y * D20;
+    const float F11 = e1y * D01 + e2y * D11 + e3y * D21;
+    const float F12 = e1y * D02 + e2y * D12 + e3y * D22;
+    const float F20 = e1z * D00 + e2z * D10 + e3z * D20;
+    const float F21 = e1z * D01 + e2z * D11 + e3z * D21;
+    const float F22 = e1z * D02 + e2z * D12 + e3z * D22;
+    // cofactor (Energy.cpp:61-73) and J
+    const float C00 = F11 * F22 - F12 * F21, C01 = F12 * F20 - F10 * F22, C02 = F10 * F21 - F11 * F20;
+    const float C10 = F21 * F02 - F22 * F01, C11 = F22 * F00 - F20 * F02, C12 = F20 * F01 - F21 * F00;
+    const float C20 = F01 * F12 - F02 * F11, C21 = F02 * F10 - F00 * F12, C22 = F00 * F11 - F01 * F10;
+    const float J = F00 * C00 + F01 * C01 + F02 * C02;
+    const float aj = tlam * (J - 1.0f);
+    float q0, q1, q2;
+    if (k == 0) {
+        q0 = -(D00 + D10 + D20);
+        q1 = -(D01 + D11 + D21);
+        q2 = -(D02 + D12 + D22);
+    } else if (k == 1) {
+        q0 = D00; q1 = D01; q2 = D02;
+    } else if (k == 2) {
+        q0 = D10; q1 = D11; q2 = D12;
+    } else {
+        q0 = D20; q1 = D21; q2 = D22;
+    }
+    // P q = mu (F q - C q) + aj C q
+    const float Cq0 = C00 * q0 + C01 * q1 + C02 * q2;
+    const float Cq1 = C10 * q0 + C11 * q1 + C12 * q2;
+    const float Cq2 = C20 * q0 + C21 * q1 + C22 * q2;
+    const float Fq0 = F00 * q0 + F01 * q1 + F02 * q2;
+    const float Fq1 = F10 * q0 + F11 * q1 + F12 * q2;
+    const float Fq2 = F20 * q0 + F21 * q1 + F22 * q2;
+    const float g0 = tmu * (Fq0 - Cq0) + aj * Cq0;
+    const float g1 = tmu * (Fq1 - Cq1) + aj * Cq1;
+    const float g2 = tmu * (Fq2 - Cq2) + aj * Cq2;
+    const float qq = q0 * q0 + q1 * q1 + q2 * q2;
+    const float vm = V * tmu * qq;
+    const float vl = V * tlam;
+    const float vc0 = vl * Cq0, vc1 = vl * Cq1, vc2 = vl * Cq2;
+    A.h00 += vm + vc0 * Cq0;
+    A.h01 += vc0 * Cq1;
+    A.h02 += vc0 * Cq2;
+    A.h11 += vm + vc1 * Cq1;
+    A.h12 += vc1 * Cq2;
+    A.h22 += vm + vc2 * Cq2;
+    A.r0 -= V * g0;
+    A.r1 -= V * g1;
+    A.r2 -= V * g2;
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-color vertex solve.
+//
+// One group of LPV lanes per free vertex of the color; lane j of the group takes the
+// vertex's records j, j+LPV, ...; each pass issues the loads of U records (and then their
+// neighbour gathers) before any arithmetic so more memory requests are in flight.  The group
+// sums the 6 distinct H entries and 3 force entries (DPP), the last lane adds the inertia
+// term m/h^2 (VBD.cpp:170-173), optional penalty contact, and applies the adjugate 3x3 solve
 // when |det H| > 1e-8 (VBD.cpp:141-153, Types.h:85-112).  With SAVE the vertex's position
 // before its update (its position at the start of the sweep) is written to `save` for the
 // Chebyshev blend.
 // ---------------------------------------------------------------------------------------
-template <int LPV, bool MAT, bool CONTACT, bool SAVE>
+template <int LPV, int U, bool MAT, bool CONTACT, bool SAVE>
 __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* __restrict__ pos,
                                                      const f4* __restrict__ inert, const float* __restrict__ mass,
                                                      const int* __restrict__ row, const i4* __restrict__ ptet,
                                                      const f4* __restrict__ pa, const f4* __restrict__ pb,
-                                                     const f2* __restrict__ pc, const f2* __restrict__ pmat,
+                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
                                                      f4* __restrict__ save, float mu, float lambda, float sdt,
                                                      ContactArgs ct) {
     const int gpb = blockDim.x / LPV;
@@ -78,91 +159,55 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* _
     const bool valid = g < count;
     const int s = slot0 + g;
 
-    float h00 = 0.f, h01 = 0.f, h02 = 0.f, h11 = 0.f, h12 = 0.f, h22 = 0.f;
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    Acc A = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    f4 xs = {0.f, 0.f, 0.f, 0.f};
     if (valid) {
+        xs = pos[s];
         const int p0 = row[s], p1 = row[s + 1];
-        for (int p = p0 + lane; p < p1; p += LPV) {
-            const i4 t = __builtin_nontemporal_load(&ptet[p]);
-            const f4 a = __builtin_nontemporal_load(&pa[p]);
-            const f4 b = __builtin_nontemporal_load(&pb[p]);
-            const f2 c = __builtin_nontemporal_load(&pc[p]);
-            float tmu = mu, tlam = lambda;
-            if constexpr (MAT) {
-                const f2 mm = __builtin_nontemporal_load(&pmat[p]);
-                tmu = mm.x;
-                tlam = mm.y;
+        for (int base = p0 + lane; base < p1; base += LPV * U) {
+            i4 t[U];
+            f4 a[U], b[U];
+            float c[U];
+            f2 mm[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int p = base + u * LPV;
+                if (p < p1) {
+                    t[u] = __builtin_nontemporal_load(&ptet[p]);
+                    a[u] = __builtin_nontemporal_load(&pa[p]);
+                    b[u] = __builtin_nontemporal_load(&pb[p]);
+                    c[u] = __builtin_nontemporal_load(&pc[p]);
+                    if constexpr (MAT) mm[u] = __builtin_nontemporal_load(&pmat[p]);
+                }
             }
-            const f4 x0 = pos[t.x], x1 = pos[t.y], x2 = pos[t.z], x3 = pos[t.w];
-            const float D00 = a.x, D01 = a.y, D02 = a.z, D10 = a.w;
-            const float D11 = b.x, D12 = b.y, D20 = b.z, D21 = b.w;
-            const float D22 = c.x, V = c.y;
-            // Ds columns (Energy.cpp:11-13)
-            const float e1x = x1.x - x0.x, e1y = x1.y - x0.y, e1z = x1.z - x0.z;
-            const float e2x = x2.x - x0.x, e2y = x2.y - x0.y, e2z = x2.z - x0.z;
-            const float e3x = x3.x - x0.x, e3y = x3.y - x0.y, e3z = x3.z - x0.z;
-            // F = Ds * DmInv
-            const float F00 = e1x * D00 + e2x * D10 + e3x * D20;
-            const float F01 = e1x * D01 + e2x * D11 + e3x * D21;
-            const float F02 = e1x * D02 + e2x * D12 + e3x * D22;
-            const float F10 = e1y * D00 + e2y * D10 + e3y * D20;
-            const float F11 = e1y * D01 + e2y * D11 + e3y * D21;
-            const float F12 = e1y * D02 + e2y * D12 + e3y * D22;
-            const float F20 = e1z * D00 + e2z * D10 + e3z * D20;
-            const float F21 = e1z * D01 + e2z * D11 + e3z * D21;
-            const float F22 = e1z * D02 + e2z * D12 + e3z * D22;
-            // cofactor (Energy.cpp:61-73) and J
-            const float C00 = F11 * F22 - F12 * F21, C01 = F12 * F20 - F10 * F22, C02 = F10 * F21 - F11 * F20;
-            const float C10 = F21 * F02 - F22 * F01, C11 = F22 * F00 - F20 * F02, C12 = F20 * F01 - F21 * F00;
-            const float C20 = F01 * F12 - F02 * F11, C21 = F02 * F10 - F00 * F12, C22 = F00 * F11 - F01 * F10;
-            const float J = F00 * C00 + F01 * C01 + F02 * C02;
-            const float aj = tlam * (J - 1.0f);
-            // corner of this vertex in the tet and its shape vector q
-            const int k = (t.x == s) ? 0 : (t.y == s) ? 1 : (t.z == s) ? 2 : 3;
-            float q0, q1, q2;
-            if (k == 0) {
-                q0 = -(D00 + D10 + D20);
-                q1 = -(D01 + D11 + D21);
-                q2 = -(D02 + D12 + D22);
-            } else if (k == 1) {
-                q0 = D00; q1 = D01; q2 = D02;
-            } else if (k == 2) {
-                q0 = D10; q1 = D11; q2 = D12;
-            } else {
-                q0 = D20; q1 = D21; q2 = D22;
+            f4 ya[U], yb[U], yc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (base + u * LPV < p1) {
+                    ya[u] = pos[t[u].x & SLOT_MASK];
+                    yb[u] = pos[t[u].y];
+                    yc[u] = pos[t[u].z];
+                }
             }
-            // P q = mu (F q - C q) + aj C q
-            const float Cq0 = C00 * q0 + C01 * q1 + C02 * q2;
-            const float Cq1 = C10 * q0 + C11 * q1 + C12 * q2;
-            const float Cq2 = C20 * q0 + C21 * q1 + C22 * q2;
-            const float Fq0 = F00 * q0 + F01 * q1 + F02 * q2;
-            const float Fq1 = F10 * q0 + F11 * q1 + F12 * q2;
-            const float Fq2 = F20 * q0 + F21 * q1 + F22 * q2;
-            const float g0 = tmu * (Fq0 - Cq0) + aj * Cq0;
-            const float g1 = tmu * (Fq1 - Cq1) + aj * Cq1;
-            const float g2 = tmu * (Fq2 - Cq2) + aj * Cq2;
-            const float qq = q0 * q0 + q1 * q1 + q2 * q2;
-            const float vm = V * tmu * qq;
-            const float vl = V * tlam;
-            const float vc0 = vl * Cq0, vc1 = vl * Cq1, vc2 = vl * Cq2;
-            h00 += vm + vc0 * Cq0;
-            h01 += vc0 * Cq1;
-            h02 += vc0 * Cq2;
-            h11 += vm + vc1 * Cq1;
-            h12 += vc1 * Cq2;
-            h22 += vm + vc2 * Cq2;
-            r0 -= V * g0;
-            r1 -= V * g1;
-            r2 -= V * g2;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (base + u * LPV < p1) {
+                    float tmu = mu, tlam = lambda;
+                    if constexpr (MAT) {
+                        tmu = mm[u].x;
+                        tlam = mm[u].y;
+                    }
+                    pair_accumulate(t[u], a[u], b[u], c[u], tmu, tlam, xs, ya[u], yb[u], yc[u], A);
+                }
+            }
         }
     }
-    h00 = gsum<LPV>(h00); h01 = gsum<LPV>(h01); h02 = gsum<LPV>(h02);
-    h11 = gsum<LPV>(h11); h12 = gsum<LPV>(h12); h22 = gsum<LPV>(h22);
-    r0 = gsum<LPV>(r0); r1 = gsum<LPV>(r1); r2 = gsum<LPV>(r2);
+    float h00 = gsum<LPV>(A.h00), h01 = gsum<LPV>(A.h01), h02 = gsum<LPV>(A.h02);
+    float h11 = gsum<LPV>(A.h11), h12 = gsum<LPV>(A.h12), h22 = gsum<LPV>(A.h22);
+    float r0 = gsum<LPV>(A.r0), r1 = gsum<LPV>(A.r1), r2 = gsum<LPV>(A.r2);
     if (!valid || lane != LPV - 1) return;
 
     const float m = mass[s];
-    const f4 xs = pos[s];
     const f4 ys = inert[s];
     const float ms = m * sdt;
     h00 += ms; h11 += ms; h22 += ms;
@@ -192,7 +237,7 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* _
     pos[s] = xn;
 }
 
-template <int LPV, bool MAT, bool CONTACT, bool SAVE>
+template <int LPV, int U, bool MAT, bool CONTACT, bool SAVE>
 static hipError_t launch_color_t(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
                                  const ContactArgs& ct, f4* save, hipStream_t s) {
     const int slot0 = m.colorSlotOff[color];
@@ -201,17 +246,17 @@ static hipError_t launch_color_t(const DevMesh& m, int color, int block, float s
     const int gpb = block / LPV;
     int nb = (count + gpb - 1) / gpb;
     nb = (nb + 7) & ~7;
-    hipLaunchKernelGGL((k_color_solve<LPV, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.pos,
+    hipLaunchKernelGGL((k_color_solve<LPV, U, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.pos,
                        m.inert, m.mass, m.row, m.ptet, m.pa, m.pb, m.pc, m.pmat, save, mu, lambda, sdt, ct);
     return hipGetLastError();
 }
 
-template <int LPV>
-static hipError_t launch_color_l(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
-                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+template <int LPV, int U>
+static hipError_t launch_color_lu(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                  const ContactArgs& ct, f4* save, hipStream_t s) {
     const bool mat = m.perTetMat, con = ct.on != 0, sv = save != nullptr;
 #define VBD_DISPATCH(MT, CT, SV) \
-    if (mat == MT && con == CT && sv == SV) return launch_color_t<LPV, MT, CT, SV>(m, color, block, sdt, mu, lambda, ct, save, s);
+    if (mat == MT && con == CT && sv == SV) return launch_color_t<LPV, U, MT, CT, SV>(m, color, block, sdt, mu, lambda, ct, save, s);
     VBD_DISPATCH(false, false, false) VBD_DISPATCH(false, false, true)
     VBD_DISPATCH(false, true, false) VBD_DISPATCH(false, true, true)
     VBD_DISPATCH(true, false, false) VBD_DISPATCH(true, false, true)
@@ -220,17 +265,24 @@ static hipError_t launch_color_l(const DevMesh& m, int color, int block, float s
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_color(const DevMesh& m, int color, int lpv, int block, float sdt, float mu, float lambda,
+bool launch_config_supported(int lpv, int unroll, int block) {
+    if (block < 64 || block > 256 || (block % 64) != 0) return false;
+    if (lpv == 4 || lpv == 8 || lpv == 16) return unroll >= 1 && unroll <= 3;
+    if (lpv == 32 || lpv == 64) return unroll == 1;
+    return false;
+}
+
+hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int block, float sdt, float mu, float lambda,
                         const ContactArgs& ct, f4* save, hipStream_t s) {
-    if (block < 64 || block > 256 || (block % 64) != 0) return hipErrorInvalidValue;
-    switch (lpv) {
-        case 4: return launch_color_l<4>(m, color, block, sdt, mu, lambda, ct, save, s);
-        case 8: return launch_color_l<8>(m, color, block, sdt, mu, lambda, ct, save, s);
-        case 16: return launch_color_l<16>(m, color, block, sdt, mu, lambda, ct, save, s);
-        case 32: return launch_color_l<32>(m, color, block, sdt, mu, lambda, ct, save, s);
-        case 64: return launch_color_l<64>(m, color, block, sdt, mu, lambda, ct, save, s);
-        default: return hipErrorInvalidValue;
-    }
+    if (!launch_config_supported(lpv, unroll, block)) return hipErrorInvalidValue;
+#define VBD_LU(L, UU) \
+    if (lpv == L && unroll == UU) return launch_color_lu<L, UU>(m, color, block, sdt, mu, lambda, ct, save, s);
+    VBD_LU(4, 1) VBD_LU(4, 2) VBD_LU(4, 3)
+    VBD_LU(8, 1) VBD_LU(8, 2) VBD_LU(8, 3)
+    VBD_LU(16, 1) VBD_LU(16, 2) VBD_LU(16, 3)
+    VBD_LU(32, 1) VBD_LU(64, 1)
+#undef VBD_LU
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -552,7 +604,7 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
                                                      const float* __restrict__ dminv, const float* __restrict__ vol,
                                                      const float* __restrict__ tmu, const float* __restrict__ tlam,
                                                      const int* __restrict__ row, i4* __restrict__ ptet,
-                                                     f4* __restrict__ pa, f4* __restrict__ pb, f2* __restrict__ pc,
+                                                     f4* __restrict__ pa, f4* __restrict__ pb, float* __restrict__ pc,
                                                      f2* __restrict__ pmat) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nFree) return;
@@ -560,22 +612,26 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
     int p = row[s];
     for (int e = off[v]; e < off[v + 1]; ++e, ++p) {
         const int t = entry[e] >> 2;
+        const int k = entry[e] & 3;
         const i4 tt = tets[t];
+        const int c[4] = {tt.x, tt.y, tt.z, tt.w};
+        int o[3];
+        int n = 0;
+        for (int j = 0; j < 4; ++j)
+            if (j != k) o[n++] = inv[c[j]];
         i4 r;
-        r.x = inv[tt.x];
-        r.y = inv[tt.y];
-        r.z = inv[tt.z];
-        r.w = inv[tt.w];
+        r.x = o[0] | (k << 30);
+        r.y = o[1];
+        r.z = o[2];
+        r.w = __float_as_int(vol[t]);
         ptet[p] = r;
         const float* D = dminv + 9 * (size_t)t;
         f4 a, b;
-        f2 c;
         a.x = D[0]; a.y = D[1]; a.z = D[2]; a.w = D[3];
         b.x = D[4]; b.y = D[5]; b.z = D[6]; b.w = D[7];
-        c.x = D[8]; c.y = vol[t];
         pa[p] = a;
         pb[p] = b;
-        pc[p] = c;
+        pc[p] = D[8];
         if (pmat) {
             f2 mm;
             mm.x = tmu[t];

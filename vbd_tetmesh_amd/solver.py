@@ -326,13 +326,14 @@ class DeviceMesh(_Handle):
     def set_graph(self, enabled: bool):
         check(lib.vbd_set_graph_enabled(self.h, int(bool(enabled))), self.h)
 
-    def set_launch_config(self, lanes: int, block: int):
-        check(lib.vbd_set_launch_config(self.h, lanes, block), self.h)
+    def set_launch_config(self, lanes: int, block: int = 256, pairs_per_lane: int = 1):
+        check(lib.vbd_set_launch_config(self.h, lanes, block, pairs_per_lane), self.h)
 
     def autotune(self, iterations: int = 3):
-        l, b, ms = C.c_int32(), C.c_int32(), C.c_float()
-        check(lib.vbd_autotune(self.h, iterations, C.byref(l), C.byref(b), C.byref(ms)), self.h)
-        return l.value, b.value, ms.value
+        """Returns (lanes_per_vertex, pairs_per_lane, block_size, ms per sweep or -1 if cached)."""
+        l, u, b, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        check(lib.vbd_autotune(self.h, iterations, C.byref(l), C.byref(u), C.byref(b), C.byref(ms)), self.h)
+        return l.value, u.value, b.value, ms.value
 
     def time_iterations(self, iterations: int):
         cms, bms = C.c_double(), C.c_double()
