@@ -127,6 +127,8 @@ int vbd_upload_velocities(vbd_solver* h, const float* velocity);
 int vbd_download_positions(vbd_solver* h, float* positions);
 int vbd_download_velocities(vbd_solver* h, float* velocity);
 int vbd_download_inertia(vbd_solver* h, float* inertia);
+/* Positions at the start of the last substep (TetMesh::mVertPrevPos after VBDsolver::step). */
+int vbd_download_previous_positions(vbd_solver* h, float* positions);
 int vbd_get_info(const vbd_solver* h, vbd_info* info);
 /* GPU-built adjacency in the caller's numbering: offsets (num_verts+1), entries (4*num_tets)
  * = tet*4 + local corner. */

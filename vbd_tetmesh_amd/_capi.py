@@ -98,6 +98,7 @@ SIGNATURES = [
     ("vbd_download_positions", C.c_int, [H, f32p]),
     ("vbd_download_velocities", C.c_int, [H, f32p]),
     ("vbd_download_inertia", C.c_int, [H, f32p]),
+    ("vbd_download_previous_positions", C.c_int, [H, f32p]),
     ("vbd_get_info", C.c_int, [H, C.POINTER(Info)]),
     ("vbd_get_adjacency", C.c_int, [H, i32p, i32p]),
     ("vbd_get_color_lists", C.c_int, [H, i32p, i32p]),

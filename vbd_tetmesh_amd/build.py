@@ -22,6 +22,24 @@ def hipcc() -> str:
     return "hipcc"
 
 
+DEMO_SRC = os.path.join(HERE, "csrc", "dropin_demo.cpp")
+DEMO_OUT = os.path.join(HERE, "vbd_dropin_demo")
+
+
+def build_demo(verbose: bool = False) -> str:
+    """The C++ drop-in driver (include/vbd/*.h over the C ABI), linked against the library."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), DEMO_SRC, "-L", HERE,
+           "-lvbd_hip", "-Wl,-rpath,$ORIGIN", "-o", DEMO_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed to build vbd_dropin_demo")
+    os.replace(DEMO_OUT + ".tmp", DEMO_OUT)
+    return DEMO_OUT
+
+
 def build_library(verbose: bool = False) -> str:
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
            *[os.path.join(HERE, s) for s in SOURCES], "-o", OUT + ".tmp"]
@@ -37,3 +55,4 @@ def build_library(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_demo(verbose=True))

@@ -515,6 +515,14 @@ int vbd_download_inertia(vbd_solver* h, float* y) {
     return download_vertex_array(h, h->m.inert, y);
 }
 
+int vbd_download_previous_positions(vbd_solver* h, float* x) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!x && h->m.nV) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    return download_vertex_array(h, h->m.xprev, x);
+}
+
 int vbd_get_adjacency(vbd_solver* h, int32_t* offsets, int32_t* entries) {
     int rc = check_ready(h);
     if (rc) return rc;

@@ -224,3 +224,34 @@ def concat(meshes: List[Mesh], name: str = "batch") -> tuple:
     u = Mesh(name=name, rest=rest, x0=x0, tets=tets, fixed=fixed, mu=meshes[0].mu, lam=meshes[0].lam,
              density=meshes[0].density, tet_mu=tmu, tet_lambda=tlam, gravity=meshes[0].gravity)
     return u, vo, to
+
+
+def write_tetgen(prefix: str, positions: np.ndarray, tets: np.ndarray, one_based: bool = True) -> None:
+    """Write TetGen .node/.ele files (the input format of the reference loader, main.cpp:14-168)."""
+    base = 1 if one_based else 0
+    with open(prefix + ".node", "w") as f:
+        f.write(f"{positions.shape[0]} 3 0 0\n")
+        for i, p in enumerate(positions):
+            f.write(f"{i + base} {repr(float(p[0]))} {repr(float(p[1]))} {repr(float(p[2]))}\n")
+    with open(prefix + ".ele", "w") as f:
+        f.write(f"{tets.shape[0]} 4 0\n")
+        for i, t in enumerate(tets):
+            f.write(f"{i + base} {t[0] + base} {t[1] + base} {t[2] + base} {t[3] + base}\n")
+
+
+def read_tetgen(prefix: str):
+    """Read TetGen .node/.ele files: (positions float32 (N,3), tets int32 (T,4)); 0- or 1-based
+    ids detected from the first node index, as main.cpp:38-39 and :112-114 do."""
+    with open(prefix + ".node") as f:
+        lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
+    n = int(lines[0][0])
+    rows = lines[1:1 + n]
+    start0 = int(rows[0][0]) == 0
+    pos = np.array([[float(r[1]), float(r[2]), float(r[3])] for r in rows], dtype=np.float32)
+    with open(prefix + ".ele") as f:
+        lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
+    m = int(lines[0][0])
+    tets = np.array([[int(v) for v in r[1:5]] for r in lines[1:1 + m]], dtype=np.int64)
+    if not start0:
+        tets -= 1
+    return pos, tets.astype(np.int32)

@@ -276,6 +276,9 @@ class DeviceMesh(_Handle):
     def inertia(self) -> np.ndarray:
         return self._down(lib.vbd_download_inertia)
 
+    def previous_positions(self) -> np.ndarray:
+        return self._down(lib.vbd_download_previous_positions)
+
     def set_positions(self, x):
         self._up(lib.vbd_upload_positions, x)
 
@@ -455,11 +458,14 @@ class VBDsolver:
         self._ensure_device(mesh, per_tet_material)
         self._dev.set_positions(mesh.mVertPos)
         self._dev.set_velocities(mesh.mVelocity)
-        mesh.mVertPrevPos[:] = mesh.mVertPos
         self._dev.step(1)
+        self.syncToHost(mesh)
+
+    def syncToHost(self, mesh: TetMesh):
         mesh.mVertPos[:] = self._dev.positions()
         mesh.mVelocity[:] = self._dev.velocities()
         mesh.mInertia[:] = self._dev.inertia()
+        mesh.mVertPrevPos[:] = self._dev.previous_positions()
 
     def step_device(self, frames: int = 1):
         self._dev.step(frames)
