@@ -97,6 +97,72 @@ def cpu_baseline(m, target_seconds: float = 10.0):
             "single_thread_sample": f"first {n1} free vertices of each color, 1 thread ({c1} solves in {s1:.1f} s)"}
 
 
+def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, iterations: int = 20):
+    """Batched workload: `count` independent beams of uneven size (nx ~ U{8..64}, ny = nz ~
+    U{2..8}, per-beam mu, nu), per-tet materials, 20 iterations with Chebyshev 0.95, all meshes
+    stepped by one hipGraph per frame, against the same meshes stepped one at a time with plain
+    (non-graph) launches, and one at a time with a hipGraph per mesh."""
+    from vbd_tetmesh_amd import VBDParams, meshgen
+    from vbd_tetmesh_amd.solver import DeviceMesh, prepare_host
+    ms = meshgen.batch_beams(count, seed=0)
+    params = VBDParams(dt=H, numIterations=iterations, gravity=(0.0, -9.8, 0.0), useAcceleration=True,
+                       accelerationRho=0.95)
+    descs = []
+    for m in ms:
+        t, off, verts, dminv, vol = prepare_host(m)
+        descs.append(dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                          rest_volume=vol, color_off=off, color_verts=verts, tet_mu=t.tetMu, tet_lambda=t.tetLambda))
+    nv = sum(m.num_verts for m in ms)
+    nt = sum(m.num_tets for m in ms)
+    b = DeviceMesh().init_batch(descs)
+    b.set_params(params)
+    tune = b.autotune(3)
+    b.set_positions(np.concatenate([d["positions"] for d in descs]))
+    b.step(1)  # warm-up (captures the graph)
+    b.synchronize()
+    b.set_positions(np.concatenate([d["positions"] for d in descs]))
+    b.set_velocities(np.zeros((nv, 3), np.float32))
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(frames)
+    b.synchronize()
+    tb = (time.perf_counter() - t0) / frames
+    xb = b.positions()
+    if not np.all(np.isfinite(xb)):
+        raise SystemExit("non-finite positions in the batched run")
+
+    def per_mesh(graph: bool):
+        ds = []
+        for kw in descs:
+            d = DeviceMesh().init(**kw)
+            d.set_params(params)
+            d.set_launch_config(tune[0], tune[2], tune[1])
+            d.set_graph(graph)
+            d.step(1)
+            ds.append(d)
+        for d in ds:
+            d.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(baseline_frames):
+            for d in ds:
+                d.step(1)
+        for d in ds:
+            d.synchronize()
+        return (time.perf_counter() - t0) / baseline_frames
+
+    tp = per_mesh(False)
+    tg = per_mesh(True)
+    vi = float(nv) * iterations
+    return {"workload": f"{count} beams, {nv} verts, {nt} tets, per-tet materials, {iterations} iters/step, "
+                        f"Chebyshev 0.95, one hipGraph per frame over all meshes",
+            "value": round(vi / tb / 1e6, 2), "unit": "M vertex-iters/s", "ms_per_step": round(tb * 1e3, 4),
+            "frames": frames,
+            "per_mesh_plain_ms_per_step": round(tp * 1e3, 3), "per_mesh_graph_ms_per_step": round(tg * 1e3, 3),
+            "speedup_vs_per_mesh_plain": round(tp / tb, 2), "speedup_vs_per_mesh_graph": round(tg / tb, 2),
+            "baseline_frames": baseline_frames,
+            "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
+
+
 def copy_probe(mb: int):
     """Device copy of a known size with torch, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
     import torch
@@ -122,6 +188,7 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--unroll", type=int, default=1)
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only the copy probe")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -208,6 +275,13 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(overall / HBM_PEAK_GBS, 4),
                           "blend_avg_launch_ms": round(bms / max(bn, 1), 4)},
     }
+    if not args.no_batch:
+        del d
+        bb = bench_batch()
+        out["batched"] = bb
+        log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step; per-mesh plain "
+            f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
+            f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
     if not args.no_cpu_baseline:
         cb = cpu_baseline(m)
         out["cpu_baseline"] = cb
