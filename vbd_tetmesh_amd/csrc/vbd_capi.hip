@@ -275,7 +275,7 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
     const int nV = d->num_verts, nT = d->num_tets, nC = d->num_colors;
     if (nV < 0 || nT < 0 || nC < 0) return fail(h, VBD_ERR_ARG, "negative size");
     if ((int64_t)nT * 4 >= (int64_t)1 << 31) return fail(h, VBD_ERR_ARG, "4*num_tets must fit int32");
-    if (nV >= (1 << 30)) return fail(h, VBD_ERR_ARG, "num_verts must be below 2^30");
+    if (nV >= (1 << 27)) return fail(h, VBD_ERR_ARG, "num_verts must be below 2^27 (32-bit byte offsets of 16-byte positions)");
     if ((nV > 0 && (!d->positions || !d->mass || !d->is_fixed)) ||
         (nT > 0 && (!d->tets || !d->dminv || !d->rest_volume)) || !d->color_offsets || (nV > 0 && !d->color_verts))
         return fail(h, VBD_ERR_ARG, "required mesh array is null");

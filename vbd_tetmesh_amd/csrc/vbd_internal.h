@@ -38,11 +38,10 @@ struct DevMesh {
     int* row = nullptr;     // nFree+1 pair offsets
     int* perm = nullptr;    // nV slot -> caller vertex id
     int* inv = nullptr;     // nV caller vertex id -> slot
-    i4* ptet = nullptr;     // nPairs  other 3 corner slots in tet order (corner k of the vertex in the
-                            //         top 2 bits of .x), V0 bits in .w
-    f4* pa = nullptr;       // nPairs  DmInv(0,0) (0,1) (0,2) (1,0)
-    f4* pb = nullptr;       // nPairs  DmInv(1,1) (1,2) (2,0) (2,1)
-    float* pc = nullptr;    // nPairs  DmInv(2,2)
+    i4* ptet = nullptr;     // nPairs  slots of the tet's 3 other corners (tet order), V0 bits in .w
+    f4* pa = nullptr;       // nPairs  Q_0.xyz, Q_1.x  (shape vectors of those 3 corners)
+    f4* pb = nullptr;       // nPairs  Q_1.yz, Q_2.xy
+    float* pc = nullptr;    // nPairs  Q_2.z
     f2* pmat = nullptr;     // nPairs  mu, lambda (per-tet materials only)
     // GPU-built structures kept in the caller's numbering (exposed for verification)
     int* adjOff = nullptr;    // nV+1

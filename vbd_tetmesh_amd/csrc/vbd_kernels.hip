@@ -49,72 +49,73 @@ __device__ __forceinline__ int xcd_block() {
 // ---------------------------------------------------------------------------------------
 // Per-(vertex, tet) contribution.
 //
-// A pair record holds the tet's three other corner slots, the vertex's own corner index k
-// (top two bits of the first slot), V0, and DmInv row-major.  The vertex's own position xs
-// is loaded once per vertex.  With q = shape vector of corner k (q_0 = -(row0+row1+row2)
-// of DmInv, q_k = row k-1, i.e. row k of Q = Ps*DmInv, Energy.cpp:101-106):
-//   F = Ds*DmInv, J = det F, C = cof F, P = mu(F - C) + lambda(J-1) C   (Energy.cpp:93-97)
+// A pair record holds the slots of the tet's three other corners (in tet corner order), V0,
+// and the shape vectors of those three corners, Q_j = row of Q = Ps*DmInv for that corner
+// (Energy.cpp:101-106: q_0 = -(row0+row1+row2) of DmInv, q_n = row n-1).  Because the four
+// shape vectors sum to zero, F = Ds*DmInv = sum_n x_n q_n^T = sum_j (y_j - x_s) Q_j^T with
+// y_j the other corners and x_s the vertex itself, whose shape vector is q = -(Q_0+Q_1+Q_2).
+// This form needs no corner index and no selects.  Then (Energy.cpp:93-97)
+//   J = det F, C = cof F, P = mu(F - C) + lambda(J-1) C,
 //   g = V P q,  H = V [mu |q|^2 I + lambda (C q)(C q)^T]
-// H is exactly the (k,k) 3x3 block of the 12x12 Hessian of Energy.cpp:115-154: the
-// (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.
+// and H is exactly the vertex's 3x3 diagonal block of the 12x12 Hessian of
+// Energy.cpp:115-154: the (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.
 // ---------------------------------------------------------------------------------------
 struct Acc {
     float h00, h01, h02, h11, h12, h22, r0, r1, r2;
 };
 
-constexpr int SLOT_MASK = 0x3FFFFFFF;
+struct Kin {  // F, its cofactor C and J of one tet, seen from one of its corners
+    float F00, F01, F02, F10, F11, F12, F20, F21, F22;
+    float C00, C01, C02, C10, C11, C12, C20, C21, C22;
+    float J, q0, q1, q2;
+};
 
-__device__ __forceinline__ void pair_accumulate(const i4 t, const f4 a, const f4 b, const float D22, const float tmu,
-                                                const float tlam, const f4 xs, const f4 ya, const f4 yb,
-                                                const f4 yc, Acc& A) {
-    const int k = (int)((unsigned)t.x >> 30);
-    const float V = __int_as_float(t.w);
-    // corners in tet order: the vertex itself at k, the other three in order
-    const f4 x0 = k == 0 ? xs : ya;
-    const f4 x1 = k == 0 ? ya : (k == 1 ? xs : yb);
-    const f4 x2 = k <= 1 ? yb : (k == 2 ? xs : yc);
-    const f4 x3 = k == 3 ? xs : yc;
-    const float D00 = a.x, D01 = a.y, D02 = a.z, D10 = a.w;
-    const float D11 = b.x, D12 = b.y, D20 = b.z, D21 = b.w;
-    // Ds columns (Energy.cpp:11-13)
-    const float e1x = x1.x - x0.x, e1y = x1.y - x0.y, e1z = x1.z - x0.z;
-    const float e2x = x2.x - x0.x, e2y = x2.y - x0.y, e2z = x2.z - x0.z;
-    const float e3x = x3.x - x0.x, e3y = x3.y - x0.y, e3z = x3.z - x0.z;
-    // F = Ds * DmInv
-    const float F00 = e1x * D00 + e2x * D10 + e3x * D20;
-    const float F01 = e1x * D01 + e2x * D11 + e3x * D21;
-    const float F02 = e1x * D02 + e2x * D12 + e3x * D22;
-    const float F10 = e1y * D00 + e2y * D10 + e3y * D20;
-    const float F11 = e1y * D01 + e2y * D11 + e3y * D21;
-    const float F12 = e1y * D02 + e2y * D12 + e3y * D22;
-    const float F20 = e1z * D00 + e2z * D10 + e3z * D20;
-    const float F21 = e1z * D01 + e2z * D11 + e3z * D21;
-    const float F22 = e1z * D02 + e2z * D12 + e3z * D22;
+__device__ __forceinline__ Kin pair_kinematics(const f4 a, const f4 b, const float c, const f4 xs, const f4 ya,
+                                               const f4 yb, const f4 yc) {
+    // Q_0 = (a.x, a.y, a.z), Q_1 = (a.w, b.x, b.y), Q_2 = (b.z, b.w, c)
+    Kin k;
+    const float e0x = ya.x - xs.x, e0y = ya.y - xs.y, e0z = ya.z - xs.z;
+    const float e1x = yb.x - xs.x, e1y = yb.y - xs.y, e1z = yb.z - xs.z;
+    const float e2x = yc.x - xs.x, e2y = yc.y - xs.y, e2z = yc.z - xs.z;
+    k.F00 = e0x * a.x + e1x * a.w + e2x * b.z;
+    k.F01 = e0x * a.y + e1x * b.x + e2x * b.w;
+    k.F02 = e0x * a.z + e1x * b.y + e2x * c;
+    k.F10 = e0y * a.x + e1y * a.w + e2y * b.z;
+    k.F11 = e0y * a.y + e1y * b.x + e2y * b.w;
+    k.F12 = e0y * a.z + e1y * b.y + e2y * c;
+    k.F20 = e0z * a.x + e1z * a.w + e2z * b.z;
+    k.F21 = e0z * a.y + e1z * b.x + e2z * b.w;
+    k.F22 = e0z * a.z + e1z * b.y + e2z * c;
     // cofactor (Energy.cpp:61-73) and J
-    const float C00 = F11 * F22 - F12 * F21, C01 = F12 * F20 - F10 * F22, C02 = F10 * F21 - F11 * F20;
-    const float C10 = F21 * F02 - F22 * F01, C11 = F22 * F00 - F20 * F02, C12 = F20 * F01 - F21 * F00;
-    const float C20 = F01 * F12 - F02 * F11, C21 = F02 * F10 - F00 * F12, C22 = F00 * F11 - F01 * F10;
-    const float J = F00 * C00 + F01 * C01 + F02 * C02;
-    const float aj = tlam * (J - 1.0f);
-    float q0, q1, q2;
-    if (k == 0) {
-        q0 = -(D00 + D10 + D20);
-        q1 = -(D01 + D11 + D21);
-        q2 = -(D02 + D12 + D22);
-    } else if (k == 1) {
-        q0 = D00; q1 = D01; q2 = D02;
-    } else if (k == 2) {
-        q0 = D10; q1 = D11; q2 = D12;
-    } else {
-        q0 = D20; q1 = D21; q2 = D22;
-    }
+    k.C00 = k.F11 * k.F22 - k.F12 * k.F21;
+    k.C01 = k.F12 * k.F20 - k.F10 * k.F22;
+    k.C02 = k.F10 * k.F21 - k.F11 * k.F20;
+    k.C10 = k.F21 * k.F02 - k.F22 * k.F01;
+    k.C11 = k.F22 * k.F00 - k.F20 * k.F02;
+    k.C12 = k.F20 * k.F01 - k.F21 * k.F00;
+    k.C20 = k.F01 * k.F12 - k.F02 * k.F11;
+    k.C21 = k.F02 * k.F10 - k.F00 * k.F12;
+    k.C22 = k.F00 * k.F11 - k.F01 * k.F10;
+    k.J = k.F00 * k.C00 + k.F01 * k.C01 + k.F02 * k.C02;
+    k.q0 = -(a.x + a.w + b.z);
+    k.q1 = -(a.y + b.x + b.w);
+    k.q2 = -(a.z + b.y + c);
+    return k;
+}
+
+__device__ __forceinline__ void pair_accumulate(const f4 a, const f4 b, const float c, const float V,
+                                                const float tmu, const float tlam, const f4 xs, const f4 ya,
+                                                const f4 yb, const f4 yc, Acc& A) {
+    const Kin k = pair_kinematics(a, b, c, xs, ya, yb, yc);
+    const float aj = tlam * (k.J - 1.0f);
+    const float q0 = k.q0, q1 = k.q1, q2 = k.q2;
     // P q = mu (F q - C q) + aj C q
-    const float Cq0 = C00 * q0 + C01 * q1 + C02 * q2;
-    const float Cq1 = C10 * q0 + C11 * q1 + C12 * q2;
-    const float Cq2 = C20 * q0 + C21 * q1 + C22 * q2;
-    const float Fq0 = F00 * q0 + F01 * q1 + F02 * q2;
-    const float Fq1 = F10 * q0 + F11 * q1 + F12 * q2;
-    const float Fq2 = F20 * q0 + F21 * q1 + F22 * q2;
+    const float Cq0 = k.C00 * q0 + k.C01 * q1 + k.C02 * q2;
+    const float Cq1 = k.C10 * q0 + k.C11 * q1 + k.C12 * q2;
+    const float Cq2 = k.C20 * q0 + k.C21 * q1 + k.C22 * q2;
+    const float Fq0 = k.F00 * q0 + k.F01 * q1 + k.F02 * q2;
+    const float Fq1 = k.F10 * q0 + k.F11 * q1 + k.F12 * q2;
+    const float Fq2 = k.F20 * q0 + k.F21 * q1 + k.F22 * q2;
     const float g0 = tmu * (Fq0 - Cq0) + aj * Cq0;
     const float g1 = tmu * (Fq1 - Cq1) + aj * Cq1;
     const float g2 = tmu * (Fq2 - Cq2) + aj * Cq2;
@@ -133,6 +134,17 @@ __device__ __forceinline__ void pair_accumulate(const i4 t, const f4 a, const f4
     A.r2 -= V * g2;
 }
 
+// Neighbour position gather through a raw buffer resource: 32-bit byte offsets instead of
+// 64-bit address arithmetic per load.
+__device__ __forceinline__ f4 gather_pos(__amdgpu_buffer_rsrc_t rsrc, int slot) {
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, slot << 4, 0, 0));
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc(const f4* pos, int n) {
+    // stride 0 (raw), num_records in bytes, flags = default data format for gfx9 raw loads
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<f4*>(pos), (short)0, n * 16, 0x00020000);
+}
+
 // ---------------------------------------------------------------------------------------
 // Per-color vertex solve.
 //
@@ -146,7 +158,7 @@ __device__ __forceinline__ void pair_accumulate(const i4 t, const f4 a, const f4
 // Chebyshev blend.
 // ---------------------------------------------------------------------------------------
 template <int LPV, int U, bool MAT, bool CONTACT, bool SAVE>
-__global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* __restrict__ pos,
+__global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, int nslots, f4* __restrict__ pos,
                                                      const f4* __restrict__ inert, const float* __restrict__ mass,
                                                      const int* __restrict__ row, const i4* __restrict__ ptet,
                                                      const f4* __restrict__ pa, const f4* __restrict__ pb,
@@ -162,6 +174,7 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* _
     Acc A = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     f4 xs = {0.f, 0.f, 0.f, 0.f};
     if (valid) {
+        const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, nslots);
         xs = pos[s];
         const int p0 = row[s], p1 = row[s + 1];
         for (int base = p0 + lane; base < p1; base += LPV * U) {
@@ -184,9 +197,9 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* _
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (base + u * LPV < p1) {
-                    ya[u] = pos[t[u].x & SLOT_MASK];
-                    yb[u] = pos[t[u].y];
-                    yc[u] = pos[t[u].z];
+                    ya[u] = gather_pos(prs, t[u].x);
+                    yb[u] = gather_pos(prs, t[u].y);
+                    yc[u] = gather_pos(prs, t[u].z);
                 }
             }
 #pragma unroll
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, f4* _
                         tmu = mm[u].x;
                         tlam = mm[u].y;
                     }
-                    pair_accumulate(t[u], a[u], b[u], c[u], tmu, tlam, xs, ya[u], yb[u], yc[u], A);
+                    pair_accumulate(a[u], b[u], c[u], __int_as_float(t[u].w), tmu, tlam, xs, ya[u], yb[u], yc[u], A);
                 }
             }
         }
@@ -246,7 +259,7 @@ static hipError_t launch_color_t(const DevMesh& m, int color, int block, float s
     const int gpb = block / LPV;
     int nb = (count + gpb - 1) / gpb;
     nb = (nb + 7) & ~7;
-    hipLaunchKernelGGL((k_color_solve<LPV, U, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.pos,
+    hipLaunchKernelGGL((k_color_solve<LPV, U, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.nV, m.pos,
                        m.inert, m.mass, m.row, m.ptet, m.pa, m.pb, m.pc, m.pmat, save, mu, lambda, sdt, ct);
     return hipGetLastError();
 }
@@ -615,23 +628,38 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
         const int k = entry[e] & 3;
         const i4 tt = tets[t];
         const int c[4] = {tt.x, tt.y, tt.z, tt.w};
+        const float* D = dminv + 9 * (size_t)t;
+        // shape vectors q_n (rows of Q = Ps * DmInv)
+        float q[4][3];
+        for (int j = 0; j < 3; ++j) {
+            q[0][j] = -D[j] - D[3 + j] - D[6 + j];
+            q[1][j] = D[j];
+            q[2][j] = D[3 + j];
+            q[3][j] = D[6 + j];
+        }
         int o[3];
+        float Q[9];
         int n = 0;
         for (int j = 0; j < 4; ++j)
-            if (j != k) o[n++] = inv[c[j]];
+            if (j != k) {
+                o[n] = inv[c[j]];
+                Q[3 * n] = q[j][0];
+                Q[3 * n + 1] = q[j][1];
+                Q[3 * n + 2] = q[j][2];
+                ++n;
+            }
         i4 r;
-        r.x = o[0] | (k << 30);
+        r.x = o[0];
         r.y = o[1];
         r.z = o[2];
         r.w = __float_as_int(vol[t]);
         ptet[p] = r;
-        const float* D = dminv + 9 * (size_t)t;
         f4 a, b;
-        a.x = D[0]; a.y = D[1]; a.z = D[2]; a.w = D[3];
-        b.x = D[4]; b.y = D[5]; b.z = D[6]; b.w = D[7];
+        a.x = Q[0]; a.y = Q[1]; a.z = Q[2]; a.w = Q[3];
+        b.x = Q[4]; b.y = Q[5]; b.z = Q[6]; b.w = Q[7];
         pa[p] = a;
         pb[p] = b;
-        pc[p] = D[8];
+        pc[p] = Q[8];
         if (pmat) {
             f2 mm;
             mm.x = tmu[t];
