@@ -38,10 +38,10 @@ struct DevMesh {
     int* row = nullptr;     // nFree+1 pair offsets
     int* perm = nullptr;    // nV slot -> caller vertex id
     int* inv = nullptr;     // nV caller vertex id -> slot
-    i4* ptet = nullptr;     // nPairs  slots of the tet's 3 other corners (tet order), V0 bits in .w
-    f4* pa = nullptr;       // nPairs  Q_0.xyz, Q_1.x  (shape vectors of those 3 corners)
-    f4* pb = nullptr;       // nPairs  Q_1.yz, Q_2.xy
-    float* pc = nullptr;    // nPairs  Q_2.z
+    i4* ptet = nullptr;     // nPairs  3 corner slots (bit 31 of .x: the vertex is corner 0), V0 bits in .w
+    f4* pa = nullptr;       // nPairs  R0.xyz, R1.x  (three DmInv rows, see k_build_pairs)
+    f4* pb = nullptr;       // nPairs  R1.yz, R2.xy
+    float* pc = nullptr;    // nPairs  R2.z
     f2* pmat = nullptr;     // nPairs  mu, lambda (per-tet materials only)
     // GPU-built structures kept in the caller's numbering (exposed for verification)
     int* adjOff = nullptr;    // nV+1

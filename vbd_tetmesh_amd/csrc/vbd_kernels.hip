@@ -49,17 +49,23 @@ __device__ __forceinline__ int xcd_block() {
 // ---------------------------------------------------------------------------------------
 // Per-(vertex, tet) contribution.
 //
-// A pair record holds the slots of the tet's three other corners (in tet corner order), V0,
-// and the shape vectors of those three corners, Q_j = row of Q = Ps*DmInv for that corner
-// (Energy.cpp:101-106: q_0 = -(row0+row1+row2) of DmInv, q_n = row n-1).  Because the four
-// shape vectors sum to zero, F = Ds*DmInv = sum_n x_n q_n^T = sum_j (y_j - x_s) Q_j^T with
-// y_j the other corners and x_s the vertex itself, whose shape vector is q = -(Q_0+Q_1+Q_2).
-// This form needs no corner index and no selects.  Then (Energy.cpp:93-97)
-//   J = det F, C = cof F, P = mu(F - C) + lambda(J-1) C,
+// F is formed as the reference forms it, F = Ds*DmInv with Ds = [x1-x0, x2-x0, x3-x0]
+// (Energy.cpp:11-21), rewritten without a corner index: a pair record holds three corner
+// slots (a, b, c), V0, and three DmInv rows (R0, R1, R2) such that
+//   vertex is corner 0 (flag bit set):  a, b, c = corners 1, 2, 3;  R = rows 0, 1, 2;
+//                                       F = sum_j (y_j - x_s) R_j^T,  q = -(R0+R1+R2)
+//   vertex is corner k > 0:             a = corner 0, b, c = the remaining corners in tet
+//                                       order; R0 = row k-1, R1, R2 = their rows;
+//                                       F = (x_s - y_a) R0^T + (y_b - y_a) R1^T + (y_c - y_a) R2^T,
+//                                       q = R0
+// where q is the vertex's shape vector, row k of Q = Ps*DmInv (Energy.cpp:101-106).  Then
+// (Energy.cpp:93-97)  J = det F, C = cof F, P = mu(F - C) + lambda(J-1) C,
 //   g = V P q,  H = V [mu |q|^2 I + lambda (C q)(C q)^T]
 // and H is exactly the vertex's 3x3 diagonal block of the 12x12 Hessian of
 // Energy.cpp:115-154: the (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.
 // ---------------------------------------------------------------------------------------
+constexpr int SLOT_MASK = 0x7FFFFFFF;
+
 struct Acc {
     float h00, h01, h02, h11, h12, h22, r0, r1, r2;
 };
@@ -70,13 +76,15 @@ struct Kin {  // F, its cofactor C and J of one tet, seen from one of its corner
     float J, q0, q1, q2;
 };
 
-__device__ __forceinline__ Kin pair_kinematics(const f4 a, const f4 b, const float c, const f4 xs, const f4 ya,
-                                               const f4 yb, const f4 yc) {
-    // Q_0 = (a.x, a.y, a.z), Q_1 = (a.w, b.x, b.y), Q_2 = (b.z, b.w, c)
+__device__ __forceinline__ Kin pair_kinematics(const bool corner0, const f4 a, const f4 b, const float c,
+                                               const f4 xs, const f4 ya, const f4 yb, const f4 yc) {
+    // R0 = (a.x, a.y, a.z), R1 = (a.w, b.x, b.y), R2 = (b.z, b.w, c)
     Kin k;
-    const float e0x = ya.x - xs.x, e0y = ya.y - xs.y, e0z = ya.z - xs.z;
-    const float e1x = yb.x - xs.x, e1y = yb.y - xs.y, e1z = yb.z - xs.z;
-    const float e2x = yc.x - xs.x, e2y = yc.y - xs.y, e2z = yc.z - xs.z;
+    const f4 base = corner0 ? xs : ya;
+    const f4 first = corner0 ? ya : xs;
+    const float e0x = first.x - base.x, e0y = first.y - base.y, e0z = first.z - base.z;
+    const float e1x = yb.x - base.x, e1y = yb.y - base.y, e1z = yb.z - base.z;
+    const float e2x = yc.x - base.x, e2y = yc.y - base.y, e2z = yc.z - base.z;
     k.F00 = e0x * a.x + e1x * a.w + e2x * b.z;
     k.F01 = e0x * a.y + e1x * b.x + e2x * b.w;
     k.F02 = e0x * a.z + e1x * b.y + e2x * c;
@@ -97,16 +105,16 @@ __device__ __forceinline__ Kin pair_kinematics(const f4 a, const f4 b, const flo
     k.C21 = k.F02 * k.F10 - k.F00 * k.F12;
     k.C22 = k.F00 * k.F11 - k.F01 * k.F10;
     k.J = k.F00 * k.C00 + k.F01 * k.C01 + k.F02 * k.C02;
-    k.q0 = -(a.x + a.w + b.z);
-    k.q1 = -(a.y + b.x + b.w);
-    k.q2 = -(a.z + b.y + c);
+    k.q0 = corner0 ? -(a.x + a.w + b.z) : a.x;
+    k.q1 = corner0 ? -(a.y + b.x + b.w) : a.y;
+    k.q2 = corner0 ? -(a.z + b.y + c) : a.z;
     return k;
 }
 
-__device__ __forceinline__ void pair_accumulate(const f4 a, const f4 b, const float c, const float V,
-                                                const float tmu, const float tlam, const f4 xs, const f4 ya,
-                                                const f4 yb, const f4 yc, Acc& A) {
-    const Kin k = pair_kinematics(a, b, c, xs, ya, yb, yc);
+__device__ __forceinline__ void pair_accumulate(const bool corner0, const f4 a, const f4 b, const float c,
+                                                const float V, const float tmu, const float tlam, const f4 xs,
+                                                const f4 ya, const f4 yb, const f4 yc, Acc& A) {
+    const Kin k = pair_kinematics(corner0, a, b, c, xs, ya, yb, yc);
     const float aj = tlam * (k.J - 1.0f);
     const float q0 = k.q0, q1 = k.q1, q2 = k.q2;
     // P q = mu (F q - C q) + aj C q
@@ -197,7 +205,7 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, int n
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (base + u * LPV < p1) {
-                    ya[u] = gather_pos(prs, t[u].x);
+                    ya[u] = gather_pos(prs, t[u].x & SLOT_MASK);
                     yb[u] = gather_pos(prs, t[u].y);
                     yc[u] = gather_pos(prs, t[u].z);
                 }
@@ -210,7 +218,8 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, int n
                         tmu = mm[u].x;
                         tlam = mm[u].y;
                     }
-                    pair_accumulate(a[u], b[u], c[u], __int_as_float(t[u].w), tmu, tlam, xs, ya[u], yb[u], yc[u], A);
+                    pair_accumulate(t[u].x < 0, a[u], b[u], c[u], __int_as_float(t[u].w), tmu, tlam, xs, ya[u], yb[u],
+                                    yc[u], A);
                 }
             }
         }
@@ -629,27 +638,27 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
         const i4 tt = tets[t];
         const int c[4] = {tt.x, tt.y, tt.z, tt.w};
         const float* D = dminv + 9 * (size_t)t;
-        // shape vectors q_n (rows of Q = Ps * DmInv)
-        float q[4][3];
-        for (int j = 0; j < 3; ++j) {
-            q[0][j] = -D[j] - D[3 + j] - D[6 + j];
-            q[1][j] = D[j];
-            q[2][j] = D[3 + j];
-            q[3][j] = D[6 + j];
+        // record layout: see the pair contribution comment above (base corner 0 as in Ds)
+        int o[3], rows[3];
+        if (k == 0) {
+            o[0] = inv[c[1]]; o[1] = inv[c[2]]; o[2] = inv[c[3]];
+            rows[0] = 0; rows[1] = 1; rows[2] = 2;
+        } else {
+            o[0] = inv[c[0]];
+            rows[0] = k - 1;
+            int n = 1;
+            for (int j = 1; j < 4; ++j)
+                if (j != k) {
+                    o[n] = inv[c[j]];
+                    rows[n] = j - 1;
+                    ++n;
+                }
         }
-        int o[3];
         float Q[9];
-        int n = 0;
-        for (int j = 0; j < 4; ++j)
-            if (j != k) {
-                o[n] = inv[c[j]];
-                Q[3 * n] = q[j][0];
-                Q[3 * n + 1] = q[j][1];
-                Q[3 * n + 2] = q[j][2];
-                ++n;
-            }
+        for (int n = 0; n < 3; ++n)
+            for (int j = 0; j < 3; ++j) Q[3 * n + j] = D[3 * rows[n] + j];
         i4 r;
-        r.x = o[0];
+        r.x = o[0] | (k == 0 ? (int)0x80000000 : 0);
         r.y = o[1];
         r.z = o[2];
         r.w = __float_as_int(vol[t]);
