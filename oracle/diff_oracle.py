@@ -164,7 +164,7 @@ def solve_linear(P: DiffProblem, A, b, mu, lam, rtol=1e-13):
     return x
 
 
-def solve_step(P: DiffProblem, x_n, v_n, mu, lam, tol=1e-10, max_newton=50):
+def solve_step(P: DiffProblem, x_n, v_n, mu, lam, tol=1e-10, max_newton=50, guess=None):
     """One implicit step by Newton on G; returns (x_{n+1}, v_{n+1}, newton iterations).
     Stops at |grad G| <= tol |M g| (the float64 rounding floor of grad G is near 1e-12 |M g|
     on these meshes) or when Newton stops making progress below 1e-8 |M g|."""
@@ -176,7 +176,7 @@ def solve_step(P: DiffProblem, x_n, v_n, mu, lam, tol=1e-10, max_newton=50):
     y = x_n.copy()
     y[free] = x_n[free] + h * vp[free]
     scale = np.linalg.norm((P.mass[free, None] * P.gravity[None, :]).ravel()) + 1e-300
-    x = x_n.copy()
+    x = x_n.copy() if guess is None else guess.copy()
 
     def G(z):
         return 0.5 / h ** 2 * np.sum(P.mass[:, None] * (z - y) ** 2) + energy(P, z, mu, lam)
@@ -211,10 +211,11 @@ def solve_step(P: DiffProblem, x_n, v_n, mu, lam, tol=1e-10, max_newton=50):
     return x, v, it
 
 
-def forward(P: DiffProblem, x0, v0, mu, lam, steps, tol=1e-10):
+def forward(P: DiffProblem, x0, v0, mu, lam, steps, tol=1e-10, guesses=None):
+    """guesses: optional list of Newton starting points per step (e.g. a nearby trajectory)."""
     xs, vs = [x0.copy()], [v0.copy()]
-    for _ in range(steps):
-        x, v, _ = solve_step(P, xs[-1], vs[-1], mu, lam, tol)
+    for n in range(steps):
+        x, v, _ = solve_step(P, xs[-1], vs[-1], mu, lam, tol, guess=None if guesses is None else guesses[n + 1])
         xs.append(x)
         vs.append(v)
     return xs, vs

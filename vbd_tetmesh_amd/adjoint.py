@@ -1,0 +1,163 @@
+"""Differentiable rollouts of converged VBD steps (implicit differentiation on the device).
+
+Forward: `steps` implicit-Euler steps, each iterated on the GPU until
+|grad G| <= rel_tol |M g| (vbd_step_converged), from x_0 = rest with initial velocity v_0 and
+per-tet materials (mu, lambda).  The reference has no differentiation; the step itself follows
+VBDsolver::step (VBD.cpp:60-101, TetMesh.h:305-348).
+
+Backward, per step n (x_{n+1} = argmin G, v_{n+1} = c (x_{n+1} - x_n)/h, c = 0.999):
+    a_hat = a + (c/h) b
+    A(x_{n+1}) w = a_hat           colored block Gauss-Seidel on the device (vbd_adjoint_solve)
+    mu_bar  += -V (F - C) : dF(w),  lambda_bar += -V (J - 1) C : dF(w)   (vbd_parameter_gradients)
+    a <- (M/h^2) w - (c/h) b,  b <- (M/h) w
+and v0_bar = b at the end.  Positions and velocities are checkpointed every `checkpoint_every`
+steps; each segment's states are recomputed from its checkpoint during the backward pass (the
+device step is bit-deterministic, so the recomputed states equal the forward ones).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from .solver import DeviceMesh, VBDParams, prepare_host
+
+
+@dataclass
+class RolloutStats:
+    forward_iterations: List[int] = field(default_factory=list)
+    forward_residuals: List[float] = field(default_factory=list)
+    adjoint_sweeps: List[int] = field(default_factory=list)
+    adjoint_residuals: List[float] = field(default_factory=list)
+    forward_seconds: float = 0.0
+    backward_seconds: float = 0.0
+
+
+class DiffSim:
+    """Device rollout + adjoint for one mesh with per-tet materials."""
+
+    def __init__(self, mesh, steps: int = 30, h: float = 1.0 / 60.0, rho: float = 0.95, max_iterations: int = 4000,
+                 check_every: int = 10, rel_tol: float = 1e-5, adj_max_iterations: int = 8000,
+                 adj_rel_tol: float = 1e-5, adj_rho: float = 0.9, checkpoint_every: int = 5, device: int = 0,
+                 damping: float = 0.999):
+        if mesh.tet_mu is None:
+            raise ValueError("the differentiable rollout needs per-tet materials (mesh.tet_mu / tet_lambda)")
+        t, off, verts, dminv, vol = prepare_host(mesh)
+        self.dev = DeviceMesh(device).init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, off, verts,
+                                           tet_mu=mesh.tet_mu, tet_lambda=mesh.tet_lambda)
+        self.params = VBDParams(dt=h, numIterations=max_iterations, substeps=1, gravity=mesh.gravity,
+                                useAcceleration=rho > 0, accelerationRho=rho, velocityDamping=damping)
+        self.dev.set_params(self.params)
+        self.steps, self.h, self.c = steps, h, damping
+        self.max_iterations, self.check_every, self.rel_tol = max_iterations, check_every, rel_tol
+        self.adj_max_iterations, self.adj_rel_tol, self.adj_rho = adj_max_iterations, adj_rel_tol, adj_rho
+        self.k = max(1, checkpoint_every)
+        self.x0 = t.mVertPos.copy()
+        self.mass = t.vertexMass.astype(np.float64)
+        self.free = t.isFixed == 0
+        self.num_tets = t.numTets
+        self.stats = RolloutStats()
+
+    # -- forward
+    def _step(self):
+        it, res = self.dev.step_converged(self.max_iterations, self.check_every, 0.0, self.rel_tol)
+        return it, res
+
+    def rollout(self, mu, lam, v0) -> Tuple[np.ndarray, Dict[int, Tuple[np.ndarray, np.ndarray]]]:
+        """Returns (x_N, checkpoints {n: (x_n, v_n)})."""
+        t0 = time.perf_counter()
+        self.dev.set_materials(mu, lam)
+        self.dev.set_positions(self.x0)
+        self.dev.set_velocities(np.ascontiguousarray(v0, np.float32).reshape(-1, 3))
+        ck = {}
+        self.stats = RolloutStats()
+        for n in range(self.steps):
+            if n % self.k == 0:
+                ck[n] = (self.dev.positions(), self.dev.velocities())
+            it, res = self._step()
+            self.stats.forward_iterations.append(it)
+            self.stats.forward_residuals.append(res)
+        x = self.dev.positions()
+        self.stats.forward_seconds = time.perf_counter() - t0
+        return x, ck
+
+    def _segment_states(self, start: int, end: int, ck) -> List[np.ndarray]:
+        """Positions x_start .. x_end recomputed from the checkpoint at `start`."""
+        x, v = ck[start]
+        self.dev.set_positions(x)
+        self.dev.set_velocities(v)
+        xs = [x]
+        for _ in range(start, end):
+            self._step()
+            xs.append(self.dev.positions())
+        return xs
+
+    # -- backward
+    def backward(self, ck, dL_dxN, dL_dvN=None):
+        """Gradients (mu_bar (T,), lambda_bar (T,), v0_bar (N,3)) in float64."""
+        t0 = time.perf_counter()
+        h, c = self.h, self.c
+        free = self.free
+        m = self.mass[:, None]
+        a = np.asarray(dL_dxN, np.float64).reshape(-1, 3).copy()
+        b = np.zeros_like(a) if dL_dvN is None else np.asarray(dL_dvN, np.float64).reshape(-1, 3).copy()
+        a[~free] = 0.0
+        b[~free] = 0.0
+        gmu = np.zeros(self.num_tets)
+        glam = np.zeros(self.num_tets)
+        starts = sorted(ck.keys())
+        w = None
+        for si in range(len(starts) - 1, -1, -1):
+            start = starts[si]
+            end = starts[si + 1] if si + 1 < len(starts) else self.steps
+            xs = self._segment_states(start, end, ck)
+            for n in range(end - 1, start - 1, -1):
+                ahat = a + (c / h) * b
+                self.dev.set_positions(xs[n + 1 - start])
+                w, sweeps, rr = self.dev.adjoint_solve(ahat, w0=w, max_iterations=self.adj_max_iterations,
+                                                       check_every=self.check_every, rel_tol=self.adj_rel_tol,
+                                                       rho=self.adj_rho)
+                self.stats.adjoint_sweeps.append(sweeps)
+                self.stats.adjoint_residuals.append(rr)
+                dm, dl = self.dev.parameter_gradients(w)
+                gmu += dm
+                glam += dl
+                w64 = w.astype(np.float64)
+                a_new = m / h ** 2 * w64 - (c / h) * b
+                b_new = m / h * w64
+                a_new[~free] = 0.0
+                b_new[~free] = 0.0
+                a, b = a_new, b_new
+        self.stats.backward_seconds = time.perf_counter() - t0
+        return gmu, glam, b
+
+    def loss_and_grads(self, mu, lam, v0, x_target):
+        x, ck = self.rollout(mu, lam, v0)
+        d = x.astype(np.float64) - x_target
+        L = float(np.sum(d * d))
+        gmu, glam, gv0 = self.backward(ck, 2.0 * d)
+        return L, gmu, glam, gv0
+
+
+def make_autograd_function():
+    """torch.autograd.Function over DiffSim: x_N = rollout(mu, lambda, v0)."""
+    import torch
+
+    class VBDRollout(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, mu, lam, v0, sim: DiffSim):
+            x, ck = sim.rollout(mu.detach().cpu().numpy(), lam.detach().cpu().numpy(), v0.detach().cpu().numpy())
+            ctx.sim, ctx.ck = sim, ck
+            ctx.dev = mu.device
+            return torch.from_numpy(x).to(device=mu.device, dtype=torch.float32)
+
+        @staticmethod
+        def backward(ctx, gx):
+            gmu, glam, gv0 = ctx.sim.backward(ctx.ck, gx.detach().cpu().numpy())
+            dev = ctx.dev
+            return (torch.from_numpy(gmu.astype(np.float32)).to(dev), torch.from_numpy(glam.astype(np.float32)).to(dev),
+                    torch.from_numpy(gv0.astype(np.float32)).to(dev), None)
+
+    return VBDRollout

@@ -34,6 +34,15 @@ struct vbd_solver {
     bool graphValid = false;
     double* d_energy = nullptr;
     std::vector<int32_t> meshVertOffsets;  // batched uploads: per-mesh vertex offsets
+    // differentiation scratch, allocated on first use (slot order, nV entries)
+    f4* d_w = nullptr;
+    f4* d_rhs = nullptr;
+    f4* d_out = nullptr;
+    f4* d_wsave[2] = {nullptr, nullptr};
+    double* d_r2 = nullptr;
+    double* d_sum = nullptr;   // 2 doubles
+    float* d_tgrad = nullptr;  // 2 * nT
+    float* d_x3 = nullptr;     // 3 * nV staging
 };
 
 namespace {
@@ -72,6 +81,15 @@ void drop_graph(vbd_solver* h) {
     if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
     h->graphExec = nullptr;
     h->graphValid = false;
+}
+
+void free_scratch(vbd_solver* h) {
+    void* ps[] = {h->d_w, h->d_rhs, h->d_out, h->d_wsave[0], h->d_wsave[1], h->d_r2, h->d_sum, h->d_tgrad, h->d_x3};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    h->d_w = h->d_rhs = h->d_out = h->d_wsave[0] = h->d_wsave[1] = nullptr;
+    h->d_r2 = h->d_sum = nullptr;
+    h->d_tgrad = h->d_x3 = nullptr;
 }
 
 // VBDsolver::getAcceleratorOmega (VBD.cpp:302-311): float arguments, double literals, so the
@@ -263,6 +281,7 @@ void vbd_destroy(vbd_solver* h) {
     DeviceGuard g(h->device);
     (void)hipStreamSynchronize(h->stream);
     drop_graph(h);
+    free_scratch(h);
     free_device_mesh(h->m);
     if (h->d_energy) (void)hipFree(h->d_energy);
     (void)hipStreamDestroy(h->stream);
@@ -318,6 +337,7 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
     DeviceGuard g(h->device);
     (void)hipStreamSynchronize(h->stream);
     drop_graph(h);
+    free_scratch(h);
     free_device_mesh(h->m);
     h->inited = false;
 
@@ -836,6 +856,232 @@ int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy) {
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
     return VBD_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// differentiation entry points
+// ---------------------------------------------------------------------------------------
+namespace {
+
+hipError_t ensure_scratch(vbd_solver* h) {
+    const size_t n = (size_t)(h->m.nV > 0 ? h->m.nV : 1);
+    const size_t t = (size_t)(h->m.nT > 0 ? h->m.nT : 1);
+    hipError_t e = hipSuccess;
+    if (!h->d_w && (e = hipMalloc(&h->d_w, sizeof(f4) * n)) != hipSuccess) return e;
+    if (!h->d_rhs && (e = hipMalloc(&h->d_rhs, sizeof(f4) * n)) != hipSuccess) return e;
+    if (!h->d_out && (e = hipMalloc(&h->d_out, sizeof(f4) * n)) != hipSuccess) return e;
+    if (!h->d_wsave[0] && (e = hipMalloc(&h->d_wsave[0], sizeof(f4) * n)) != hipSuccess) return e;
+    if (!h->d_wsave[1] && (e = hipMalloc(&h->d_wsave[1], sizeof(f4) * n)) != hipSuccess) return e;
+    if (!h->d_r2 && (e = hipMalloc(&h->d_r2, sizeof(double) * n)) != hipSuccess) return e;
+    if (!h->d_sum && (e = hipMalloc(&h->d_sum, sizeof(double) * 2)) != hipSuccess) return e;
+    if (!h->d_tgrad && (e = hipMalloc(&h->d_tgrad, sizeof(float) * 2 * t)) != hipSuccess) return e;
+    if (!h->d_x3 && (e = hipMalloc(&h->d_x3, sizeof(float) * 3 * n)) != hipSuccess) return e;
+    return e;
+}
+
+int pass_lpv(const vbd_solver* h) { return (h->lpv == 8 || h->lpv == 32) ? h->lpv : 16; }
+
+// host 3N (caller order) -> slot f4 buffer; pinned slots zeroed when zero_pinned
+hipError_t upload_vec(vbd_solver* h, const float* src, f4* dst, bool zero_pinned) {
+    hipError_t e = hipMemcpyAsync(h->d_x3, src, sizeof(float) * 3 * (size_t)h->m.nV, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_scatter_pos(h->m.nV, h->d_x3, h->m.inv, dst, h->stream);
+    if (e == hipSuccess && zero_pinned && h->m.nV > h->m.nFree)
+        e = hipMemsetAsync(dst + h->m.nFree, 0, sizeof(f4) * (size_t)(h->m.nV - h->m.nFree), h->stream);
+    return e;
+}
+
+hipError_t download_vec(vbd_solver* h, const f4* src, float* dst) {
+    hipError_t e = launch_gather_pos(h->m.nV, src, h->m.inv, h->d_x3, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst, h->d_x3, sizeof(float) * 3 * (size_t)h->m.nV, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e;
+}
+
+// sum of the first n entries of d_r2 -> host (deterministic)
+hipError_t sum_r2(vbd_solver* h, int n, double* out) {
+    hipError_t e = launch_sum_fixed(n, h->d_r2, h->d_sum, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, h->d_sum, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e;
+}
+
+float step_mu(const vbd_solver* h) { return h->paramsSet ? h->params.mu : 0.f; }
+float step_lambda(const vbd_solver* h) { return h->paramsSet ? h->params.lambda : 0.f; }
+float step_sdt(const vbd_solver* h) { return inv_h2_ref(h->params.dt / (float)h->params.substeps); }
+
+hipError_t forward_residual(vbd_solver* h, double* norm2) {
+    const ContactArgs ct = contact_of(h->params);
+    hipError_t e = launch_pass(3, h->m, 0, h->m.nFree, pass_lpv(h), 256, nullptr, nullptr, h->d_out, h->d_r2, nullptr,
+                               step_mu(h), step_lambda(h), step_sdt(h), ct, h->stream);
+    if (e == hipSuccess) e = sum_r2(h, h->m.nFree, norm2);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vbd_set_materials(vbd_solver* h, const float* tet_mu, const float* tet_lambda) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->m.perTetMat) return fail(h, VBD_ERR_STATE, "mesh was uploaded without per-tet materials");
+    if ((!tet_mu || !tet_lambda) && h->m.nT) return fail(h, VBD_ERR_ARG, "null material array");
+    if (!finite_array(tet_mu, h->m.nT) || !finite_array(tet_lambda, h->m.nT))
+        return fail(h, VBD_ERR_NUMERIC, "non-finite material");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, hipMemcpyAsync(h->m.tmu, tet_mu, sizeof(float) * h->m.nT, hipMemcpyHostToDevice, h->stream));
+    HIP_CHECK(h, hipMemcpyAsync(h->m.tlam, tet_lambda, sizeof(float) * h->m.nT, hipMemcpyHostToDevice, h->stream));
+    HIP_CHECK(h, launch_update_pmat(h->m, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_residual_norm(vbd_solver* h, double* norm, double* gravity_load) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_scratch(h));
+    double r2 = 0, l2 = 0;
+    HIP_CHECK(h, forward_residual(h, &r2));
+    HIP_CHECK(h, launch_gravity_load2(h->m, h->params.gravity, h->d_r2, h->stream));
+    HIP_CHECK(h, sum_r2(h, h->m.nFree, &l2));
+    if (norm) *norm = std::sqrt(r2);
+    if (gravity_load) *gravity_load = std::sqrt(l2);
+    return VBD_OK;
+}
+
+int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_every, float abs_tol, float rel_tol,
+                       int32_t* iterations, double* final_norm) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
+    if (h->params.substeps != 1) return fail(h, VBD_ERR_ARG, "converged steps support substeps == 1");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_scratch(h));
+    const vbd_step_params& p = h->params;
+    const float hstep = p.dt;
+    const float sdt = inv_h2_ref(hstep);
+    const ContactArgs ct = contact_of(p);
+    double l2 = 0;
+    HIP_CHECK(h, launch_gravity_load2(h->m, p.gravity, h->d_r2, h->stream));
+    HIP_CHECK(h, sum_r2(h, h->m.nFree, &l2));
+    const double tol = std::max((double)abs_tol, (double)rel_tol * std::sqrt(l2));
+    HIP_CHECK(h, launch_predict(h->m, hstep, p.gravity, h->stream));
+    if (p.start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
+    float omega = 1.0f;
+    int it = 0;
+    double r2 = 0;
+    bool checked_last = false;
+    for (it = 0; it < max_iterations;) {
+        f4* save = p.use_acceleration ? h->m.save[it & 1] : nullptr;
+        HIP_CHECK(h, enqueue_sweep(h, sdt, p.mu, p.lambda, ct, save));
+        if (p.use_acceleration) {
+            omega = omega_ref(it + 1, p.acceleration_rho, omega);
+            if (omega > 1.0f) HIP_CHECK(h, launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream));
+        }
+        ++it;
+        checked_last = false;
+        if (it % check_every == 0 || it == max_iterations) {
+            HIP_CHECK(h, forward_residual(h, &r2));
+            checked_last = true;
+            if (std::sqrt(r2) <= tol) break;
+        }
+    }
+    if (!checked_last) HIP_CHECK(h, forward_residual(h, &r2));
+    HIP_CHECK(h, launch_velocity(h->m, hstep, p.velocity_damping, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    if (iterations) *iterations = it;
+    if (final_norm) *final_norm = std::sqrt(r2);
+    return VBD_OK;
+}
+
+int vbd_hessian_vector(vbd_solver* h, const float* w, float* out) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    if ((!w || !out) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_scratch(h));
+    HIP_CHECK(h, upload_vec(h, w, h->d_w, false));
+    HIP_CHECK(h, hipMemsetAsync(h->d_out, 0, sizeof(f4) * (size_t)(h->m.nV > 0 ? h->m.nV : 1), h->stream));
+    HIP_CHECK(h, launch_pass(2, h->m, 0, h->m.nFree, pass_lpv(h), 256, h->d_w, nullptr, h->d_out, nullptr, nullptr,
+                             step_mu(h), step_lambda(h), step_sdt(h), contact_of(h->params), h->stream));
+    HIP_CHECK(h, download_vec(h, h->d_out, out));
+    return VBD_OK;
+}
+
+int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_iterations, int32_t check_every,
+                      float rel_tol, float rho, int32_t* iterations, double* rel_residual) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    if ((!w || !rhs) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
+    if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_scratch(h));
+    const ContactArgs ct = contact_of(h->params);
+    const float mu = step_mu(h), lam = step_lambda(h), sdt = step_sdt(h);
+    const int lpv = pass_lpv(h);
+    HIP_CHECK(h, upload_vec(h, rhs, h->d_rhs, true));
+    HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
+    double b2 = 0, r2 = 0;
+    HIP_CHECK(h, launch_norm2_f4(h->m.nFree, h->d_rhs, h->d_r2, h->stream));
+    HIP_CHECK(h, sum_r2(h, h->m.nFree, &b2));
+    const double bn = std::sqrt(b2);
+    float omega = 1.0f;
+    int it = 0;
+    bool checked_last = false;
+    for (it = 0; it < max_iterations && bn > 0;) {
+        f4* save = rho > 0.f ? h->d_wsave[it & 1] : nullptr;
+        for (int c = 0; c < h->m.nColors; ++c) {
+            const int s0 = h->m.colorSlotOff[c], cnt = h->m.colorSlotOff[c + 1] - s0;
+            HIP_CHECK(h, launch_pass(0, h->m, s0, cnt, lpv, 256, h->d_w, h->d_rhs, nullptr, nullptr, save, mu, lam, sdt,
+                                     ct, h->stream));
+        }
+        if (rho > 0.f) {
+            omega = omega_ref(it + 1, rho, omega);
+            if (omega > 1.0f) HIP_CHECK(h, launch_blend_vec(h->m.nFree, h->d_w, h->d_wsave[(it + 1) & 1], omega, h->stream));
+        }
+        ++it;
+        checked_last = false;
+        if (it % check_every == 0 || it == max_iterations) {
+            HIP_CHECK(h, launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2, nullptr, mu,
+                                     lam, sdt, ct, h->stream));
+            HIP_CHECK(h, sum_r2(h, h->m.nFree, &r2));
+            checked_last = true;
+            if (std::sqrt(r2) <= rel_tol * bn) break;
+        }
+    }
+    if (!checked_last && bn > 0) {
+        HIP_CHECK(h, launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2, nullptr, mu, lam,
+                                 sdt, ct, h->stream));
+        HIP_CHECK(h, sum_r2(h, h->m.nFree, &r2));
+    }
+    HIP_CHECK(h, download_vec(h, h->d_w, w));
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = bn > 0 ? std::sqrt(r2) / bn : 0.0;
+    return VBD_OK;
+}
+
+int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dlambda) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((!w || !dmu || !dlambda) && h->m.nT) return fail(h, VBD_ERR_ARG, "null argument");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_scratch(h));
+    HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
+    HIP_CHECK(h, launch_param_grad(h->m, h->d_w, h->d_tgrad, h->d_tgrad + h->m.nT, h->stream));
+    if (h->m.nT) {
+        HIP_CHECK(h, hipMemcpyAsync(dmu, h->d_tgrad, sizeof(float) * h->m.nT, hipMemcpyDeviceToHost, h->stream));
+        HIP_CHECK(h, hipMemcpyAsync(dlambda, h->d_tgrad + h->m.nT, sizeof(float) * h->m.nT, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+}  // extern "C"
 
 // ---------------------------------------------------------------------------------------
 // host helpers (evaluated without fused multiply-adds, like the reference's scalar code)

@@ -75,6 +75,16 @@ hipError_t launch_scatter_pos(int n, const float* src3, const int* inv, f4* dst,
 hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, hipStream_t s);
 hipError_t launch_energy(const DevMesh& m, float mu, float lambda, double* out, hipStream_t s);
 hipError_t launch_copy_f4(int n, const f4* src, f4* dst, hipStream_t s);
+// differentiation passes (mode: 0 adjoint GS update, 1 adjoint residual, 2 A*w, 3 forward residual)
+hipError_t launch_pass(int mode, const DevMesh& m, int slot0, int count, int lpv, int block, f4* w, const f4* rhs,
+                       f4* out, double* r2, f4* save, float mu, float lambda, float sdt, const ContactArgs& ct,
+                       hipStream_t s);
+hipError_t launch_sum_fixed(int n, const double* v, double* out, hipStream_t s);
+hipError_t launch_norm2_f4(int n, const f4* v, double* r2, hipStream_t s);
+hipError_t launch_gravity_load2(const DevMesh& m, const float g[3], double* r2, hipStream_t s);
+hipError_t launch_param_grad(const DevMesh& m, const f4* w, float* dmu, float* dlam, hipStream_t s);
+hipError_t launch_update_pmat(const DevMesh& m, hipStream_t s);
+hipError_t launch_blend_vec(int n, f4* v, const f4* pp, float omega, hipStream_t s);
 
 // Device mesh build.  Host inputs are already on the device (d_* pointers, caller numbering).
 // colorOf: color per vertex (host coloring); fixed: pinned flags.  Allocates every DevMesh

@@ -308,6 +308,403 @@ hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int bl
 }
 
 // ---------------------------------------------------------------------------------------
+// Differentiation kernels.
+//
+// With A = M/h^2 + K(x) (K = Hessian of E, plus the penalty-contact term k e_a e_a^T where a
+// vertex penetrates), the adjoint of one implicit step solves A w = rhs.  The per-(vertex, tet)
+// Hessian-vector product needs the directional derivative of P along dF = sum_n w_n q_n^T:
+//   dP = mu (dF - dC) + lambda (C : dF) C + lambda (J - 1) dC,   dC = d cof(F)[dF]
+// (Energy.cpp:76-88 and :123-127), and (K_t w)_k = V dP q_k.
+// ---------------------------------------------------------------------------------------
+enum PassMode { MODE_ADJ_SOLVE = 0, MODE_ADJ_RESID = 1, MODE_HVP = 2, MODE_FWD_RESID = 3 };
+
+__device__ __forceinline__ void pair_hvp(const Kin& k, const bool corner0, const f4 a, const f4 b, const float c,
+                                         const float V, const float tmu, const float tlam, const f4 ws, const f4 wa,
+                                         const f4 wb, const f4 wc, float& o0, float& o1, float& o2) {
+    const f4 base = corner0 ? ws : wa;
+    const f4 first = corner0 ? wa : ws;
+    const float d0x = first.x - base.x, d0y = first.y - base.y, d0z = first.z - base.z;
+    const float d1x = wb.x - base.x, d1y = wb.y - base.y, d1z = wb.z - base.z;
+    const float d2x = wc.x - base.x, d2y = wc.y - base.y, d2z = wc.z - base.z;
+    const float G00 = d0x * a.x + d1x * a.w + d2x * b.z;
+    const float G01 = d0x * a.y + d1x * b.x + d2x * b.w;
+    const float G02 = d0x * a.z + d1x * b.y + d2x * c;
+    const float G10 = d0y * a.x + d1y * a.w + d2y * b.z;
+    const float G11 = d0y * a.y + d1y * b.x + d2y * b.w;
+    const float G12 = d0y * a.z + d1y * b.y + d2y * c;
+    const float G20 = d0z * a.x + d1z * a.w + d2z * b.z;
+    const float G21 = d0z * a.y + d1z * b.x + d2z * b.w;
+    const float G22 = d0z * a.z + d1z * b.y + d2z * c;
+    const float dJ = k.C00 * G00 + k.C01 * G01 + k.C02 * G02 + k.C10 * G10 + k.C11 * G11 + k.C12 * G12 +
+                     k.C20 * G20 + k.C21 * G21 + k.C22 * G22;
+    // cofactor derivative (Energy.cpp:76-88) with dF = G
+    const float dC00 = G11 * k.F22 + k.F11 * G22 - G12 * k.F21 - k.F12 * G21;
+    const float dC01 = G12 * k.F20 + k.F12 * G20 - G10 * k.F22 - k.F10 * G22;
+    const float dC02 = G10 * k.F21 + k.F10 * G21 - G11 * k.F20 - k.F11 * G20;
+    const float dC10 = G21 * k.F02 + k.F21 * G02 - G22 * k.F01 - k.F22 * G01;
+    const float dC11 = G22 * k.F00 + k.F22 * G00 - G20 * k.F02 - k.F20 * G02;
+    const float dC12 = G20 * k.F01 + k.F20 * G01 - G21 * k.F00 - k.F21 * G00;
+    const float dC20 = G01 * k.F12 + k.F01 * G12 - G02 * k.F11 - k.F02 * G11;
+    const float dC21 = G02 * k.F10 + k.F02 * G10 - G00 * k.F12 - k.F00 * G12;
+    const float dC22 = G00 * k.F11 + k.F00 * G11 - G01 * k.F10 - k.F01 * G10;
+    const float ld = tlam * dJ;
+    const float aj = tlam * (k.J - 1.0f);
+    const float q0 = k.q0, q1 = k.q1, q2 = k.q2;
+#define VBD_DP(r, c) (tmu * (G##r##c - dC##r##c) + ld * k.C##r##c + aj * dC##r##c)
+    o0 = V * (VBD_DP(0, 0) * q0 + VBD_DP(0, 1) * q1 + VBD_DP(0, 2) * q2);
+    o1 = V * (VBD_DP(1, 0) * q0 + VBD_DP(1, 1) * q1 + VBD_DP(1, 2) * q2);
+    o2 = V * (VBD_DP(2, 0) * q0 + VBD_DP(2, 1) * q1 + VBD_DP(2, 2) * q2);
+#undef VBD_DP
+}
+
+// One pass over free vertices [slot0, slot0 + count) with LPV lanes per vertex.
+//   MODE_ADJ_SOLVE:  block Gauss-Seidel update of w for one color: r = rhs - (A w)_s,
+//                    w_s += D_ss^-1 r with D_ss the exact 3x3 diagonal block of A; SAVE keeps the
+//                    pre-update w_s for Chebyshev acceleration.
+//   MODE_ADJ_RESID:  out[s] = rhs_s - (A w)_s,  r2[s] = |out[s]|^2
+//   MODE_HVP:        out[s] = (A w)_s
+//   MODE_FWD_RESID:  out[s] = grad G = m/h^2 (x_s - y_s) + sum_t g + contact,  r2[s] = |.|^2
+template <int MODE, int LPV, bool MAT, bool CONTACT, bool SAVE>
+__global__ __launch_bounds__(256) void k_vertex_pass(int slot0, int count, int nslots, const f4* __restrict__ pos,
+                                                     const f4* __restrict__ inert, const float* __restrict__ mass,
+                                                     const int* __restrict__ row, const i4* __restrict__ ptet,
+                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
+                                                     f4* __restrict__ w, const f4* __restrict__ rhs,
+                                                     f4* __restrict__ out, double* __restrict__ r2,
+                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
+                                                     ContactArgs ct) {
+    const int gpb = blockDim.x / LPV;
+    const int g = xcd_block() * gpb + (int)threadIdx.x / LPV;
+    const int lane = (int)threadIdx.x % LPV;
+    const bool valid = g < count;
+    const int s = slot0 + g;
+    constexpr bool NEED_W = MODE != MODE_FWD_RESID;
+
+    Acc A = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float kw0 = 0.f, kw1 = 0.f, kw2 = 0.f;
+    f4 xs = {0.f, 0.f, 0.f, 0.f}, ws = {0.f, 0.f, 0.f, 0.f};
+    if (valid) {
+        const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, nslots);
+        __amdgpu_buffer_rsrc_t wrs = prs;
+        if constexpr (NEED_W) {
+            wrs = pos_rsrc(w, nslots);
+            ws = w[s];
+        }
+        xs = pos[s];
+        const int p0 = row[s], p1 = row[s + 1];
+        for (int p = p0 + lane; p < p1; p += LPV) {
+            const i4 t = ptet[p];
+            const f4 a = pa[p];
+            const f4 b = pb[p];
+            const float c = pc[p];
+            float tmu = mu, tlam = lambda;
+            if constexpr (MAT) {
+                const f2 mm = pmat[p];
+                tmu = mm.x;
+                tlam = mm.y;
+            }
+            const int sa = t.x & SLOT_MASK;
+            const bool c0 = t.x < 0;
+            const f4 ya = gather_pos(prs, sa), yb = gather_pos(prs, t.y), yc = gather_pos(prs, t.z);
+            const float V = __int_as_float(t.w);
+            if constexpr (MODE == MODE_FWD_RESID) {
+                pair_accumulate(c0, a, b, c, V, tmu, tlam, xs, ya, yb, yc, A);
+            } else {
+                const f4 wa = gather_pos(wrs, sa), wb = gather_pos(wrs, t.y), wc = gather_pos(wrs, t.z);
+                const Kin k = pair_kinematics(c0, a, b, c, xs, ya, yb, yc);
+                float o0, o1, o2;
+                pair_hvp(k, c0, a, b, c, V, tmu, tlam, ws, wa, wb, wc, o0, o1, o2);
+                kw0 += o0;
+                kw1 += o1;
+                kw2 += o2;
+                if constexpr (MODE == MODE_ADJ_SOLVE) {
+                    const float Cq0 = k.C00 * k.q0 + k.C01 * k.q1 + k.C02 * k.q2;
+                    const float Cq1 = k.C10 * k.q0 + k.C11 * k.q1 + k.C12 * k.q2;
+                    const float Cq2 = k.C20 * k.q0 + k.C21 * k.q1 + k.C22 * k.q2;
+                    const float vm = V * tmu * (k.q0 * k.q0 + k.q1 * k.q1 + k.q2 * k.q2);
+                    const float vl = V * tlam;
+                    A.h00 += vm + vl * Cq0 * Cq0;
+                    A.h01 += vl * Cq0 * Cq1;
+                    A.h02 += vl * Cq0 * Cq2;
+                    A.h11 += vm + vl * Cq1 * Cq1;
+                    A.h12 += vl * Cq1 * Cq2;
+                    A.h22 += vm + vl * Cq2 * Cq2;
+                }
+            }
+        }
+    }
+    kw0 = gsum<LPV>(kw0); kw1 = gsum<LPV>(kw1); kw2 = gsum<LPV>(kw2);
+    float h00 = 0.f, h01 = 0.f, h02 = 0.f, h11 = 0.f, h12 = 0.f, h22 = 0.f, r0 = 0.f, r1 = 0.f, r2v = 0.f;
+    if constexpr (MODE == MODE_ADJ_SOLVE) {
+        h00 = gsum<LPV>(A.h00); h01 = gsum<LPV>(A.h01); h02 = gsum<LPV>(A.h02);
+        h11 = gsum<LPV>(A.h11); h12 = gsum<LPV>(A.h12); h22 = gsum<LPV>(A.h22);
+    }
+    if constexpr (MODE == MODE_FWD_RESID) {
+        r0 = gsum<LPV>(A.r0); r1 = gsum<LPV>(A.r1); r2v = gsum<LPV>(A.r2);
+    }
+    if (!valid || lane != LPV - 1) return;
+    const float m = mass[s];
+    const float ms = m * sdt;
+    // penalty contact: Hessian k e_a e_a^T and gradient -k (plane - x_a) e_a while penetrating
+    float ck = 0.f, cd = 0.f;
+    int cax = 0;
+    if constexpr (CONTACT) {
+        cax = ct.axis;
+        const float xa = cax == 0 ? xs.x : (cax == 1 ? xs.y : xs.z);
+        cd = ct.plane - xa;
+        if (cd > 0.f) ck = ct.k;
+        else cd = 0.f;
+    }
+    if constexpr (MODE == MODE_FWD_RESID) {
+        const f4 ys = inert[s];
+        f4 o;
+        o.x = (m * (xs.x - ys.x)) * sdt - r0;
+        o.y = (m * (xs.y - ys.y)) * sdt - r1;
+        o.z = (m * (xs.z - ys.z)) * sdt - r2v;
+        o.w = 0.f;
+        if constexpr (CONTACT) {
+            if (cax == 0) o.x -= ck * cd;
+            else if (cax == 1) o.y -= ck * cd;
+            else o.z -= ck * cd;
+        }
+        out[s] = o;
+        r2[s] = (double)o.x * o.x + (double)o.y * o.y + (double)o.z * o.z;
+        return;
+    }
+    // (A w)_s
+    float aw0 = ms * ws.x + kw0, aw1 = ms * ws.y + kw1, aw2 = ms * ws.z + kw2;
+    if constexpr (CONTACT) {
+        if (cax == 0) aw0 += ck * ws.x;
+        else if (cax == 1) aw1 += ck * ws.y;
+        else aw2 += ck * ws.z;
+    }
+    if constexpr (MODE == MODE_HVP) {
+        f4 o;
+        o.x = aw0; o.y = aw1; o.z = aw2; o.w = 0.f;
+        out[s] = o;
+        return;
+    }
+    const f4 b4 = rhs[s];
+    const float e0 = b4.x - aw0, e1 = b4.y - aw1, e2 = b4.z - aw2;
+    if constexpr (MODE == MODE_ADJ_RESID) {
+        f4 o;
+        o.x = e0; o.y = e1; o.z = e2; o.w = 0.f;
+        out[s] = o;
+        r2[s] = (double)e0 * e0 + (double)e1 * e1 + (double)e2 * e2;
+        return;
+    }
+    // MODE_ADJ_SOLVE
+    h00 += ms; h11 += ms; h22 += ms;
+    if constexpr (CONTACT) {
+        if (cax == 0) h00 += ck;
+        else if (cax == 1) h11 += ck;
+        else h22 += ck;
+    }
+    const float det = h00 * (h11 * h22 - h12 * h12) - h01 * (h01 * h22 - h12 * h02) + h02 * (h01 * h12 - h11 * h02);
+    f4 wn = ws;
+    if (fabsf(det) > 1e-8f) {
+        const float inv = 1.0f / det;
+        const float a00 = h11 * h22 - h12 * h12, a01 = h02 * h12 - h01 * h22, a02 = h01 * h12 - h02 * h11;
+        const float a11 = h00 * h22 - h02 * h02, a12 = h02 * h01 - h00 * h12, a22 = h00 * h11 - h01 * h01;
+        wn.x = ws.x + inv * (a00 * e0 + a01 * e1 + a02 * e2);
+        wn.y = ws.y + inv * (a01 * e0 + a11 * e1 + a12 * e2);
+        wn.z = ws.z + inv * (a02 * e0 + a12 * e1 + a22 * e2);
+    }
+    if constexpr (SAVE) save[s] = ws;
+    w[s] = wn;
+}
+
+template <int MODE, int LPV, bool MAT, bool CONTACT, bool SAVE>
+static hipError_t launch_pass_t(const DevMesh& m, int slot0, int count, int block, f4* w, const f4* rhs, f4* out,
+                                double* r2, f4* save, float mu, float lambda, float sdt, const ContactArgs& ct,
+                                hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int gpb = block / LPV;
+    int nb = (count + gpb - 1) / gpb;
+    nb = (nb + 7) & ~7;
+    hipLaunchKernelGGL((k_vertex_pass<MODE, LPV, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.nV,
+                       m.pos, m.inert, m.mass, m.row, m.ptet, m.pa, m.pb, m.pc, m.pmat, w, rhs, out, r2, save, mu,
+                       lambda, sdt, ct);
+    return hipGetLastError();
+}
+
+template <int MODE, int LPV>
+static hipError_t launch_pass_l(const DevMesh& m, int slot0, int count, int block, f4* w, const f4* rhs, f4* out,
+                                double* r2, f4* save, float mu, float lambda, float sdt, const ContactArgs& ct,
+                                hipStream_t s) {
+    const bool mat = m.perTetMat, con = ct.on != 0, sv = save != nullptr;
+#define VBD_PD(MT, CT, SV) \
+    if (mat == MT && con == CT && sv == SV) \
+        return launch_pass_t<MODE, LPV, MT, CT, SV>(m, slot0, count, block, w, rhs, out, r2, save, mu, lambda, sdt, ct, s);
+    if constexpr (MODE == MODE_ADJ_SOLVE) {
+        VBD_PD(false, false, true) VBD_PD(false, true, true) VBD_PD(true, false, true) VBD_PD(true, true, true)
+    }
+    VBD_PD(false, false, false) VBD_PD(false, true, false) VBD_PD(true, false, false) VBD_PD(true, true, false)
+#undef VBD_PD
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass(int mode, const DevMesh& m, int slot0, int count, int lpv, int block, f4* w, const f4* rhs,
+                       f4* out, double* r2, f4* save, float mu, float lambda, float sdt, const ContactArgs& ct,
+                       hipStream_t s) {
+    if (block < 64 || block > 256 || (block % 64) != 0 || (block % lpv) != 0) return hipErrorInvalidValue;
+#define VBD_PM(MD, L) \
+    if (mode == MD && lpv == L) return launch_pass_l<MD, L>(m, slot0, count, block, w, rhs, out, r2, save, mu, lambda, sdt, ct, s);
+    VBD_PM(MODE_ADJ_SOLVE, 8) VBD_PM(MODE_ADJ_SOLVE, 16) VBD_PM(MODE_ADJ_SOLVE, 32)
+    VBD_PM(MODE_ADJ_RESID, 8) VBD_PM(MODE_ADJ_RESID, 16) VBD_PM(MODE_ADJ_RESID, 32)
+    VBD_PM(MODE_HVP, 8) VBD_PM(MODE_HVP, 16) VBD_PM(MODE_HVP, 32)
+    VBD_PM(MODE_FWD_RESID, 8) VBD_PM(MODE_FWD_RESID, 16) VBD_PM(MODE_FWD_RESID, 32)
+#undef VBD_PM
+    return hipErrorInvalidValue;
+}
+
+// Deterministic sum of n doubles: one block, fixed strided partition, fixed tree.
+__global__ __launch_bounds__(1024) void k_sum_fixed(int n, const double* __restrict__ v, double* __restrict__ out) {
+    __shared__ double sh[1024];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) acc += v[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_norm2_f4(int n, const f4* __restrict__ v, double* __restrict__ r2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f4 a = v[i];
+    r2[i] = (double)a.x * a.x + (double)a.y * a.y + (double)a.z * a.z;
+}
+
+hipError_t launch_norm2_f4(int n, const f4* v, double* r2, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_norm2_f4, dim3((n + 255) / 256), dim3(256), 0, s, n, v, r2);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_fixed(int n, const double* v, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, s, n, v, out);
+    return hipGetLastError();
+}
+
+// |m_s g|^2 per free slot (scale of the forward convergence test)
+__global__ __launch_bounds__(256) void k_gravity_load2(int n, const float* __restrict__ mass, float gx, float gy,
+                                                       float gz, double* __restrict__ r2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = mass[i];
+    r2[i] = m * m * ((double)gx * gx + (double)gy * gy + (double)gz * gz);
+}
+
+hipError_t launch_gravity_load2(const DevMesh& m, const float g[3], double* r2, hipStream_t s) {
+    if (m.nFree == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gravity_load2, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.mass, g[0], g[1],
+                       g[2], r2);
+    return hipGetLastError();
+}
+
+// Per-tet parameter gradients -(d^2E / dx dtheta)^T w in caller tet order:
+//   dmu_t = -V (F - C) : dF(w),   dlambda_t = -V (J - 1) C : dF(w)
+__global__ __launch_bounds__(256) void k_param_grad(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
+                                                    const f4* __restrict__ pos, const f4* __restrict__ w,
+                                                    const float* __restrict__ dminv, const float* __restrict__ vol,
+                                                    float* __restrict__ dmu, float* __restrict__ dlam) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const i4 tt = tets[t];
+    const int sl[4] = {inv[tt.x], inv[tt.y], inv[tt.z], inv[tt.w]};
+    const f4 x0 = pos[sl[0]], x1 = pos[sl[1]], x2 = pos[sl[2]], x3 = pos[sl[3]];
+    const f4 w0 = w[sl[0]], w1 = w[sl[1]], w2 = w[sl[2]], w3 = w[sl[3]];
+    const float* D = dminv + 9 * (size_t)t;
+    const float ds[3][3] = {{x1.x - x0.x, x2.x - x0.x, x3.x - x0.x},
+                            {x1.y - x0.y, x2.y - x0.y, x3.y - x0.y},
+                            {x1.z - x0.z, x2.z - x0.z, x3.z - x0.z}};
+    const float dw[3][3] = {{w1.x - w0.x, w2.x - w0.x, w3.x - w0.x},
+                            {w1.y - w0.y, w2.y - w0.y, w3.y - w0.y},
+                            {w1.z - w0.z, w2.z - w0.z, w3.z - w0.z}};
+    float F[3][3], G[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            F[r][c] = ds[r][0] * D[c] + ds[r][1] * D[3 + c] + ds[r][2] * D[6 + c];
+            G[r][c] = dw[r][0] * D[c] + dw[r][1] * D[3 + c] + dw[r][2] * D[6 + c];
+        }
+    float C[3][3];
+    C[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    C[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    C[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    C[1][0] = F[2][1] * F[0][2] - F[2][2] * F[0][1];
+    C[1][1] = F[2][2] * F[0][0] - F[2][0] * F[0][2];
+    C[1][2] = F[2][0] * F[0][1] - F[2][1] * F[0][0];
+    C[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    C[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    C[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const float J = F[0][0] * C[0][0] + F[0][1] * C[0][1] + F[0][2] * C[0][2];
+    float fc = 0.f, cg = 0.f;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            fc += (F[r][c] - C[r][c]) * G[r][c];
+            cg += C[r][c] * G[r][c];
+        }
+    dmu[t] = -vol[t] * fc;
+    dlam[t] = -vol[t] * (J - 1.0f) * cg;
+}
+
+hipError_t launch_param_grad(const DevMesh& m, const f4* w, float* dmu, float* dlam, hipStream_t s) {
+    if (m.nT == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_param_grad, dim3((m.nT + 255) / 256), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos, w, m.dminv,
+                       m.vol, dmu, dlam);
+    return hipGetLastError();
+}
+
+// Rewrite the per-record (mu, lambda) stream from per-tet arrays (same record order as the build).
+__global__ __launch_bounds__(256) void k_update_pmat(int nFree, const int* __restrict__ perm,
+                                                     const int* __restrict__ off, const int* __restrict__ entry,
+                                                     const int* __restrict__ row, const float* __restrict__ tmu,
+                                                     const float* __restrict__ tlam, f2* __restrict__ pmat) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nFree) return;
+    const int v = perm[s];
+    int p = row[s];
+    for (int e = off[v]; e < off[v + 1]; ++e, ++p) {
+        const int t = entry[e] >> 2;
+        f2 mm;
+        mm.x = tmu[t];
+        mm.y = tlam[t];
+        pmat[p] = mm;
+    }
+}
+
+hipError_t launch_update_pmat(const DevMesh& m, hipStream_t s) {
+    if (m.nFree == 0 || !m.perTetMat) return hipSuccess;
+    hipLaunchKernelGGL(k_update_pmat, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.perm, m.adjOff,
+                       m.adjEntry, m.row, m.tmu, m.tlam, m.pmat);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_blend_range(int n, f4* __restrict__ v, const f4* __restrict__ pp,
+                                                     float omega) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f4 x = v[i];
+    const f4 q = pp[i];
+    f4 r = x;
+    r.x = omega * (x.x - q.x) + q.x;
+    r.y = omega * (x.y - q.y) + q.y;
+    r.z = omega * (x.z - q.z) + q.z;
+    v[i] = r;
+}
+
+hipError_t launch_blend_vec(int n, f4* v, const f4* pp, float omega, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_blend_range, dim3((n + 255) / 256), dim3(256), 0, s, n, v, pp, omega);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
 // Chebyshev blend over the free slots: x = omega (x - x_pp) + x_pp   (VBD.cpp:290-300)
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_blend(int n, f4* __restrict__ pos, const f4* __restrict__ pp, float omega) {

@@ -206,6 +206,20 @@ def diff_cube(n: int = 16, side: float = 1.0, seed: int = 0, lo: float = 5e4, hi
                 gravity=(0.0, -9.8, 0.0), meta=dict(n=n, cell=cell, seed=seed))
 
 
+def diff_workload(n: int = 16, seed: int = 0, v_scale: float = 0.05):
+    """The differentiable workload: (mesh with per-tet mu field (seed), lambda = 4 mu,
+    ground-truth mu field (seed + 1), its lambda = 4 mu, initial velocity v0 (N,3)).
+    v0 is a seeded smooth field of magnitude ~v_scale m/s on free vertices."""
+    m = diff_cube(n, seed=seed)
+    cent = m.rest[m.tets].astype(np.float64).mean(axis=1)
+    mu_true = smooth_field(cent, seed + 1, 5e4, 2e5).astype(np.float32)
+    lam_true = (4.0 * mu_true).astype(np.float32)
+    p = m.rest.astype(np.float64)
+    v0 = np.stack([smooth_field(p, seed + 10 + c, -v_scale, v_scale) for c in range(3)], axis=1)
+    v0[m.fixed == 1] = 0.0
+    return m, mu_true, lam_true, v0.astype(np.float32)
+
+
 def concat(meshes: List[Mesh], name: str = "batch") -> tuple:
     """Disjoint union of meshes: returns (Mesh with per-tet materials, vert_offsets, tet_offsets)."""
     vo = np.zeros(len(meshes) + 1, dtype=np.int64)

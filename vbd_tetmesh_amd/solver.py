@@ -354,6 +354,50 @@ class DeviceMesh(_Handle):
         check(lib.vbd_total_energy(self.h, mu, lam, C.byref(e)), self.h)
         return e.value
 
+    # -- differentiation
+    def set_materials(self, tet_mu, tet_lambda):
+        mu = np.ascontiguousarray(tet_mu, np.float32)
+        lam = np.ascontiguousarray(tet_lambda, np.float32)
+        check(lib.vbd_set_materials(self.h, ptr(mu, C.c_float), ptr(lam, C.c_float)), self.h)
+
+    def residual_norm(self):
+        """(|grad G| at the current state, |M g|)."""
+        r, l = C.c_double(), C.c_double()
+        check(lib.vbd_residual_norm(self.h, C.byref(r), C.byref(l)), self.h)
+        return r.value, l.value
+
+    def step_converged(self, max_iterations: int, check_every: int = 10, abs_tol: float = 0.0, rel_tol: float = 1e-5):
+        """One converged time step; returns (iterations, final |grad G|)."""
+        it, nrm = C.c_int32(), C.c_double()
+        check(lib.vbd_step_converged(self.h, max_iterations, check_every, abs_tol, rel_tol, C.byref(it), C.byref(nrm)),
+              self.h)
+        return it.value, nrm.value
+
+    def hessian_vector(self, w) -> np.ndarray:
+        w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+        out = np.zeros_like(w)
+        check(lib.vbd_hessian_vector(self.h, ptr(w, C.c_float), ptr(out, C.c_float)), self.h)
+        return out
+
+    def adjoint_solve(self, rhs, w0=None, max_iterations: int = 5000, check_every: int = 10, rel_tol: float = 1e-6,
+                      rho: float = 0.0):
+        """Solve (M/h^2 + K) w = rhs; returns (w, sweeps, relative residual)."""
+        rhs = np.ascontiguousarray(rhs, np.float32).reshape(-1, 3)
+        w = np.zeros_like(rhs) if w0 is None else np.ascontiguousarray(w0, np.float32).reshape(-1, 3).copy()
+        it, rr = C.c_int32(), C.c_double()
+        check(lib.vbd_adjoint_solve(self.h, ptr(rhs, C.c_float), ptr(w, C.c_float), max_iterations, check_every,
+                                    rel_tol, rho, C.byref(it), C.byref(rr)), self.h)
+        return w, it.value, rr.value
+
+    def parameter_gradients(self, w):
+        w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+        nt = self.info().num_tets
+        dmu = np.zeros(max(nt, 1), np.float32)
+        dlam = np.zeros(max(nt, 1), np.float32)
+        check(lib.vbd_parameter_gradients(self.h, ptr(w, C.c_float), ptr(dmu, C.c_float), ptr(dlam, C.c_float)),
+              self.h)
+        return dmu[:nt], dlam[:nt]
+
 
 class VBD_GPU_Solver:
     """VBD_GPU_Solver (VBD_GPU_Wrapper.h:7-45) on the gfx950 C ABI."""
