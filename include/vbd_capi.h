@@ -194,18 +194,21 @@ int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy);
 int vbd_set_materials(vbd_solver* h, const float* tet_mu, const float* tet_lambda);
 /* |grad G| at the current positions and inertia target, and |M g| (the gravity load). */
 int vbd_residual_norm(vbd_solver* h, double* norm, double* gravity_load);
-/* One time step (substeps == 1) iterated until |grad G| <= max(abs_tol, rel_tol |M g|),
- * tested every check_every iterations, or max_iterations; then the velocity update.
- * Deterministic: the same inputs give bit-identical results (no atomics in any reduction). */
+/* One time step (substeps == 1) iterated until |grad G| <= max(abs_tol, rel_tol |M g|), tested
+ * every check_every iterations, or until the best |grad G| has not improved by 1% for
+ * stall_checks consecutive tests (0 disables; the float32 floor of |grad G|), or max_iterations;
+ * then the velocity update.  Deterministic: the same inputs give bit-identical results (no
+ * atomics in any reduction). */
 int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_every, float abs_tol, float rel_tol,
-                       int32_t* iterations, double* final_norm);
+                       int32_t stall_checks, int32_t* iterations, double* final_norm);
 /* out = A w at the current positions (rows of free vertices). */
 int vbd_hessian_vector(vbd_solver* h, const float* w, float* out);
 /* Solves A w = rhs by colored block Gauss-Seidel (the forward colors, exact 3x3 diagonal blocks)
  * with Chebyshev acceleration when rho > 0.  w is the initial guess on entry and the solution
- * on exit; stops when |rhs - A w| <= rel_tol |rhs| (tested every check_every sweeps). */
+ * on exit; stops when |rhs - A w| <= rel_tol |rhs| (tested every check_every sweeps) or on
+ * stagnation as in vbd_step_converged. */
 int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_iterations, int32_t check_every,
-                      float rel_tol, float rho, int32_t* iterations, double* rel_residual);
+                      float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual);
 /* Per-tet -(d^2 E / dx dmu)^T w and -(d^2 E / dx dlambda)^T w at the current positions:
  * dmu_t = -V (F - C) : dF(w), dlambda_t = -V (J - 1) C : dF(w), in the caller's tet order. */
 int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dlambda);

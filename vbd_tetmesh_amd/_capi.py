@@ -121,9 +121,10 @@ SIGNATURES = [
     ("vbd_total_energy", C.c_int, [H, C.c_float, C.c_float, f64p]),
     ("vbd_set_materials", C.c_int, [H, f32p, f32p]),
     ("vbd_residual_norm", C.c_int, [H, f64p, f64p]),
-    ("vbd_step_converged", C.c_int, [H, C.c_int32, C.c_int32, C.c_float, C.c_float, i32p, f64p]),
+    ("vbd_step_converged", C.c_int, [H, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, i32p, f64p]),
     ("vbd_hessian_vector", C.c_int, [H, f32p, f32p]),
-    ("vbd_adjoint_solve", C.c_int, [H, f32p, f32p, C.c_int32, C.c_int32, C.c_float, C.c_float, i32p, f64p]),
+    ("vbd_adjoint_solve", C.c_int, [H, f32p, f32p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, i32p,
+                                    f64p]),
     ("vbd_parameter_gradients", C.c_int, [H, f32p, f32p, f32p]),
 ]
 

@@ -39,9 +39,9 @@ class DiffSim:
     """Device rollout + adjoint for one mesh with per-tet materials."""
 
     def __init__(self, mesh, steps: int = 30, h: float = 1.0 / 60.0, rho: float = 0.95, max_iterations: int = 4000,
-                 check_every: int = 10, rel_tol: float = 1e-5, adj_max_iterations: int = 8000,
-                 adj_rel_tol: float = 1e-5, adj_rho: float = 0.9, checkpoint_every: int = 5, device: int = 0,
-                 damping: float = 0.999):
+                 check_every: int = 10, rel_tol: float = 1e-6, stall_checks: int = 5, adj_max_iterations: int = 8000,
+                 adj_rel_tol: float = 1e-5, adj_rho: float = 0.9, adj_stall_checks: int = 5, checkpoint_every: int = 5,
+                 device: int = 0, damping: float = 0.999):
         if mesh.tet_mu is None:
             raise ValueError("the differentiable rollout needs per-tet materials (mesh.tet_mu / tet_lambda)")
         t, off, verts, dminv, vol = prepare_host(mesh)
@@ -52,6 +52,7 @@ class DiffSim:
         self.dev.set_params(self.params)
         self.steps, self.h, self.c = steps, h, damping
         self.max_iterations, self.check_every, self.rel_tol = max_iterations, check_every, rel_tol
+        self.stall_checks, self.adj_stall_checks = stall_checks, adj_stall_checks
         self.adj_max_iterations, self.adj_rel_tol, self.adj_rho = adj_max_iterations, adj_rel_tol, adj_rho
         self.k = max(1, checkpoint_every)
         self.x0 = t.mVertPos.copy()
@@ -62,7 +63,7 @@ class DiffSim:
 
     # -- forward
     def _step(self):
-        it, res = self.dev.step_converged(self.max_iterations, self.check_every, 0.0, self.rel_tol)
+        it, res = self.dev.step_converged(self.max_iterations, self.check_every, 0.0, self.rel_tol, self.stall_checks)
         return it, res
 
     def rollout(self, mu, lam, v0) -> Tuple[np.ndarray, Dict[int, Tuple[np.ndarray, np.ndarray]]]:
@@ -118,7 +119,7 @@ class DiffSim:
                 self.dev.set_positions(xs[n + 1 - start])
                 w, sweeps, rr = self.dev.adjoint_solve(ahat, w0=w, max_iterations=self.adj_max_iterations,
                                                        check_every=self.check_every, rel_tol=self.adj_rel_tol,
-                                                       rho=self.adj_rho)
+                                                       rho=self.adj_rho, stall_checks=self.adj_stall_checks)
                 self.stats.adjoint_sweeps.append(sweeps)
                 self.stats.adjoint_residuals.append(rr)
                 dm, dl = self.dev.parameter_gradients(w)

@@ -952,7 +952,7 @@ int vbd_residual_norm(vbd_solver* h, double* norm, double* gravity_load) {
 }
 
 int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_every, float abs_tol, float rel_tol,
-                       int32_t* iterations, double* final_norm) {
+                       int32_t stall_checks, int32_t* iterations, double* final_norm) {
     int rc = check_ready(h);
     if (rc) return rc;
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
@@ -971,8 +971,8 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
     HIP_CHECK(h, launch_predict(h->m, hstep, p.gravity, h->stream));
     if (p.start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
     float omega = 1.0f;
-    int it = 0;
-    double r2 = 0;
+    int it = 0, stalled = 0;
+    double r2 = 0, best = 1e300;
     bool checked_last = false;
     for (it = 0; it < max_iterations;) {
         f4* save = p.use_acceleration ? h->m.save[it & 1] : nullptr;
@@ -986,7 +986,15 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
         if (it % check_every == 0 || it == max_iterations) {
             HIP_CHECK(h, forward_residual(h, &r2));
             checked_last = true;
-            if (std::sqrt(r2) <= tol) break;
+            const double r = std::sqrt(r2);
+            if (r <= tol) break;
+            // stagnation at the float32 floor: no 1% gain on the best residual for stall_checks checks
+            if (r < 0.99 * best) {
+                best = r;
+                stalled = 0;
+            } else if (stall_checks > 0 && ++stalled >= stall_checks) {
+                break;
+            }
         }
     }
     if (!checked_last) HIP_CHECK(h, forward_residual(h, &r2));
@@ -1013,7 +1021,7 @@ int vbd_hessian_vector(vbd_solver* h, const float* w, float* out) {
 }
 
 int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_iterations, int32_t check_every,
-                      float rel_tol, float rho, int32_t* iterations, double* rel_residual) {
+                      float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual) {
     int rc = check_ready(h);
     if (rc) return rc;
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
@@ -1031,7 +1039,8 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
     HIP_CHECK(h, sum_r2(h, h->m.nFree, &b2));
     const double bn = std::sqrt(b2);
     float omega = 1.0f;
-    int it = 0;
+    int it = 0, stalled = 0;
+    double best = 1e300;
     bool checked_last = false;
     for (it = 0; it < max_iterations && bn > 0;) {
         f4* save = rho > 0.f ? h->d_wsave[it & 1] : nullptr;
@@ -1051,7 +1060,14 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
                                      lam, sdt, ct, h->stream));
             HIP_CHECK(h, sum_r2(h, h->m.nFree, &r2));
             checked_last = true;
-            if (std::sqrt(r2) <= rel_tol * bn) break;
+            const double r = std::sqrt(r2);
+            if (r <= rel_tol * bn) break;
+            if (r < 0.99 * best) {
+                best = r;
+                stalled = 0;
+            } else if (stall_checks > 0 && ++stalled >= stall_checks) {
+                break;
+            }
         }
     }
     if (!checked_last && bn > 0) {

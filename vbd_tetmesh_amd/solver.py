@@ -366,11 +366,12 @@ class DeviceMesh(_Handle):
         check(lib.vbd_residual_norm(self.h, C.byref(r), C.byref(l)), self.h)
         return r.value, l.value
 
-    def step_converged(self, max_iterations: int, check_every: int = 10, abs_tol: float = 0.0, rel_tol: float = 1e-5):
+    def step_converged(self, max_iterations: int, check_every: int = 10, abs_tol: float = 0.0, rel_tol: float = 1e-5,
+                       stall_checks: int = 5):
         """One converged time step; returns (iterations, final |grad G|)."""
         it, nrm = C.c_int32(), C.c_double()
-        check(lib.vbd_step_converged(self.h, max_iterations, check_every, abs_tol, rel_tol, C.byref(it), C.byref(nrm)),
-              self.h)
+        check(lib.vbd_step_converged(self.h, max_iterations, check_every, abs_tol, rel_tol, stall_checks, C.byref(it),
+                                     C.byref(nrm)), self.h)
         return it.value, nrm.value
 
     def hessian_vector(self, w) -> np.ndarray:
@@ -380,13 +381,13 @@ class DeviceMesh(_Handle):
         return out
 
     def adjoint_solve(self, rhs, w0=None, max_iterations: int = 5000, check_every: int = 10, rel_tol: float = 1e-6,
-                      rho: float = 0.0):
+                      rho: float = 0.0, stall_checks: int = 0):
         """Solve (M/h^2 + K) w = rhs; returns (w, sweeps, relative residual)."""
         rhs = np.ascontiguousarray(rhs, np.float32).reshape(-1, 3)
         w = np.zeros_like(rhs) if w0 is None else np.ascontiguousarray(w0, np.float32).reshape(-1, 3).copy()
         it, rr = C.c_int32(), C.c_double()
         check(lib.vbd_adjoint_solve(self.h, ptr(rhs, C.c_float), ptr(w, C.c_float), max_iterations, check_every,
-                                    rel_tol, rho, C.byref(it), C.byref(rr)), self.h)
+                                    rel_tol, rho, stall_checks, C.byref(it), C.byref(rr)), self.h)
         return w, it.value, rr.value
 
     def parameter_gradients(self, w):
