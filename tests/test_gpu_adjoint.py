@@ -1,0 +1,180 @@
+"""GPU differentiation: HVP, parameter gradients, adjoint solve, converged steps and whole
+rollout gradients against the FP64 CPU adjoint oracle (oracle/diff_oracle.py), which is
+itself pinned by central finite differences (tests/test_diff_oracle.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import diff_oracle as D
+from vbd_tetmesh_amd import meshgen
+from vbd_tetmesh_amd.adjoint import DiffSim
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "diffcube16_fp64.npz")
+
+
+def relmax(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def small():
+    m, mu_t, lam_t, v0 = meshgen.diff_workload(4, seed=0)
+    P = D.make_problem(m)
+    sim = DiffSim(m, steps=10)
+    rng = np.random.default_rng(0)
+    x = (m.rest + rng.normal(scale=0.01, size=m.rest.shape)).astype(np.float32)
+    x[m.fixed == 1] = m.rest[m.fixed == 1]
+    return m, mu_t, lam_t, v0, P, sim, x
+
+
+def test_hessian_vector_product_matches_fp64(small):
+    """(M/h^2 + K(x)) w on the device vs the assembled FP64 Hessian (tolerance 1e-5 rel)."""
+    m, _, _, _, P, sim, x = small
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    sim.dev.set_positions(x)
+    w = np.random.default_rng(1).normal(size=x.shape).astype(np.float32)
+    w[m.fixed == 1] = 0
+    hv = sim.dev.hessian_vector(w)
+    ref = D.hessian_vector(P, x.astype(np.float64), w.astype(np.float64), m.tet_mu.astype(np.float64),
+                           m.tet_lambda.astype(np.float64))
+    fr = m.fixed == 0
+    err = relmax(hv[fr], ref[fr])
+    print(f"hvp rel {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_parameter_gradients_match_fp64(small):
+    m, _, _, _, P, sim, x = small
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    sim.dev.set_positions(x)
+    w = np.random.default_rng(2).normal(size=x.shape).astype(np.float32)
+    w[m.fixed == 1] = 0
+    dm, dl = sim.dev.parameter_gradients(w)
+    rm, rl = D.param_grads(P, x.astype(np.float64), w.astype(np.float64), m.tet_mu.astype(np.float64),
+                           m.tet_lambda.astype(np.float64))
+    print(f"param grads rel {relmax(dm, rm):.2e} {relmax(dl, rl):.2e}", flush=True)
+    assert relmax(dm, rm) <= 1e-5 and relmax(dl, rl) <= 1e-5
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.9])
+def test_adjoint_solve_matches_direct_solve(small, rho):
+    """Colored block Gauss-Seidel (with / without Chebyshev) vs a float64 direct solve."""
+    m, _, _, _, P, sim, x = small
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    sim.dev.set_positions(x)
+    rhs = np.random.default_rng(3).normal(size=x.shape).astype(np.float32)
+    rhs[m.fixed == 1] = 0
+    w, sweeps, rr = sim.dev.adjoint_solve(rhs, rel_tol=1e-6, rho=rho)
+    mu, lam = m.tet_mu.astype(np.float64), m.tet_lambda.astype(np.float64)
+    A = D.system_matrix(P, x.astype(np.float64), mu, lam)
+    wd = D.solve_linear(P, A, rhs.astype(np.float64).ravel(), mu, lam).reshape(-1, 3)
+    fr = m.fixed == 0
+    err = relmax(w[fr], wd[fr])
+    print(f"adjoint rho={rho}: {sweeps} sweeps, residual {rr:.1e}, vs direct {err:.2e}", flush=True)
+    assert rr <= 1e-6 and err <= 1e-5
+    assert np.all(w[m.fixed == 1] == 0)
+
+
+def test_converged_step_matches_newton(small):
+    m, _, _, v0, P, sim, _ = small
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    sim.dev.set_positions(m.rest)
+    sim.dev.set_velocities(v0)
+    it, res = sim._step()
+    x = sim.dev.positions()
+    x64, v64, _ = D.solve_step(P, m.rest.astype(np.float64), v0.astype(np.float64), m.tet_mu.astype(np.float64),
+                               m.tet_lambda.astype(np.float64))
+    disp = np.abs(x64 - m.rest).max()
+    err = np.abs(x - x64).max() / disp
+    print(f"converged step: {it} iterations, |grad G| {res:.2e}, error vs Newton {err:.2e} of max displacement",
+          flush=True)
+    assert err <= 1e-3
+    assert np.abs(sim.dev.velocities() - v64).max() <= 1e-3 * np.abs(v64).max() + 1e-6
+
+
+def test_checkpoint_recomputation_is_bitwise(small):
+    """A segment recomputed from its checkpoint reproduces the forward states exactly."""
+    m, _, _, v0, _, _, _ = small
+    sim = DiffSim(m, steps=6, checkpoint_every=3)
+    x, ck = sim.rollout(m.tet_mu, m.tet_lambda, v0)
+    every = DiffSim(m, steps=6, checkpoint_every=1)
+    _, ck1 = every.rollout(m.tet_mu, m.tet_lambda, v0)
+    xs = sim._segment_states(3, 6, ck)
+    for j, n in enumerate(range(3, 6)):
+        assert np.array_equal(xs[j], ck1[n][0])
+    assert np.array_equal(xs[-1], x)
+
+
+def test_small_rollout_gradients_match_fp64_adjoint(small):
+    """4^3 cube, 10 converged steps: loss and all gradients vs the FP64 adjoint (1e-3 rel)."""
+    m, mu_t, lam_t, v0, P, _, _ = small
+    x_t = D.forward(P, m.rest.astype(np.float64), v0.astype(np.float64), mu_t.astype(np.float64),
+                    lam_t.astype(np.float64), 10)[0][-1]
+    sim = DiffSim(m, steps=10)
+    L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    L64, gmu64, glam64, gv064, _ = D.loss_and_grads(P, m.rest.astype(np.float64), v0.astype(np.float64),
+                                                    m.tet_mu.astype(np.float64), m.tet_lambda.astype(np.float64), 10,
+                                                    x_t)
+    errs = {k: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for k, a, b in
+            (("mu", gmu, gmu64), ("lambda", glam, glam64), ("v0", gv0, gv064))}
+    print(f"small rollout: loss {L:.6e} vs {L64:.6e}; gradient rel errors {errs}", flush=True)
+    assert abs(L - L64) <= 1e-3 * L64
+    assert all(e <= 1e-3 for e in errs.values())
+
+
+def test_torch_autograd_function(small):
+    import torch
+    from vbd_tetmesh_amd.adjoint import make_autograd_function
+    m, mu_t, lam_t, v0, P, _, _ = small
+    sim = DiffSim(m, steps=4, checkpoint_every=2)
+    F = make_autograd_function()
+    dev = "cuda"
+    mu = torch.tensor(m.tet_mu, device=dev, requires_grad=True)
+    lam = torch.tensor(m.tet_lambda, device=dev, requires_grad=True)
+    v = torch.tensor(v0, device=dev, requires_grad=True)
+    target = torch.tensor(m.rest, device=dev)
+    x = F.apply(mu, lam, v, sim)
+    loss = ((x - target) ** 2).sum()
+    loss.backward()
+    L, gmu, glam, gv0 = DiffSim(m, steps=4, checkpoint_every=2).loss_and_grads(m.tet_mu, m.tet_lambda, v0, m.rest)
+    assert abs(loss.item() - L) <= 1e-6 * L
+    assert np.allclose(mu.grad.cpu().numpy(), gmu, rtol=1e-5, atol=1e-6 * np.abs(gmu).max())
+    assert np.allclose(lam.grad.cpu().numpy(), glam, rtol=1e-5, atol=1e-6 * np.abs(glam).max())
+    assert np.allclose(v.grad.cpu().numpy(), gv0, rtol=1e-5, atol=1e-6 * np.abs(gv0).max())
+
+
+@pytest.mark.skipif(not os.path.exists(GOLDEN), reason="FP64 golden fixture not generated")
+def test_diffcube16_gradients_match_fp64_and_finite_differences():
+    """Differentiable workload: 16^3 cube (4913 verts, 24576 tets), per-tet mu field, lambda = 4 mu,
+    30 converged steps, loss |x_30 - x_target|^2.  GPU adjoint vs the committed FP64 CPU adjoint
+    and its central finite differences (eps = 1e-3 rel) on 32 seeded parameters.
+    Tolerance: per parameter |g_gpu - g_ref| <= 1e-3 * max|g_ref| of that parameter family, and
+    relative error of each whole gradient vector <= 1e-2."""
+    z = np.load(GOLDEN)
+    m, mu_t, lam_t, v0 = meshgen.diff_workload(int(z["n"]), seed=0)
+    sim = DiffSim(m, steps=int(z["steps"]))
+    L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, z["x_target"])
+    fwd, bwd = sim.stats.forward_seconds, sim.stats.backward_seconds
+    ref = {"mu": z["grad_mu"].astype(np.float64), "lambda": z["grad_lambda"].astype(np.float64),
+           "v0": z["grad_v0"].astype(np.float64)}
+    got = {"mu": gmu, "lambda": glam, "v0": gv0}
+    vec = {k: float(np.linalg.norm(got[k] - ref[k]) / np.linalg.norm(ref[k])) for k in ref}
+    fd = z["fd"]
+    picks = [("mu", int(i)) for i in z["idx_mu"]] + [("lambda", int(i)) for i in z["idx_lambda"]] + \
+        [("v0", (int(i), int(c))) for i, c in z["idx_v0"]]
+    worst_fd, worst_adj = 0.0, 0.0
+    for (kind, i), f in zip(picks, fd):
+        g = got[kind][i]
+        r = ref[kind][i]
+        scale = np.abs(ref[kind]).max()
+        worst_fd = max(worst_fd, abs(g - f) / scale)
+        worst_adj = max(worst_adj, abs(g - r) / scale)
+    print(f"diffcube16: loss {L:.6e} vs {float(z['loss']):.6e}; vector rel errors {vec}; "
+          f"32 params: max |gpu - fd| {worst_fd:.2e}, |gpu - fp64 adjoint| {worst_adj:.2e} (of family max); "
+          f"forward {fwd:.2f}s backward {bwd:.2f}s ratio {bwd / fwd:.2f}", flush=True)
+    assert abs(L - float(z["loss"])) <= 1e-3 * float(z["loss"])
+    assert worst_fd <= 1e-3 and worst_adj <= 1e-3
+    assert all(e <= 1e-2 for e in vec.values())

@@ -39,7 +39,7 @@ class DiffSim:
     """Device rollout + adjoint for one mesh with per-tet materials."""
 
     def __init__(self, mesh, steps: int = 30, h: float = 1.0 / 60.0, rho: float = 0.95, max_iterations: int = 4000,
-                 check_every: int = 10, rel_tol: float = 1e-6, stall_checks: int = 5, adj_max_iterations: int = 8000,
+                 check_every: int = 10, rel_tol: float = 1e-6, stall_checks: int = 10, adj_max_iterations: int = 8000,
                  adj_rel_tol: float = 1e-5, adj_rho: float = 0.9, adj_stall_checks: int = 5, checkpoint_every: int = 5,
                  device: int = 0, damping: float = 0.999):
         if mesh.tet_mu is None:
