@@ -129,6 +129,26 @@ SIGNATURES = [
 ]
 
 
+def _share_torch_hip_runtime():
+    """PyTorch wheels bundle their own HIP/HSA runtime (same sonames as the system ROCm).  Two
+    runtimes in one process cannot both open the GPU, so when torch is installed its runtime is
+    loaded first (RTLD_GLOBAL) and libvbd_hip.so binds to it; torch and this library then share
+    one runtime whichever is imported first.  VBD_SYSTEM_HIP=1 keeps the system runtime."""
+    if os.environ.get("VBD_SYSTEM_HIP") == "1":
+        return None
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+_runtime = _share_torch_hip_runtime()
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
