@@ -163,6 +163,61 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
+def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30):
+    """Irregular workload: sphere voxelised on a 96^3 grid, interior vertices jittered, ids
+    shuffled, mu = 5e4, lambda = 2e5, gravity -z, penalty plane z = -1.2 (k = 1e4 N/m),
+    30 iterations/step, Chebyshev 0.95, dropped from rest."""
+    from vbd_tetmesh_amd import VBDParams, meshgen
+    from vbd_tetmesh_amd.solver import upload
+    m = meshgen.voxel_sphere(n, seed=0)
+    params = VBDParams(dt=H, numIterations=iterations, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95,
+                       mu=m.mu, lambda_=m.lam, contact=(2, -1.2, 1e4))
+    d = upload(m, params=params)
+    info = d.info()
+    tune = d.autotune(3)
+    d.set_positions(m.x0)
+    d.step(1)
+    d.synchronize()
+    d.set_positions(m.x0)
+    d.set_velocities(np.zeros_like(m.x0))
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.step(frames)
+    d.synchronize()
+    secs = time.perf_counter() - t0
+    cms, cn, bms, bn = d.time_iterations(iterations)
+    cbytes, bbytes = d.algorithmic_bytes()
+    vi = float(info.num_verts) * iterations * frames
+    return {"workload": f"sphere {n}^3 voxels, {info.num_verts} verts, {info.num_tets} tets, {info.num_colors} colors, "
+                        f"max valence {info.max_valence}, shuffled ids, contact plane z=-1.2, {iterations} iters/step",
+            "value": round(vi / secs / 1e6, 2), "unit": "M vertex-iters/s", "ms_per_step": round(secs * 1e3 / frames, 3),
+            "color_kernel_frac": round(cbytes / info.num_colors / (cms / cn / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "step_frac": round(vi / secs * (cbytes + bbytes) / info.num_verts / 1e9 / HBM_PEAK_GBS, 4),
+            "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
+
+
+def bench_adjoint(n: int = 16, steps: int = 30):
+    """Differentiable workload: 16^3 cube, per-tet mu field, lambda = 4 mu, 30 converged steps;
+    forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation +
+    colored block-GS adjoint + parameter gradients) wall time on the device."""
+    from vbd_tetmesh_amd import meshgen
+    from vbd_tetmesh_amd.adjoint import DiffSim
+    m, mu_t, lam_t, v0 = meshgen.diff_workload(n, seed=0)
+    sim = DiffSim(m, steps=steps)
+    x_t = m.rest.copy()
+    sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)  # warm-up
+    L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    st = sim.stats
+    vi = float(m.num_verts) * sum(st.forward_iterations)
+    return {"workload": f"cube {n}^3 ({m.num_verts} verts, {m.num_tets} tets), per-tet mu/lambda, {steps} converged "
+                        f"steps, checkpoint every 5, gradients w.r.t. mu, lambda, v0",
+            "forward_s": round(st.forward_seconds, 4), "backward_s": round(st.backward_seconds, 4),
+            "adjoint_over_forward": round(st.backward_seconds / st.forward_seconds, 2),
+            "forward_iterations_mean": round(float(np.mean(st.forward_iterations)), 1),
+            "adjoint_sweeps_mean": round(float(np.mean(st.adjoint_sweeps)), 1),
+            "forward_value": round(vi / st.forward_seconds / 1e6, 2), "unit": "M vertex-iters/s"}
+
+
 def copy_probe(mb: int):
     """Device copy of a known size with torch, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
     import torch
@@ -189,6 +244,8 @@ def main():
     ap.add_argument("--unroll", type=int, default=1)
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only the copy probe")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
+    ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -282,6 +339,15 @@ def main():
         log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step; per-mesh plain "
             f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
             f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
+    if not args.no_sphere:
+        sp = bench_sphere()
+        out["sphere"] = sp
+        log(f"[bench] sphere: {sp['value']:.1f} M vertex-iters/s, color kernel frac {sp['color_kernel_frac']}")
+    if not args.no_adjoint:
+        ad = bench_adjoint()
+        out["adjoint"] = ad
+        log(f"[bench] adjoint: forward {ad['forward_s']:.3f}s backward {ad['backward_s']:.3f}s "
+            f"(x{ad['adjoint_over_forward']})")
     if not args.no_cpu_baseline:
         cb = cpu_baseline(m)
         out["cpu_baseline"] = cb
