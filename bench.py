@@ -163,7 +163,7 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
-def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30):
+def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30, slot_order: int = 0):
     """Irregular workload: sphere voxelised on a 96^3 grid, interior vertices jittered, ids
     shuffled, mu = 5e4, lambda = 2e5, gravity -z, penalty plane z = -1.2 (k = 1e4 N/m),
     30 iterations/step, Chebyshev 0.95, dropped from rest."""
@@ -172,7 +172,7 @@ def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30):
     m = meshgen.voxel_sphere(n, seed=0)
     params = VBDParams(dt=H, numIterations=iterations, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95,
                        mu=m.mu, lambda_=m.lam, contact=(2, -1.2, 1e4))
-    d = upload(m, params=params)
+    d = upload(m, params=params, slot_order=slot_order)
     info = d.info()
     tune = d.autotune(3)
     d.set_positions(m.x0)
@@ -242,6 +242,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--unroll", type=int, default=1)
+    ap.add_argument("--slot-order", type=int, default=0, help="vertex order inside a color: 0 auto, 1 vertex id, 2 Morton")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only the copy probe")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
     ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
@@ -252,7 +253,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (multi-GPU sharding "
-                                   "is design-only, see DESIGN.md)"}))
+                                   "is described but not built)"}))
         sys.exit(2)
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if ws > 1:
@@ -266,7 +267,7 @@ def main():
     m = meshgen.noisy_cube(args.n, seed=0)
     params = VBDParams(dt=H, numIterations=args.iterations, substeps=1, gravity=m.gravity, useAcceleration=True,
                        accelerationRho=0.95, mu=m.mu, lambda_=m.lam)
-    d = upload(m, params=params)
+    d = upload(m, params=params, slot_order=args.slot_order)
     info = d.info()
     log(f"[bench] cube{args.n}: nV={info.num_verts} nT={info.num_tets} colors={info.num_colors} "
         f"pairs={info.num_pairs} device={info.device_bytes / 2**30:.2f} GiB setup {time.time() - t0:.1f}s")
@@ -340,7 +341,7 @@ def main():
             f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
             f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
     if not args.no_sphere:
-        sp = bench_sphere()
+        sp = bench_sphere(slot_order=args.slot_order)
         out["sphere"] = sp
         log(f"[bench] sphere: {sp['value']:.1f} M vertex-iters/s, color kernel frac {sp['color_kernel_frac']}")
     if not args.no_adjoint:

@@ -69,6 +69,10 @@ typedef struct vbd_mesh_desc {
     int32_t num_colors;
     const int32_t* color_offsets; /* num_colors+1 offsets into color_verts */
     const int32_t* color_verts;   /* every vertex exactly once (VBDsolver::vertexColors) */
+    int32_t slot_order;           /* device vertex order inside a color: 0 = auto (Morton order of the
+                                     positions when that brings tet corners closer together than the
+                                     caller's ids, else the ids), 1 = ascending vertex id, 2 = Morton.
+                                     Results are identical either way; only memory locality changes. */
 } vbd_mesh_desc;
 
 /* Per-step parameters, VBDParams (VBD.h:14-25) plus build-defined extensions. */
@@ -167,7 +171,11 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
  * number of color launches, total blend milliseconds, number of blend launches. */
 int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int32_t* color_launches,
                         double* blend_ms, int32_t* blend_launches);
-/* Algorithmic bytes of one sweep's color launches and blend launches (see DESIGN). */
+/* Algorithmic bytes of one sweep, a fixed yardstick independent of the device layout:
+ * color launches 45 B per free vertex (x read 12, x write 12, inertia 12, mass 4, CSR row 4,
+ * fixed flag 1) + 60 B per (vertex, incident tet) (CSR entry 4, tet ids 16, DmInv 36, V0 4;
+ * +8 B with per-tet mu, lambda); blend launches 36 B per free vertex.  Neighbour-position
+ * gathers are not counted (cache-resident). */
 int vbd_algorithmic_bytes(const vbd_solver* h, double* color_bytes_per_sweep, double* blend_bytes_per_sweep);
 
 /* ---- host helpers (CPU; no device work) ------------------------------------------- */

@@ -234,6 +234,27 @@ def test_sphere_with_contact_matches_oracle():
     assert rel <= TOL
 
 
+@pytest.mark.parametrize("gen", ["sphere", "cube"])
+def test_slot_order_does_not_change_results(gen):
+    """Auto, ascending-id and Morton vertex order inside a color give bitwise identical frames:
+    same-color vertices share no tet, so the update of one never reads another."""
+    m = meshgen.voxel_sphere(20, seed=4) if gen == "sphere" else meshgen.noisy_cube(12, seed=2)
+    contact = (2, -1.1, 1e4) if gen == "sphere" else None
+    out = []
+    for order in (0, 1, 2):
+        d = upload(m, params=params_for(m, 25, contact=contact), slot_order=order)
+        d.step(15)
+        out.append(d.positions())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_invalid_slot_order_is_rejected():
+    from vbd_tetmesh_amd import _capi as A
+    with pytest.raises(A.VBDError) as ei:
+        upload(meshgen.beam(3, 1, 1), slot_order=3)
+    assert ei.value.code == A.VBD_ERR_ARG
+
+
 def test_uniform_per_tet_material_equals_scalar_bitwise():
     m = meshgen.beam(10, 3, 3)
     a, _ = run_gpu_frames(m, params_for(m, 15), 3)

@@ -44,6 +44,7 @@ class MeshDesc(C.Structure):
         ("num_colors", C.c_int32),
         ("color_offsets", i32p),
         ("color_verts", i32p),
+        ("slot_order", C.c_int32),
     ]
 
 

@@ -366,11 +366,84 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
     if (d->inertia) HIP_CHECK(h, up(s_y, d->inertia, sizeof(float) * 3 * (size_t)nV));
     if (d->velocity) HIP_CHECK(h, up(s_v, d->velocity, sizeof(float) * 3 * (size_t)nV));
     HIP_CHECK(h, up(s_m, d->mass, sizeof(float) * (size_t)nV));
+    // Slot order: free vertices color-major, pinned last; inside a group either ascending caller
+    // id or a Morton (Z-order) key of the positions, so that consecutive slots -- and the
+    // neighbour gathers of consecutive vertices -- are close in space.  Results do not depend on
+    // the order inside a color (same-color vertices share no tet).
+    if (d->slot_order < 0 || d->slot_order > 2) return fail(h, VBD_ERR_ARG, "slot_order must be 0, 1 or 2");
+    std::vector<int32_t> perm, slotOff;
+    if (d->slot_order != 1 && nV > 0) {
+        std::vector<uint64_t> key(nV);
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+        for (int v = 0; v < nV; ++v)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], d->positions[3 * (size_t)v + c]);
+                hi[c] = std::max(hi[c], d->positions[3 * (size_t)v + c]);
+            }
+        auto spread = [](uint64_t x) {  // 21 bits -> every third bit
+            x &= 0x1fffff;
+            x = (x | x << 32) & 0x1f00000000ffffULL;
+            x = (x | x << 16) & 0x1f0000ff0000ffULL;
+            x = (x | x << 8) & 0x100f00f00f00f00fULL;
+            x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+            x = (x | x << 2) & 0x1249249249249249ULL;
+            return x;
+        };
+        for (int v = 0; v < nV; ++v) {
+            uint64_t q[3];
+            for (int c = 0; c < 3; ++c) {
+                const float ext = hi[c] - lo[c];
+                const float t = ext > 0 ? (d->positions[3 * (size_t)v + c] - lo[c]) / ext : 0.f;
+                q[c] = (uint64_t)std::min(2097151.0f, std::max(0.0f, t * 2097151.0f));
+            }
+            key[v] = spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2;
+        }
+        std::vector<int32_t> byKey(nV);
+        for (int v = 0; v < nV; ++v) byKey[v] = v;
+        std::sort(byKey.begin(), byKey.end(),
+                  [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+        bool morton = d->slot_order == 2;
+        if (d->slot_order == 0) {
+            // auto: keep the caller's order unless the Morton order brings the four corners of
+            // a tet closer together on average (shuffled or scattered ids)
+            std::vector<int32_t> rank(nV);
+            for (int i = 0; i < nV; ++i) rank[byKey[i]] = i;
+            double spanId = 0, spanKey = 0;
+            for (int t = 0; t < nT; ++t) {
+                const int32_t* c = d->tets + 4 * (size_t)t;
+                int lo1 = c[0], hi1 = c[0], lo2 = rank[c[0]], hi2 = rank[c[0]];
+                for (int k = 1; k < 4; ++k) {
+                    lo1 = std::min(lo1, c[k]);
+                    hi1 = std::max(hi1, c[k]);
+                    lo2 = std::min(lo2, rank[c[k]]);
+                    hi2 = std::max(hi2, rank[c[k]]);
+                }
+                spanId += hi1 - lo1;
+                spanKey += hi2 - lo2;
+            }
+            morton = spanKey < spanId;
+        }
+        if (morton) {
+            // stable counting sort of the Morton order by group (color, pinned last)
+            slotOff.assign(nC + 2, 0);
+            std::vector<int64_t> start(nC + 2, 0);
+            for (int v = 0; v < nV; ++v) start[(d->is_fixed[v] ? nC : colorOf[v]) + 1]++;
+            for (int g = 0; g <= nC; ++g) start[g + 1] += start[g];
+            for (int c = 0; c <= nC; ++c) slotOff[c] = (int32_t)start[c];
+            perm.resize(nV);
+            for (int i = 0; i < nV; ++i) {
+                const int v = byKey[i];
+                perm[start[d->is_fixed[v] ? nC : colorOf[v]]++] = v;
+            }
+        }
+    }
     std::string err;
     hipError_t e = build_device_mesh(h->m, nV, nT, nC, (const i4*)s_tets.p, (const int*)s_col.p,
                                      (const uint8_t*)s_fix.p, (const float*)s_dm.p, (const float*)s_vol.p,
                                      (const float*)s_mu.p, (const float*)s_lam.p, (const float*)s_x.p,
-                                     (const float*)s_y.p, (const float*)s_v.p, (const float*)s_m.p, h->stream, err);
+                                     (const float*)s_y.p, (const float*)s_v.p, (const float*)s_m.p,
+                                     perm.empty() ? nullptr : perm.data(), slotOff.empty() ? nullptr : slotOff.data(),
+                                     h->stream, err);
     if (e != hipSuccess) return fail(h, e == hipErrorOutOfMemory ? VBD_ERR_ALLOC : VBD_ERR_HIP, "mesh build: " + err);
     h->lpv = heuristic_lpv(h->m);
     h->unroll = 1;
@@ -464,6 +537,7 @@ int vbd_init_batch(vbd_solver* h, int32_t num_meshes, const vbd_mesh_desc* meshe
     u.num_colors = nC;
     u.color_offsets = coff.data();
     u.color_verts = cverts.data();
+    u.slot_order = meshes[0].slot_order;
     int rc = vbd_init(h, &u);
     if (rc == VBD_OK) h->meshVertOffsets = voff;
     return rc;

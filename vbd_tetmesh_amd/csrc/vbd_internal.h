@@ -92,7 +92,8 @@ hipError_t launch_blend_vec(int n, f4* v, const f4* pp, float omega, hipStream_t
 hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* d_tets, const int* d_colorOf,
                              const uint8_t* d_fixed, const float* d_dminv, const float* d_vol, const float* d_mu,
                              const float* d_lam, const float* d_x3, const float* d_y3, const float* d_v3,
-                             const float* d_mass, hipStream_t s, std::string& err);
+                             const float* d_mass, const int* h_perm, const int* h_slot_off, hipStream_t s,
+                             std::string& err);
 void free_device_mesh(DevMesh& m);
 
 }  // namespace vbd

@@ -1105,7 +1105,8 @@ static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)
 hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* d_tets, const int* d_colorOf,
                              const uint8_t* d_fixed, const float* d_dminv, const float* d_vol, const float* d_mu,
                              const float* d_lam, const float* d_x3, const float* d_y3, const float* d_v3,
-                             const float* d_mass, hipStream_t s, std::string& err) {
+                             const float* d_mass, const int* h_perm, const int* h_slot_off, hipStream_t s,
+                             std::string& err) {
     hipError_t rc = hipSuccess;
     int* deg = nullptr;
     int* cursor = nullptr;
@@ -1173,26 +1174,32 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
             rc = hipErrorInvalidValue;
             goto fail;
         }
-        if (nV > 0) {
-            hipLaunchKernelGGL(k_slot_key, grid_for(nV), dim3(256), 0, s, nV, d_colorOf, d_fixed, nColors, key);
-            VBD_TRY(hipGetLastError());
-        }
-        base = 0;
-        for (int c = 0; c <= nColors; ++c) {
-            int total = 0;
+        if (h_perm) {
+            // slot order chosen by the host (color-major, spatial order inside a color)
+            VBD_TRY(hipMemcpyAsync(m.perm, h_perm, sizeof(int) * (size_t)nV, hipMemcpyHostToDevice, s));
+            for (int c = 0; c <= nColors; ++c) m.colorSlotOff[c] = h_slot_off[c];
+        } else {
             if (nV > 0) {
-                hipLaunchKernelGGL(k_key_flag, grid_for(nV), dim3(256), 0, s, nV, key, c, flag);
+                hipLaunchKernelGGL(k_slot_key, grid_for(nV), dim3(256), 0, s, nV, d_colorOf, d_fixed, nColors, key);
                 VBD_TRY(hipGetLastError());
             }
-            VBD_TRY(scan_exclusive(flag, rank, nV, s));
-            VBD_TRY(hipMemcpyAsync(&total, rank + nV, sizeof(int), hipMemcpyDeviceToHost, s));
-            VBD_TRY(hipStreamSynchronize(s));
-            if (total > 0) {
-                hipLaunchKernelGGL(k_key_place, grid_for(nV), dim3(256), 0, s, nV, key, c, rank, base, m.perm);
-                VBD_TRY(hipGetLastError());
+            base = 0;
+            for (int c = 0; c <= nColors; ++c) {
+                int total = 0;
+                if (nV > 0) {
+                    hipLaunchKernelGGL(k_key_flag, grid_for(nV), dim3(256), 0, s, nV, key, c, flag);
+                    VBD_TRY(hipGetLastError());
+                }
+                VBD_TRY(scan_exclusive(flag, rank, nV, s));
+                VBD_TRY(hipMemcpyAsync(&total, rank + nV, sizeof(int), hipMemcpyDeviceToHost, s));
+                VBD_TRY(hipStreamSynchronize(s));
+                if (total > 0) {
+                    hipLaunchKernelGGL(k_key_place, grid_for(nV), dim3(256), 0, s, nV, key, c, rank, base, m.perm);
+                    VBD_TRY(hipGetLastError());
+                }
+                base += total;
+                if (c < nColors) m.colorSlotOff[c + 1] = base;
             }
-            base += total;
-            if (c < nColors) m.colorSlotOff[c + 1] = base;
         }
         m.nFree = m.colorSlotOff[nColors];
         if (nV > 0) {

@@ -169,7 +169,7 @@ class DeviceMesh(_Handle):
 
     # -- upload
     def init(self, positions, mass, is_fixed, tets, dminv, rest_volume, color_off, color_verts, inertia=None,
-             velocity=None, tet_mu=None, tet_lambda=None):
+             velocity=None, tet_mu=None, tet_lambda=None, slot_order: int = 0):
         arrs = dict(
             positions=np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
             mass=np.ascontiguousarray(mass, np.float32),
@@ -203,6 +203,7 @@ class DeviceMesh(_Handle):
         d.num_colors = arrs["color_off"].size - 1
         d.color_offsets = ptr(arrs["color_off"], C.c_int32)
         d.color_verts = ptr(arrs["color_verts"], C.c_int32)
+        d.slot_order = int(slot_order)
         check(lib.vbd_init(self.h, C.byref(d)), self.h)
         self.num_verts = d.num_verts
         return self
@@ -213,7 +214,7 @@ class DeviceMesh(_Handle):
         arr = (A.MeshDesc * len(descs))()
         total = 0
         for i, kw in enumerate(descs):
-            a = {k: (np.ascontiguousarray(v) if v is not None else None) for k, v in kw.items()}
+            a = {k: (np.ascontiguousarray(v) if v is not None else None) for k, v in kw.items() if k != "slot_order"}
             pos = np.ascontiguousarray(a["positions"], np.float32).reshape(-1, 3)
             tets = np.ascontiguousarray(a["tets"], np.int32).reshape(-1, 4)
             fields = dict(positions=pos, mass=np.ascontiguousarray(a["mass"], np.float32),
@@ -245,6 +246,7 @@ class DeviceMesh(_Handle):
             d.num_colors = fields["color_off"].size - 1
             d.color_offsets = ptr(fields["color_off"], C.c_int32)
             d.color_verts = ptr(fields["color_verts"], C.c_int32)
+            d.slot_order = int(kw.get("slot_order", 0) or 0)
             total += pos.shape[0]
         check(lib.vbd_init_batch(self.h, len(descs), arr), self.h)
         self.num_verts = total
@@ -545,13 +547,14 @@ def prepare_host(m, mu=None, lam=None):
     return t, off, verts, dminv, vol
 
 
-def upload(m, device: int = 0, params: Optional[VBDParams] = None, per_tet: Optional[bool] = None) -> DeviceMesh:
+def upload(m, device: int = 0, params: Optional[VBDParams] = None, per_tet: Optional[bool] = None,
+           slot_order: int = 0) -> DeviceMesh:
     """Generator mesh -> DeviceMesh with params set (host prep by the product C++ helpers)."""
     t, off, verts, dminv, vol = prepare_host(m)
     use_mat = (m.tet_mu is not None) if per_tet is None else per_tet
     d = DeviceMesh(device).init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, off, verts,
                                 velocity=t.mVelocity, tet_mu=t.tetMu if use_mat else None,
-                                tet_lambda=t.tetLambda if use_mat else None)
+                                tet_lambda=t.tetLambda if use_mat else None, slot_order=slot_order)
     if params is not None:
         d.set_params(params)
     return d
