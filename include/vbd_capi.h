@@ -101,7 +101,7 @@ typedef struct vbd_info {
     int64_t num_pairs;          /* (free vertex, incident tet) records streamed per iteration */
     int32_t max_valence;
     int32_t per_tet_material;
-    int32_t lanes_per_vertex;   /* current launch configuration of the color kernel */
+    int32_t lanes_per_vertex;   /* current launch configuration of the color kernel (0: tiled) */
     int32_t pairs_per_lane;
     int32_t block_size;
     int64_t device_bytes;       /* bytes of device memory the handle holds */
@@ -157,9 +157,18 @@ int vbd_synchronize(vbd_solver* h);
 int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
 
 /* ---- launch configuration, autotuning and timing (no reference counterpart) ------- */
-/* Color-kernel mapping: lanes_per_vertex lanes share one vertex; each lane issues the loads
- * of pairs_per_lane incident-tet records per pass.  Supported: lanes 4/8/16 with 1-3 pairs
- * per lane, lanes 32/64 with 1; block_size 64/128/192/256 (a multiple of lanes). */
+/* Color-kernel mapping.
+ * lanes_per_vertex > 0: per-vertex kernel on the full records (F and cof F formed as in
+ *   Energy.cpp): lanes_per_vertex lanes share one vertex; each lane issues the loads of
+ *   pairs_per_lane incident-tet records per pass.  Supported: lanes 4/8/16 with 1-3 pairs per
+ *   lane, lanes 32/64 with 1; block_size 64/128/192/256 (a multiple of lanes).
+ * lanes_per_vertex == 0: tiled kernel on the compact records: one block per tile of
+ *   consecutive vertices holding at most block_size * pairs_per_lane records (and fewer than
+ *   block_size / 8 vertices); every thread takes pairs_per_lane records.  Supported:
+ *   block_size 64-512 (a multiple of 64), pairs_per_lane 1 or 2, block_size * pairs_per_lane
+ *   >= max valence.  Results are bitwise identical across tiled configurations.
+ * The default after vbd_init is the tiled kernel with block 256 and 1 pair per lane when the
+ * maximum valence allows it. */
 int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane);
 /* Autotuner keyed on the valence histogram: on the first mesh with a given key it times each
  * candidate mapping over `iterations` sweeps, restores the positions and keeps the fastest;

@@ -5,8 +5,9 @@ seeded smooth field, 30 steps at h = 1/60 each solved to |grad G| <= 1e-10 |M g|
 L = |x_30 - x_target|^2 with x_target from the seed-drawn ground-truth mu field.
 
 Stores: x_target, the loss, the full adjoint gradients (float32) w.r.t. per-tet mu, lambda and
-v0, 32 seeded parameter indices and their central finite differences (eps = 1e-3 relative,
-float64 forward warm-started from the base trajectory).
+v0, 32 seeded parameter indices and their central finite differences (float64 forward
+warm-started from the base trajectory; step 1e-3 of the parameter for mu and lambda, 1e-3 of
+max |v0| for v0 components, some of which are near zero).
 
 usage: python tests/golden/make_diffcube_golden.py [n] [steps] [out.npz]
 """
@@ -62,7 +63,7 @@ def main():
             elif kind == "lam":
                 e = eps_rel * lam[i]; lp[i] += e; lm[i] -= e
             else:
-                e = eps_rel * abs(v0d[i[0], i[1]]); vp[i[0], i[1]] += e; vm[i[0], i[1]] -= e
+                e = eps_rel * np.abs(v0d).max(); vp[i[0], i[1]] += e; vm[i[0], i[1]] -= e
             d = (loss_of(mp, lp, vp) - loss_of(mm, lm, vm)) / (2 * e)
             fd.append(d)
             print(f"{kind} {i} fd {d:.6e} adjoint "

@@ -150,7 +150,8 @@ def test_torch_autograd_function(small):
 def test_diffcube16_gradients_match_fp64_and_finite_differences():
     """Differentiable workload: 16^3 cube (4913 verts, 24576 tets), per-tet mu field, lambda = 4 mu,
     30 converged steps, loss |x_30 - x_target|^2.  GPU adjoint vs the committed FP64 CPU adjoint
-    and its central finite differences (eps = 1e-3 rel) on 32 seeded parameters.
+    and its central finite differences (step 1e-3 of the parameter, of max |v0| for v0) on 32 seeded
+    parameters.
     Tolerance: per parameter |g_gpu - g_ref| <= 1e-3 * max|g_ref| of that parameter family, and
     relative error of each whole gradient vector <= 1e-2."""
     z = np.load(GOLDEN)

@@ -155,7 +155,8 @@ def test_graph_replay_is_bitwise_identical_to_eager():
 
 
 @pytest.mark.parametrize("lanes,block,unroll", [(4, 64, 1), (4, 128, 3), (8, 128, 1), (8, 256, 2), (8, 192, 3),
-                                                (16, 256, 1), (16, 64, 2), (32, 256, 1), (64, 256, 1)])
+                                                (16, 256, 1), (16, 64, 2), (32, 256, 1), (64, 256, 1),
+                                                (0, 64, 1), (0, 128, 2), (0, 256, 2), (0, 512, 1)])
 def test_launch_configurations_agree(lanes, block, unroll):
     m = meshgen.noisy_cube(12, seed=3)
     prep = O.prepare(m)
@@ -169,9 +170,55 @@ def test_launch_configurations_agree(lanes, block, unroll):
 def test_unsupported_launch_configuration_is_rejected():
     from vbd_tetmesh_amd import _capi as A
     d = upload(meshgen.beam(3, 1, 1))
-    for bad in [(3, 64, 1), (32, 256, 2), (8, 96, 1), (16, 512, 1), (8, 64, 4)]:
+    for bad in [(3, 64, 1), (32, 256, 2), (8, 96, 1), (16, 512, 1), (8, 64, 4), (0, 96, 1), (0, 256, 3),
+                (0, 1024, 1), (-1, 256, 1)]:
         with pytest.raises(A.VBDError):
             d.set_launch_config(*bad)
+
+
+def fan_mesh(n):
+    """n tets around the segment (0,0,-0.3)-(0,0,1): both segment ends have valence n."""
+    ang = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    ring = np.stack([np.cos(ang), np.sin(ang), np.zeros(n)], 1)
+    rest = np.concatenate([[[0, 0, 1.0], [0, 0, -0.3]], ring]).astype(np.float32)
+    i = np.arange(n)
+    tets = np.stack([np.zeros(n), 2 + i, 2 + (i + 1) % n, np.ones(n)], 1).astype(np.int32)
+    fixed = np.zeros(n + 2, np.uint8)
+    fixed[2] = 1
+    return meshgen.Mesh(name="fan", rest=rest, x0=rest.copy(), tets=tets, fixed=fixed)
+
+
+def test_tiled_kernel_rejects_tiles_smaller_than_max_valence():
+    from vbd_tetmesh_amd import _capi as A
+    m = fan_mesh(100)
+    d = upload(m, params=params_for(m, 10))
+    assert d.info().max_valence == 100
+    with pytest.raises(A.VBDError) as ei:
+        d.set_launch_config(0, 64, 1)
+    assert ei.value.code == A.VBD_ERR_ARG
+    d.set_launch_config(0, 128, 1)  # 128 records per tile hold the valence-100 vertices
+    d.step(2)
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, oracle_params(m, 10), m.x0, None, 2)
+    assert relmax(d.positions(), of[-1]) <= TOL
+
+
+def test_tiled_configurations_are_bitwise_identical():
+    """Tiled kernel: the per-vertex summation order depends only on the vertex's valence, so every
+    (records per thread, block) choice gives the same bits; and it agrees with the oracle."""
+    m = meshgen.voxel_sphere(20, seed=5)
+    contact = (2, -1.05, 1e4)
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, oracle_params(m, 20, contact=contact), m.x0, None, 6)
+    outs = []
+    for lanes, block, k in [(0, 64, 1), (0, 256, 1), (0, 128, 2), (0, 512, 2), (0, 64, 2)]:
+        got, _ = run_gpu_frames(m, params_for(m, 20, contact=contact), 6, lanes=lanes, block=block, unroll=k)
+        outs.append(got)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    rel = max(relmax(outs[0][f], of[f]) for f in range(6))
+    print(f"sphere20 tiled: {rel:.3e}", flush=True)
+    assert rel <= TOL
 
 
 def test_noisy_cube_32_matches_oracle():
@@ -265,7 +312,8 @@ def test_uniform_per_tet_material_equals_scalar_bitwise():
     assert np.array_equal(a, b)
 
 
-def test_batch_equals_per_mesh_runs_bitwise():
+@pytest.mark.parametrize("cfg", [(8, 256, 1), (0, 256, 2), (0, 128, 1)])
+def test_batch_equals_per_mesh_runs_bitwise(cfg):
     ms = meshgen.batch_beams(12, seed=5)
     params = VBDParams(dt=H, numIterations=20, useAcceleration=True, accelerationRho=0.95, gravity=(0, -9.8, 0))
     descs = []
@@ -277,12 +325,12 @@ def test_batch_equals_per_mesh_runs_bitwise():
         descs.append(kw)
         d = DeviceMesh().init(**kw)
         d.set_params(params)
-        d.set_launch_config(8, 256, 1)
+        d.set_launch_config(*cfg)
         d.step(4)
         singles.append(d.positions())
     b = DeviceMesh().init_batch(descs)
     b.set_params(params)
-    b.set_launch_config(8, 256, 1)
+    b.set_launch_config(*cfg)
     b.step(4)
     got = b.positions()
     ref = np.concatenate(singles)

@@ -120,10 +120,17 @@ bool finite_array(const float* a, size_t n) {
     return true;
 }
 
+// One color launch with the handle's configuration: lpv > 0 selects the per-vertex lane-group
+// kernel, lpv == 0 the tiled kernel on compact records (unroll = records per thread).
+hipError_t launch_color_cfg(vbd_solver* h, int c, float sdt, float mu, float lambda, const ContactArgs& ct, f4* save) {
+    if (h->lpv == 0) return launch_color_tiled(h->m, c, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
+    return launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
+}
+
 // One Gauss-Seidel sweep over all colors.
 hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const ContactArgs& ct, f4* save) {
     for (int c = 0; c < h->m.nColors; ++c) {
-        hipError_t e = launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
+        hipError_t e = launch_color_cfg(h, c, sdt, mu, lambda, ct, save);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -238,6 +245,26 @@ int heuristic_lpv(const DevMesh& m) {
     int l = 4;
     while (l < 32 && l * 3 < mean) l *= 2;
     return l;
+}
+
+// Default launch configuration: the tiled kernel with the smallest tile (block 256 with 1 or 2
+// records per thread, block 512 with 2) that holds the largest valence; else the per-vertex
+// kernel with the histogram heuristic.
+void choose_default_config(vbd_solver* h) {
+    const int cfg[][2] = {{256, 1}, {256, 2}, {512, 2}};
+    for (const auto& c : cfg) {
+        std::string err;
+        if (h->m.tiledOk && h->m.maxValence <= c[0] * c[1] &&
+            build_tiles(h->m, c[0] * c[1], c[0] / 8 - 1, h->stream, err) == hipSuccess) {
+            h->lpv = 0;
+            h->unroll = c[1];
+            h->block = c[0];
+            return;
+        }
+    }
+    h->lpv = heuristic_lpv(h->m);
+    h->unroll = 1;
+    h->block = 256;
 }
 
 }  // namespace
@@ -445,9 +472,7 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
                                      perm.empty() ? nullptr : perm.data(), slotOff.empty() ? nullptr : slotOff.data(),
                                      h->stream, err);
     if (e != hipSuccess) return fail(h, e == hipErrorOutOfMemory ? VBD_ERR_ALLOC : VBD_ERR_HIP, "mesh build: " + err);
-    h->lpv = heuristic_lpv(h->m);
-    h->unroll = 1;
-    h->block = 256;
+    choose_default_config(h);
     h->meshVertOffsets.assign({0, nV});
     h->inited = true;
     return VBD_OK;
@@ -744,10 +769,21 @@ int vbd_synchronize(vbd_solver* h) {
 
 int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane) {
     if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
-    if (!launch_config_supported(lanes_per_vertex, pairs_per_lane, block_size) || block_size % lanes_per_vertex)
+    if (lanes_per_vertex == 0) {
+        if (!tiled_config_supported(pairs_per_lane, block_size))
+            return fail(h, VBD_ERR_ARG,
+                        "unsupported tiled configuration: block_size 64-512 (multiple of 64), pairs_per_lane 1 or 2");
+        if (!h->inited) return fail(h, VBD_ERR_STATE, "the tiled kernel needs an initialised mesh");
+        DeviceGuard g(h->device);
+        std::string err;
+        if (build_tiles(h->m, block_size * pairs_per_lane, block_size / 8 - 1, h->stream, err) != hipSuccess)
+            return fail(h, VBD_ERR_ARG, err);
+    } else if (!launch_config_supported(lanes_per_vertex, pairs_per_lane, block_size) ||
+               block_size % lanes_per_vertex) {
         return fail(h, VBD_ERR_ARG,
                     "unsupported launch configuration: lanes_per_vertex 4/8/16 with pairs_per_lane 1-3, or 32/64 with "
-                    "pairs_per_lane 1; block_size 64/128/192/256");
+                    "pairs_per_lane 1; block_size 64/128/192/256; or lanes_per_vertex 0 (tiled)");
+    }
     h->lpv = lanes_per_vertex;
     h->unroll = pairs_per_lane;
     h->block = block_size;
@@ -792,7 +828,7 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
             if (h->m.colorSlotOff[c + 1] == h->m.colorSlotOff[c]) continue;
             if ((e = mk(false)) != hipSuccess) break;
             (void)hipEventRecord(evs.back().a, h->stream);
-            e = launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lam, ct, save, h->stream);
+            e = launch_color_cfg(h, c, sdt, mu, lam, ct, save);
             (void)hipEventRecord(evs.back().b, h->stream);
         }
         if (accel && e == hipSuccess) {
@@ -834,7 +870,10 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) {
+        std::string terr;
+        if (it != g_tune_cache.end() &&
+            (it->second.l != 0 || build_tiles(h->m, it->second.u * it->second.b, it->second.b / 8 - 1, h->stream,
+                                                   terr) == hipSuccess)) {
             h->lpv = it->second.l;
             h->unroll = it->second.u;
             h->block = it->second.b;
@@ -853,15 +892,23 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     struct Cand {
         int l, u;
     };
-    const Cand cands[] = {{4, 1}, {4, 2}, {4, 3}, {8, 1}, {8, 2}, {8, 3}, {16, 1}, {16, 2}, {32, 1}};
-    const int cand_b[] = {64, 128, 256};
+    // per-vertex kernel: (lanes, pairs per lane) x blocks 64/128/256; tiled kernel (lanes 0):
+    // (records per thread) x blocks 128/256/512
+    const Cand cands[] = {{4, 1}, {4, 2}, {4, 3}, {8, 1}, {8, 2}, {8, 3}, {16, 1}, {16, 2}, {32, 1}, {0, 1}, {0, 2}};
+    const int cand_b[] = {64, 128, 256, 512};
     int bl = h->lpv, bu = h->unroll, bb = h->block;
     double best = 1e30;
     const int ol = h->lpv, ou = h->unroll, ob = h->block;
     for (const Cand& cd : cands) {
         for (int b : cand_b) {
             if (e != hipSuccess) break;
-            if (b % cd.l) continue;
+            if (cd.l > 0 && (b > 256 || b % cd.l)) continue;
+            if (cd.l == 0) {
+                if (b < 128 || !tiled_config_supported(cd.u, b) || !h->m.tiledOk || h->m.maxValence > cd.u * b)
+                    continue;
+                std::string terr;
+                if (build_tiles(h->m, cd.u * b, b / 8 - 1, h->stream, terr) != hipSuccess) continue;
+            }
             h->lpv = cd.l;
             h->unroll = cd.u;
             h->block = b;
@@ -874,6 +921,10 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
                 h->lpv = ol;
                 h->unroll = ou;
                 h->block = ob;
+                if (ol == 0) {
+                    std::string terr;
+                    (void)build_tiles(h->m, ou * ob, ob / 8 - 1, h->stream, terr);
+                }
                 (void)launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
                 (void)hipStreamSynchronize(h->stream);
                 (void)hipFree(keep);
@@ -891,6 +942,10 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(keep);
     if (e != hipSuccess) return hip_fail(h, "autotune", e);
+    if (bl == 0) {
+        std::string terr;
+        if (build_tiles(h->m, bu * bb, bb / 8 - 1, h->stream, terr) != hipSuccess) return fail(h, VBD_ERR_STATE, terr);
+    }
     h->lpv = bl;
     h->unroll = bu;
     h->block = bb;

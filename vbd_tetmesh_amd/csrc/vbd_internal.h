@@ -38,11 +38,16 @@ struct DevMesh {
     int* row = nullptr;     // nFree+1 pair offsets
     int* perm = nullptr;    // nV slot -> caller vertex id
     int* inv = nullptr;     // nV caller vertex id -> slot
-    i4* ptet = nullptr;     // nPairs  3 corner slots (bit 31 of .x: the vertex is corner 0), V0 bits in .w
+    i4* ptet = nullptr;     // nPairs  3 corner slots (27 bits; bit 31 of .x: the vertex is corner 0;
+                            //         bits 27-30 of .x/.y/.z: owner offset from chunkBase), V0 bits in .w
     f4* pa = nullptr;       // nPairs  R0.xyz, R1.x  (three DmInv rows, see k_build_pairs)
     f4* pb = nullptr;       // nPairs  R1.yz, R2.xy
     float* pc = nullptr;    // nPairs  R2.z
     f2* pmat = nullptr;     // nPairs  mu, lambda (per-tet materials only)
+    // compact records of the tiled kernel (with ptet): F q = beta (x_s - y_a) + alpha1 (y_b - o)
+    // + alpha2 (y_c - o) with o = corner 0, C q = d (y_b - y_a) x (y_c - y_a), J = C q . (x_s - y_a)
+    f4* pq = nullptr;       // nPairs  beta, alpha1, alpha2, d
+    float* pqq = nullptr;   // nPairs  V0 |q|^2
     // GPU-built structures kept in the caller's numbering (exposed for verification)
     int* adjOff = nullptr;    // nV+1
     int* adjEntry = nullptr;  // 4*nT, tet*4 + corner
@@ -53,8 +58,20 @@ struct DevMesh {
     float* vol = nullptr;   // nT
     float* tmu = nullptr;   // nT or null
     float* tlam = nullptr;  // nT or null
+    // tiled color kernel: owner bases of 4096-record chunks and the tile cut of each color
+    int* chunkBase = nullptr;          // ceil(nPairs / 4096)
+    bool tiledOk = false;              // owner offsets fit the 12 record bits
+    std::vector<int> hostRow;          // host copy of row (nFree+1)
+    int* tileStart = nullptr;          // per color: its tile start slots, then the color's end slot
+    int* tileRec = nullptr;            // row[] at each tileStart entry (first record of the tile)
+    int tileRecords = 0;               // record cap per tile of the current cut (0: none)
+    int tileVerts = 0;                 // vertex cap per tile of the current cut
+    std::vector<int> colorTileBase;    // index of each color's first entry in tileStart
+    std::vector<int> colorTileCount;   // tiles per color
     int64_t bytes = 0;
 };
+
+
 
 struct ContactArgs {
     int on;
@@ -67,6 +84,14 @@ struct ContactArgs {
 bool launch_config_supported(int lpv, int unroll, int block);
 hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int block, float sdt, float mu, float lambda,
                         const ContactArgs& ct, f4* save, hipStream_t s);
+// tiled variant: per_thread records per thread, tiles of per_thread*block records and fewer than
+// block/8 vertices (build_tiles first)
+bool tiled_config_supported(int per_thread, int block);
+hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int block, float sdt, float mu,
+                              float lambda, const ContactArgs& ct, f4* save, hipStream_t s);
+// tile cut for the tiled kernel with (per_thread, block): records = per_thread * block,
+// max_verts = block / 8 - 1
+hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err);
 hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
 hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s);
 hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t s);

@@ -64,7 +64,15 @@ __device__ __forceinline__ int xcd_block() {
 // and H is exactly the vertex's 3x3 diagonal block of the 12x12 Hessian of
 // Energy.cpp:115-154: the (lambda(J-1) - mu) d^2J/dF^2 term contracts with q q^T to zero.
 // ---------------------------------------------------------------------------------------
-constexpr int SLOT_MASK = 0x7FFFFFFF;
+// Record slot fields hold 27-bit slots.  Bit 31 of .x flags "vertex is corner 0"; bits 27-30
+// of .x, .y, .z together hold the record's owner as an offset (12 bits) from the owner of the
+// first record of its 4096-record chunk (chunkBase), which the tiled color kernel uses.
+constexpr int SLOT_MASK = 0x07FFFFFF;
+constexpr int CHUNK_SHIFT = 12;
+
+__device__ __forceinline__ int owner_offset(const i4 t) {
+    return ((t.x >> 27) & 15) | (((t.y >> 27) & 15) << 4) | (((t.z >> 27) & 15) << 8);
+}
 
 struct Acc {
     float h00, h01, h02, h11, h12, h22, r0, r1, r2;
@@ -206,8 +214,8 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, int n
             for (int u = 0; u < U; ++u) {
                 if (base + u * LPV < p1) {
                     ya[u] = gather_pos(prs, t[u].x & SLOT_MASK);
-                    yb[u] = gather_pos(prs, t[u].y);
-                    yc[u] = gather_pos(prs, t[u].z);
+                    yb[u] = gather_pos(prs, t[u].y & SLOT_MASK);
+                    yc[u] = gather_pos(prs, t[u].z & SLOT_MASK);
                 }
             }
 #pragma unroll
@@ -257,6 +265,388 @@ __global__ __launch_bounds__(256) void k_color_solve(int slot0, int count, int n
     }
     if constexpr (SAVE) save[s] = xs;
     pos[s] = xn;
+}
+
+// Vertex update from the summed elastic Hessian block h and force r: inertia term m/h^2
+// (VBD.cpp:170-173), optional penalty contact, adjugate 3x3 solve when |det H| > 1e-8
+// (VBD.cpp:141-153, Types.h:85-112).  Returns the new position.
+template <bool CONTACT>
+__device__ __forceinline__ f4 vertex_solve(const f4 xs, const f4 ys, const float m, float h00, float h01, float h02,
+                                           float h11, float h12, float h22, float r0, float r1, float r2, float sdt,
+                                           const ContactArgs& ct) {
+#pragma clang fp contract(off)
+    const float ms = m * sdt;
+    h00 += ms; h11 += ms; h22 += ms;
+    r0 += (m * (ys.x - xs.x)) * sdt;
+    r1 += (m * (ys.y - xs.y)) * sdt;
+    r2 += (m * (ys.z - xs.z)) * sdt;
+    if constexpr (CONTACT) {
+        const float xa = ct.axis == 0 ? xs.x : (ct.axis == 1 ? xs.y : xs.z);
+        const float d = ct.plane - xa;
+        if (d > 0.f) {
+            if (ct.axis == 0) { h00 += ct.k; r0 += ct.k * d; }
+            else if (ct.axis == 1) { h11 += ct.k; r1 += ct.k * d; }
+            else { h22 += ct.k; r2 += ct.k * d; }
+        }
+    }
+    const float a00 = __builtin_fmaf(h11, h22, -(h12 * h12));
+    const float a01 = __builtin_fmaf(h02, h12, -(h01 * h22));
+    const float a02 = __builtin_fmaf(h01, h12, -(h02 * h11));
+    const float a11 = __builtin_fmaf(h00, h22, -(h02 * h02));
+    const float a12 = __builtin_fmaf(h02, h01, -(h00 * h12));
+    const float a22 = __builtin_fmaf(h00, h11, -(h01 * h01));
+    const float det = __builtin_fmaf(h00, a00, __builtin_fmaf(h01, a01, h02 * a02));
+    f4 xn = xs;
+    if (fabsf(det) > 1e-8f) {
+        const float inv = 1.0f / det;
+        xn.x = __builtin_fmaf(inv, __builtin_fmaf(a00, r0, __builtin_fmaf(a01, r1, a02 * r2)), xs.x);
+        xn.y = __builtin_fmaf(inv, __builtin_fmaf(a01, r0, __builtin_fmaf(a11, r1, a12 * r2)), xs.y);
+        xn.z = __builtin_fmaf(inv, __builtin_fmaf(a02, r0, __builtin_fmaf(a12, r1, a22 * r2)), xs.z);
+    }
+    return xn;
+}
+
+// ---------------------------------------------------------------------------------------
+// Tiled per-color vertex solve on compact records.
+//
+// Per (vertex s, tet) record the closed-form block needs only F q, C q and J (q = the
+// vertex's shape vector, row s of Q = Ps DmInv, Energy.cpp:101-106).  With a, b, c the other
+// three corners (a = corner 0, or corner 1 when s is corner 0), o = corner 0 of the tet
+// (x_s or y_a), delta = x_s - y_a, u = y_b - y_a, w = y_c - y_a:
+//   F q = Ds (DmInv q) = beta delta + alpha1 (y_b - o) + alpha2 (y_c - o)
+//         (the edges of Ds from corner 0 as in Energy.cpp:11-21; beta = +-alpha0; the alpha
+//          are precomputed)
+//   C q = dJ/dx_s = d (u x w),   d = det(DmInv) * (-1, +1, -1, +1)[corner of s]
+//   J   = (C q) . delta                                         (J is affine in x_s and
+//                                                                vanishes on the face a b c)
+// and then, exactly as in the full form,
+//   g = V [mu (F q - C q) + lambda (J-1) C q],   H = mu V |q|^2 I + lambda V (C q)(C q)^T.
+// The record is ptet (3 slots, V) + pq (beta, alpha1, alpha2, d) + pqq (V |q|^2): 36 B
+// instead of 52 B, and about half the arithmetic of forming F and cof(F).
+//
+// The color's free vertices are cut into tiles of consecutive slots holding at most
+// RT = K * blockDim.x records and fewer than blockDim.x / 8 vertices (tileStart / tileRec,
+// built on the host).  One block per tile; XCD x takes the x-th eighth of the tiles.
+//   1. thread i takes records r0+i, r0+i+T, ... (K of them): the whole block streams one
+//      contiguous record range, every lane busy.  The record's owner slot comes from
+//      chunkBase[p >> 12] + the owner offset bits.  The 9 contribution values (6 Hessian
+//      entries, 3 force entries) go to LDS, one array per value; the tile's CSR row offsets,
+//      positions, inertia targets and masses are staged in LDS too.
+//   2. after a barrier, TG = 8 lanes per vertex sum its records (lane l: records l, l+TG, ...
+//      of the vertex, then a DPP tree), so the summation order depends only on the vertex's
+//      valence, not on the tile cut, the grid or the block size: results are bitwise
+//      identical across (K, blockDim) choices.  The group's last lane solves and writes the
+//      vertex.
+// ---------------------------------------------------------------------------------------
+constexpr int TG = 8;
+constexpr int BUF_NT = 2;  // nontemporal cache policy bit of the raw buffer loads
+
+template <int K, bool MAT>
+struct RecRegs {
+    i4 t[K];
+    f4 q[K];
+    float qq[K];
+    f2 mm[K];
+    int cb[K];
+};
+
+struct RecSrc {  // buffer resources over one color's records (byte offsets from its first record)
+    __amdgpu_buffer_rsrc_t t, q, qq, mm;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const void* base, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+// Loads are unconditional (index clamped to the color's last record): no branch around
+// them, so the compiler's wait counts can keep the gathers in flight with the vertex loads.
+template <int K, bool MAT>
+__device__ __forceinline__ void load_records(RecRegs<K, MAT>& R, const RecSrc& src, int r0, int pmax, int rec0,
+                                             int tid, int T, const int* __restrict__ chunkBase) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int p = min(r0 + tid + k * T, pmax);  // relative to the color's first record
+        R.t[k] = __builtin_bit_cast(i4, __builtin_amdgcn_raw_buffer_load_b128(src.t, p << 4, 0, BUF_NT));
+        R.q[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.q, p << 4, 0, BUF_NT));
+        R.qq[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(src.qq, p << 2, 0, BUF_NT));
+        if constexpr (MAT)
+            R.mm[k] = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(src.mm, p << 3, 0, BUF_NT));
+        R.cb[k] = chunkBase[(p + rec0) >> CHUNK_SHIFT];
+    }
+}
+
+// Workgroup barrier ordering LDS only.  __syncthreads() also orders global memory, which on
+// this target waits for every outstanding vector load and store.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Explicit fmaf and no contraction: every instantiation (K, MAT, ...) rounds alike, so the
+// tiled configurations give bitwise identical results.
+__device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, const float vqq, const float V,
+                                               const float tmu, const float tlam, const f4 xs, const f4 ya,
+                                               const f4 yb, const f4 yc) {
+#pragma clang fp contract(off)
+    const float dx = xs.x - ya.x, dy = xs.y - ya.y, dz = xs.z - ya.z;
+    const float ux = yb.x - ya.x, uy = yb.y - ya.y, uz = yb.z - ya.z;
+    const float wx = yc.x - ya.x, wy = yc.y - ya.y, wz = yc.z - ya.z;
+    const f4 o = corner0 ? xs : ya;
+    const float bx = yb.x - o.x, by = yb.y - o.y, bz = yb.z - o.z;
+    const float cx = yc.x - o.x, cy = yc.y - o.y, cz = yc.z - o.z;
+    const float c0 = qv.w * __builtin_fmaf(uy, wz, -(uz * wy));
+    const float c1 = qv.w * __builtin_fmaf(uz, wx, -(ux * wz));
+    const float c2 = qv.w * __builtin_fmaf(ux, wy, -(uy * wx));
+    const float J = __builtin_fmaf(c0, dx, __builtin_fmaf(c1, dy, c2 * dz));
+    const float f0 = __builtin_fmaf(qv.x, dx, __builtin_fmaf(qv.y, bx, qv.z * cx));
+    const float f1 = __builtin_fmaf(qv.x, dy, __builtin_fmaf(qv.y, by, qv.z * cy));
+    const float f2v = __builtin_fmaf(qv.x, dz, __builtin_fmaf(qv.y, bz, qv.z * cz));
+    // g/V = mu (F q - C q) + lambda (J - 1) C q; F q - C q first (near-exact cancellation)
+    const float aj = tlam * (J - 1.0f);
+    const float g0 = __builtin_fmaf(tmu, f0 - c0, aj * c0);
+    const float g1 = __builtin_fmaf(tmu, f1 - c1, aj * c1);
+    const float g2 = __builtin_fmaf(tmu, f2v - c2, aj * c2);
+    const float vm = tmu * vqq;
+    const float vl = V * tlam;
+    const float v0 = vl * c0, v1 = vl * c1, v2 = vl * c2;
+    Acc A;
+    A.h00 = __builtin_fmaf(v0, c0, vm);
+    A.h01 = v0 * c1;
+    A.h02 = v0 * c2;
+    A.h11 = __builtin_fmaf(v1, c1, vm);
+    A.h12 = v1 * c2;
+    A.h22 = __builtin_fmaf(v2, c2, vm);
+    A.r0 = -(V * g0);
+    A.r1 = -(V * g1);
+    A.r2 = -(V * g2);
+    return A;
+}
+
+template <int K, bool MAT, bool CONTACT, bool SAVE>
+__global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, const int* __restrict__ tileRec,
+                                                     int ntiles, int nslots, f4* __restrict__ pos,
+                                                     const f4* __restrict__ inert, const float* __restrict__ mass,
+                                                     const int* __restrict__ row, const int* __restrict__ chunkBase,
+                                                     const i4* __restrict__ ptet, const f4* __restrict__ pq,
+                                                     const float* __restrict__ pqq, const f2* __restrict__ pmat,
+                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
+                                                     ContactArgs ct) {
+    // LDS: 9 * RT contribution values, then VM = T/TG entries (a tile has at most VM - 1
+    // vertices): row offsets (nv + 1 of them), x, y, m
+    extern __shared__ float red[];
+    const int T = (int)blockDim.x;
+    const int RT = K * T;
+    const int VM = T / TG;
+    int* lrow = reinterpret_cast<int*>(red + 9 * RT);
+    f4* lx = reinterpret_cast<f4*>(red + 9 * RT + VM);
+    f4* ly = lx + VM;
+    float* lm = reinterpret_cast<float*>(ly + VM);
+    const int tid = (int)threadIdx.x;
+
+    const int xcd = (int)(blockIdx.x & 7);
+    const int tile = (int)(((int64_t)ntiles * xcd) >> 3) + (int)(blockIdx.x >> 3);
+    if (tile >= (int)(((int64_t)ntiles * (xcd + 1)) >> 3)) return;  // whole block
+    const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, nslots);
+    const int rec0 = tileRec[0];
+    const int nrec = tileRec[ntiles] - rec0;  // records of this color
+    const int pmax = nrec - 1;
+    RecSrc src;
+    src.t = rec_rsrc(ptet + rec0, 16 * (int64_t)nrec);
+    src.q = rec_rsrc(pq + rec0, 16 * (int64_t)nrec);
+    src.qq = rec_rsrc(pqq + rec0, 4 * (int64_t)nrec);
+    src.mm = MAT ? rec_rsrc(pmat + rec0, 8 * (int64_t)nrec) : src.qq;
+
+    const int r0 = tileRec[tile] - rec0, r1 = tileRec[tile + 1] - rec0;
+    RecRegs<K, MAT> R;
+    load_records<K, MAT>(R, src, r0, pmax, rec0, tid, T, chunkBase);
+    const int v0 = tileStart[tile], v1 = tileStart[tile + 1];
+    const int nv = v1 - v0;
+    // the tile's vertex data (clamped indices; lanes past nv load and ignore a copy)
+    const int vrow = row[min(v0 + tid, v1)];
+    const int vs = min(v0 + tid, v1 - 1);
+    const f4 vx = pos[vs];
+    const f4 vy = __builtin_nontemporal_load(&inert[vs]);
+    const float vm = __builtin_nontemporal_load(&mass[vs]);
+    f4 xs[K], ya[K], yb[K], yc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        xs[k] = gather_pos(prs, R.cb[k] + owner_offset(R.t[k]));
+        ya[k] = gather_pos(prs, R.t[k].x & SLOT_MASK);
+        yb[k] = gather_pos(prs, R.t[k].y & SLOT_MASK);
+        yc[k] = gather_pos(prs, R.t[k].z & SLOT_MASK);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = tid + k * T;
+        float tmu = mu, tlam = lambda;
+        if constexpr (MAT) {
+            tmu = R.mm[k].x;
+            tlam = R.mm[k].y;
+        }
+        const Acc A = compact_contrib(R.t[k].x < 0, R.q[k], R.qq[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k],
+                                      ya[k], yb[k], yc[k]);
+        if (r0 + q < r1) {
+            red[0 * RT + q] = A.h00;
+            red[1 * RT + q] = A.h01;
+            red[2 * RT + q] = A.h02;
+            red[3 * RT + q] = A.h11;
+            red[4 * RT + q] = A.h12;
+            red[5 * RT + q] = A.h22;
+            red[6 * RT + q] = A.r0;
+            red[7 * RT + q] = A.r1;
+            red[8 * RT + q] = A.r2;
+        }
+    }
+    if (tid <= nv) lrow[tid] = vrow - rec0 - r0;
+    if (tid < nv) {
+        lx[tid] = vx;
+        ly[tid] = vy;
+        lm[tid] = vm;
+    }
+    lds_barrier();
+
+    // one group of TG lanes per tile vertex
+    const int lane = tid % TG;
+    const int i = tid / TG;
+    const bool ok = i < nv;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
+    if (ok) {
+        const int q1 = lrow[i + 1];
+        for (int q = lrow[i] + lane; q < q1; q += TG) {
+            s0 += red[0 * RT + q];
+            s1 += red[1 * RT + q];
+            s2 += red[2 * RT + q];
+            s3 += red[3 * RT + q];
+            s4 += red[4 * RT + q];
+            s5 += red[5 * RT + q];
+            s6 += red[6 * RT + q];
+            s7 += red[7 * RT + q];
+            s8 += red[8 * RT + q];
+        }
+    }
+    s0 = gsum<TG>(s0); s1 = gsum<TG>(s1); s2 = gsum<TG>(s2);
+    s3 = gsum<TG>(s3); s4 = gsum<TG>(s4); s5 = gsum<TG>(s5);
+    s6 = gsum<TG>(s6); s7 = gsum<TG>(s7); s8 = gsum<TG>(s8);
+    if (ok && lane == TG - 1) {
+        const f4 x = lx[i];
+        const f4 xn = vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
+        if constexpr (SAVE) save[v0 + i] = x;
+        pos[v0 + i] = xn;
+    }
+}
+
+template <int K, bool MAT, bool CONTACT, bool SAVE>
+static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+    const int ntiles = m.colorTileCount[color];
+    if (ntiles <= 0) return hipSuccess;
+    const size_t lds = sizeof(float) * 9 * (size_t)K * block +
+                       (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / TG);
+    const int per = (ntiles + 7) / 8;
+    hipLaunchKernelGGL((k_color_tiled<K, MAT, CONTACT, SAVE>), dim3(8 * per), dim3(block), lds, s,
+                       m.tileStart + m.colorTileBase[color], m.tileRec + m.colorTileBase[color], ntiles, m.nV, m.pos,
+                       m.inert, m.mass, m.row, m.chunkBase, m.ptet, m.pq, m.pqq, m.pmat, save, mu, lambda, sdt, ct);
+    return hipGetLastError();
+}
+
+template <int K>
+static hipError_t launch_tiled_k(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+    const bool mat = m.perTetMat, con = ct.on != 0, sv = save != nullptr;
+#define VBD_DISPATCH(MT, CT, SV)                                                                                   \
+    if (mat == MT && con == CT && sv == SV)                                                                        \
+        return launch_tiled_t<K, MT, CT, SV>(m, color, block, sdt, mu, lambda, ct, save, s);
+    VBD_DISPATCH(false, false, false) VBD_DISPATCH(false, false, true)
+    VBD_DISPATCH(false, true, false) VBD_DISPATCH(false, true, true)
+    VBD_DISPATCH(true, false, false) VBD_DISPATCH(true, false, true)
+    VBD_DISPATCH(true, true, false) VBD_DISPATCH(true, true, true)
+#undef VBD_DISPATCH
+    return hipErrorInvalidValue;
+}
+
+bool tiled_config_supported(int per_thread, int block) {
+    if (block < 64 || block > 512 || (block % 64) != 0) return false;
+    return per_thread == 1 || per_thread == 2;
+}
+
+hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int block, float sdt, float mu,
+                              float lambda, const ContactArgs& ct, f4* save, hipStream_t s) {
+    if (!tiled_config_supported(per_thread, block) || m.tileRecords != per_thread * block ||
+        m.tileVerts != block / TG - 1 || !m.tileStart)
+        return hipErrorInvalidValue;
+    return per_thread == 1 ? launch_tiled_k<1>(m, color, block, sdt, mu, lambda, ct, save, s)
+                           : launch_tiled_k<2>(m, color, block, sdt, mu, lambda, ct, save, s);
+}
+
+// Greedy cut of every color's slot range into tiles of at most `records` records and
+// `max_verts` vertices (host; uses the host copy of the CSR row offsets).  Fails if a vertex
+// alone exceeds the record cap.
+hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err) {
+    if (m.tileRecords == records && m.tileVerts == max_verts && m.tileStart) return hipSuccess;
+    if (!m.tiledOk) {
+        err = "tiled kernel unavailable: a 4096-record chunk spans more than 4096 vertices";
+        return hipErrorInvalidValue;
+    }
+    if (m.maxValence > records) {
+        err = "records per tile (" + std::to_string(records) + ") below the maximum valence (" +
+              std::to_string(m.maxValence) + ")";
+        return hipErrorInvalidValue;
+    }
+    for (int c = 0; c < m.nColors; ++c)
+        if ((int64_t)m.hostRow[m.colorSlotOff[c + 1]] - m.hostRow[m.colorSlotOff[c]] >= (int64_t)1 << 27) {
+            err = "tiled kernel: a color holds 2^27 records or more (32-bit buffer offsets)";
+            return hipErrorInvalidValue;
+        }
+    std::vector<int> starts;
+    std::vector<int> base(m.nColors, 0), count(m.nColors, 0);
+    for (int c = 0; c < m.nColors; ++c) {
+        base[c] = (int)starts.size();
+        const int e = m.colorSlotOff[c + 1];
+        int v = m.colorSlotOff[c];
+        while (v < e) {
+            starts.push_back(v);
+            const int r = m.hostRow[v];
+            while (v < e && m.hostRow[v + 1] - r <= records && v - starts.back() < max_verts) ++v;
+            if (v == starts.back()) {  // unreachable after the valence check above
+                err = "tile cut failed";
+                return hipErrorInvalidValue;
+            }
+            ++count[c];
+        }
+        starts.push_back(e);
+    }
+    std::vector<int> recs(starts.size());
+    for (size_t i = 0; i < starts.size(); ++i) recs[i] = m.hostRow[starts[i]];
+    if (m.tileStart) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(m.tileStart);
+        (void)hipFree(m.tileRec);
+        m.tileStart = nullptr;
+        m.tileRec = nullptr;
+        m.tileRecords = 0;
+    }
+    const size_t n = starts.empty() ? 1 : starts.size();
+    hipError_t e = hipMalloc(&m.tileStart, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMalloc(&m.tileRec, sizeof(int) * n);
+    if (e == hipSuccess && !starts.empty())
+        e = hipMemcpyAsync(m.tileStart, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !starts.empty())
+        e = hipMemcpyAsync(m.tileRec, recs.data(), sizeof(int) * recs.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        err = std::string("tile upload: ") + hipGetErrorString(e);
+        if (m.tileStart) (void)hipFree(m.tileStart);
+        if (m.tileRec) (void)hipFree(m.tileRec);
+        m.tileStart = nullptr;
+        m.tileRec = nullptr;
+        return e;
+    }
+    m.colorTileBase = base;
+    m.colorTileCount = count;
+    m.tileRecords = records;
+    m.tileVerts = max_verts;
+    return hipSuccess;
 }
 
 template <int LPV, int U, bool MAT, bool CONTACT, bool SAVE>
@@ -404,14 +794,14 @@ __global__ __launch_bounds__(256) void k_vertex_pass(int slot0, int count, int n
                 tmu = mm.x;
                 tlam = mm.y;
             }
-            const int sa = t.x & SLOT_MASK;
+            const int sa = t.x & SLOT_MASK, sb = t.y & SLOT_MASK, sc = t.z & SLOT_MASK;
             const bool c0 = t.x < 0;
-            const f4 ya = gather_pos(prs, sa), yb = gather_pos(prs, t.y), yc = gather_pos(prs, t.z);
+            const f4 ya = gather_pos(prs, sa), yb = gather_pos(prs, sb), yc = gather_pos(prs, sc);
             const float V = __int_as_float(t.w);
             if constexpr (MODE == MODE_FWD_RESID) {
                 pair_accumulate(c0, a, b, c, V, tmu, tlam, xs, ya, yb, yc, A);
             } else {
-                const f4 wa = gather_pos(wrs, sa), wb = gather_pos(wrs, t.y), wc = gather_pos(wrs, t.z);
+                const f4 wa = gather_pos(wrs, sa), wb = gather_pos(wrs, sb), wc = gather_pos(wrs, sc);
                 const Kin k = pair_kinematics(c0, a, b, c, xs, ya, yb, yc);
                 float o0, o1, o2;
                 pair_hvp(k, c0, a, b, c, V, tmu, tlam, ws, wa, wb, wc, o0, o1, o2);
@@ -1022,9 +1412,10 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
                                                      const int* __restrict__ entry, const i4* __restrict__ tets,
                                                      const float* __restrict__ dminv, const float* __restrict__ vol,
                                                      const float* __restrict__ tmu, const float* __restrict__ tlam,
-                                                     const int* __restrict__ row, i4* __restrict__ ptet,
-                                                     f4* __restrict__ pa, f4* __restrict__ pb, float* __restrict__ pc,
-                                                     f2* __restrict__ pmat) {
+                                                     const int* __restrict__ row, const int* __restrict__ chunkBase,
+                                                     i4* __restrict__ ptet, f4* __restrict__ pa, f4* __restrict__ pb,
+                                                     float* __restrict__ pc, f2* __restrict__ pmat,
+                                                     f4* __restrict__ pq, float* __restrict__ pqq) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nFree) return;
     const int v = perm[s];
@@ -1054,10 +1445,12 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
         float Q[9];
         for (int n = 0; n < 3; ++n)
             for (int j = 0; j < 3; ++j) Q[3 * n + j] = D[3 * rows[n] + j];
+        // owner offset from the chunk's first owner (checked on the host to fit 12 bits)
+        const int own = (s - chunkBase[p >> CHUNK_SHIFT]) & 0xFFF;
         i4 r;
-        r.x = o[0] | (k == 0 ? (int)0x80000000 : 0);
-        r.y = o[1];
-        r.z = o[2];
+        r.x = o[0] | ((own & 15) << 27) | (k == 0 ? (int)0x80000000 : 0);
+        r.y = o[1] | (((own >> 4) & 15) << 27);
+        r.z = o[2] | (((own >> 8) & 15) << 27);
         r.w = __float_as_int(vol[t]);
         ptet[p] = r;
         f4 a, b;
@@ -1066,6 +1459,27 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
         pa[p] = a;
         pb[p] = b;
         pc[p] = Q[8];
+        {
+            // compact record (tiled kernel), in double from the FP32 DmInv rows R_n:
+            // q = -(R0+R1+R2) (corner 0) or R0, alpha_n = R_n . q, beta = -alpha_0 (corner 0) or
+            // alpha_0, d = det(DmInv) * (-1, +1, -1, +1)[k], V |q|^2
+            double R[3][3], q[3], al[3];
+            for (int n = 0; n < 3; ++n)
+                for (int j = 0; j < 3; ++j) R[n][j] = (double)Q[3 * n + j];
+            for (int j = 0; j < 3; ++j) q[j] = k == 0 ? -(R[0][j] + R[1][j] + R[2][j]) : R[0][j];
+            for (int n = 0; n < 3; ++n) al[n] = R[n][0] * q[0] + R[n][1] * q[1] + R[n][2] * q[2];
+            double M[9];
+            for (int j = 0; j < 9; ++j) M[j] = (double)D[j];
+            const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                               M[2] * (M[3] * M[7] - M[4] * M[6]);
+            f4 cq;
+            cq.x = (float)(k == 0 ? -al[0] : al[0]);
+            cq.y = (float)al[1];
+            cq.z = (float)al[2];
+            cq.w = (float)((k == 0 || k == 2) ? -det : det);
+            pq[p] = cq;
+            pqq[p] = (float)((double)vol[t] * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]));
+        }
         if (pmat) {
             f2 mm;
             mm.x = tmu[t];
@@ -1224,22 +1638,40 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
         m.nPairs = hrow_last[0];
         m.maxValence = 0;
         m.valenceHist.assign(1, 0);
+        m.hostRow.assign((size_t)m.nFree + 1, 0);
         for (int i = 0; i < m.nFree; ++i) {
             const int d = hcnt[i];
             if (d > m.maxValence) m.maxValence = d;
             if ((int)m.valenceHist.size() <= d) m.valenceHist.resize(d + 1, 0);
             m.valenceHist[d]++;
+            m.hostRow[i + 1] = m.hostRow[i] + d;
         }
+        // chunk c's base = owner slot of record c * 4096; every record's owner must lie within
+        // 4095 slots of its chunk's base (fails only with long runs of tet-less free vertices)
+        const int64_t nChunks = (m.nPairs + (1 << CHUNK_SHIFT) - 1) >> CHUNK_SHIFT;
+        std::vector<int> hcb((size_t)std::max<int64_t>(nChunks, 1), 0);
+        m.tiledOk = true;
+        for (int i = 0; i < m.nFree; ++i) {
+            const int a = m.hostRow[i], b = m.hostRow[i + 1];
+            for (int64_t c = ((int64_t)a + (1 << CHUNK_SHIFT) - 1) >> CHUNK_SHIFT; (c << CHUNK_SHIFT) < b; ++c)
+                hcb[c] = i;
+            if (a < b && i - hcb[a >> CHUNK_SHIFT] > 4095) m.tiledOk = false;
+        }
+        VBD_TRY(dalloc(m, &m.chunkBase, hcb.size()));
+        VBD_TRY(hipMemcpyAsync(m.chunkBase, hcb.data(), sizeof(int) * hcb.size(), hipMemcpyHostToDevice, s));
+        VBD_TRY(hipStreamSynchronize(s));
     }
     VBD_TRY(dalloc(m, &m.ptet, (size_t)m.nPairs));
     VBD_TRY(dalloc(m, &m.pa, (size_t)m.nPairs));
     VBD_TRY(dalloc(m, &m.pb, (size_t)m.nPairs));
     VBD_TRY(dalloc(m, &m.pc, (size_t)m.nPairs));
     if (m.perTetMat) VBD_TRY(dalloc(m, &m.pmat, (size_t)m.nPairs));
+    VBD_TRY(dalloc(m, &m.pq, (size_t)m.nPairs));
+    VBD_TRY(dalloc(m, &m.pqq, (size_t)m.nPairs));
     if (m.nFree > 0) {
         hipLaunchKernelGGL(k_build_pairs, grid_for(m.nFree), dim3(256), 0, s, m.nFree, m.perm, m.inv, m.adjOff,
-                           m.adjEntry, d_tets, d_dminv, d_vol, d_mu, d_lam, m.row, m.ptet, m.pa, m.pb, m.pc,
-                           m.perTetMat ? m.pmat : nullptr);
+                           m.adjEntry, d_tets, d_dminv, d_vol, d_mu, d_lam, m.row, m.chunkBase, m.ptet, m.pa,
+                           m.pb, m.pc, m.perTetMat ? m.pmat : nullptr, m.pq, m.pqq);
         VBD_TRY(hipGetLastError());
     }
 
@@ -1307,7 +1739,7 @@ fail:
 void free_device_mesh(DevMesh& m) {
     void* ptrs[] = {m.pos, m.inert, m.vel, m.xprev, m.save[0], m.save[1], m.mass, m.row, m.perm, m.inv,
                     m.ptet, m.pa, m.pb, m.pc, m.pmat, m.adjOff, m.adjEntry, m.colorList, m.tets, m.dminv,
-                    m.vol, m.tmu, m.tlam};
+                    m.vol, m.tmu, m.tlam, m.chunkBase, m.tileStart, m.tileRec, m.pq, m.pqq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     m = DevMesh();
