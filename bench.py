@@ -239,7 +239,7 @@ def main():
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=None, help="color-kernel lanes per vertex (0: tiled kernel)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--unroll", type=int, default=1)
     ap.add_argument("--slot-order", type=int, default=0, help="vertex order inside a color: 0 auto, 1 vertex id, 2 Morton")
@@ -272,7 +272,7 @@ def main():
     log(f"[bench] cube{args.n}: nV={info.num_verts} nT={info.num_tets} colors={info.num_colors} "
         f"pairs={info.num_pairs} device={info.device_bytes / 2**30:.2f} GiB setup {time.time() - t0:.1f}s")
     tune = None
-    if args.lanes:
+    if args.lanes is not None:
         d.set_launch_config(args.lanes, args.block, args.unroll)
     elif not args.no_autotune:
         d.set_positions(m.x0)
@@ -304,7 +304,8 @@ def main():
     per_launch_bytes = cbytes / info.num_colors
     per_launch_s = cms / cn / 1e3
     achieved = per_launch_bytes / per_launch_s / 1e9
-    traffic = read_traffic(f"traffic_color_solve_cube{args.n}")
+    kernel = "k_color_tiled" if info.lanes_per_vertex == 0 else "k_color_solve"
+    traffic = read_traffic(f"traffic_{kernel[2:]}_cube{args.n}")
     b_alg_per_vi = (cbytes + bbytes) / info.num_verts
     overall = vi / secs * b_alg_per_vi / 1e9
     out = {
@@ -327,7 +328,8 @@ def main():
                    "block_size": info.block_size},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_color_solve", "alg_bytes_per_launch": round(per_launch_bytes),
+                     "traffic_gbs": round(traffic / per_launch_s / 1e9, 1) if traffic else None,
+                     "kernel": kernel, "alg_bytes_per_launch": round(per_launch_bytes),
                      "avg_launch_ms": round(per_launch_s * 1e3, 4), "launches_timed": cn},
         "roofline_step": {"alg_bytes_per_vertex_iter": round(b_alg_per_vi, 1), "achieved": round(overall, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(overall / HBM_PEAK_GBS, 4),

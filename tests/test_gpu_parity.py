@@ -225,11 +225,15 @@ def test_noisy_cube_32_matches_oracle():
     """The 128^3 workload's setup (noise, pins, Chebyshev) at 32^3, 20 iterations, 3 frames."""
     m = meshgen.noisy_cube(32, seed=0)
     prep = O.prepare(m)
-    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=20, rho=0.95, threads=8), m.x0, None, 3)
+    p = O.make_params(dt=H, iterations=20, rho=0.95, threads=8)
+    _, _, of, _ = O.simulate(prep, p, m.x0, None, 3)
+    _, _, of64, _ = O.simulate(prep, p, m.x0, None, 3, f64=True, rest=m.rest, density=m.density)
     got, _ = run_gpu_frames(m, params_for(m, 20), 3)
     rel = max(relmax(got[f], of[f]) for f in range(3))
-    print(f"cube32: {rel:.3e}", flush=True)
-    assert rel <= TOL
+    rel64 = max(relmax(got[f], of64[f]) for f in range(3))
+    o64 = max(relmax(of[f], of64[f]) for f in range(3))
+    print(f"cube32: {rel:.3e}, gpu vs fp64 {rel64:.3e}, fp32 oracle vs fp64 {o64:.3e}", flush=True)
+    assert rel <= TOL and rel64 <= TOL
 
 
 def test_cube128_color_launches_match_oracle_on_sampled_vertices():
@@ -274,11 +278,15 @@ def test_sphere_with_contact_matches_oracle():
     prep = O.prepare(m)
     frames = 20  # contact starts near frame 12
     _, _, of, _ = O.simulate(prep, oracle_params(m, 30, contact=contact), m.x0, None, frames)
+    _, _, of64, _ = O.simulate(prep, oracle_params(m, 30, contact=contact), m.x0, None, frames, f64=True, rest=m.rest,
+                               density=m.density)
     got, _ = run_gpu_frames(m, params_for(m, 30, contact=contact), frames)
     rel = max(relmax(got[f], of[f]) for f in range(frames))
+    rel64 = max(relmax(got[f], of64[f]) for f in range(frames))
+    o64 = max(relmax(of[f], of64[f]) for f in range(frames))
     assert of[-1][:, 2].min() < -1.19  # contact is active by the last frame
-    print(f"sphere24 contact: {rel:.3e}", flush=True)
-    assert rel <= TOL
+    print(f"sphere24 contact: {rel:.3e}, gpu vs fp64 {rel64:.3e}, fp32 oracle vs fp64 {o64:.3e}", flush=True)
+    assert rel <= TOL and rel64 <= TOL
 
 
 @pytest.mark.parametrize("gen", ["sphere", "cube"])
@@ -338,9 +346,14 @@ def test_batch_equals_per_mesh_runs_bitwise(cfg):
     # and the batch agrees with the oracle on one of its meshes
     m = ms[3]
     prep = O.prepare(m)
-    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=20, rho=0.95), m.x0, None, 4)
+    p = O.make_params(dt=H, iterations=20, rho=0.95)
+    _, _, of, _ = O.simulate(prep, p, m.x0, None, 4)
+    _, _, of64, _ = O.simulate(prep, p, m.x0, None, 4, f64=True, rest=m.rest, density=m.density)
     o = sum(x.num_verts for x in ms[:3])
-    assert relmax(got[o:o + m.num_verts], of[-1]) <= TOL
+    rel, rel64 = relmax(got[o:o + m.num_verts], of[-1]), relmax(got[o:o + m.num_verts], of64[-1])
+    print(f"batch mesh 3: {rel:.3e}, gpu vs fp64 {rel64:.3e}, fp32 oracle vs fp64 {relmax(of[-1], of64[-1]):.3e}",
+          flush=True)
+    assert rel <= TOL and rel64 <= TOL
 
 
 def test_device_energy_matches_oracle():
