@@ -199,9 +199,15 @@ int vbd_color_vertices(int32_t num_verts, const int32_t* tets, const int32_t* ad
 int vbd_precompute_tet_data(const float* rest, int32_t num_tets, const int32_t* tets, float* dminv,
                             float* rest_volume);
 float vbd_accelerator_omega(int32_t order, float rho, float prev_omega);
-/* Device reduction of the diagnostic log-Neo-Hookean energy (|det F| for the singular
- * value product) over the current positions. */
+/* VBDsolver::computeTotalEnergy (VBD.cpp:258-288): device reduction of the diagnostic
+ * log-Neo-Hookean energy (|det F| for the singular value product) over the current positions,
+ * in double with a fixed summation order (bit-reproducible). */
 int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy);
+/* The incremental potential the step minimises (no reference counterpart; the consistent
+ * objective for convergence monitoring): G = sum_free m/(2h^2) |x - y|^2 + sum_tets V0 Psi(F)
+ * (+ the penalty contact energy), Psi the stable Neo-Hookean (J-1) energy of Energy.cpp:93-97,
+ * h = dt / substeps and materials from the parameters (or per-tet).  elastic = the Psi sum. */
+int vbd_incremental_potential(vbd_solver* h, double* potential, double* elastic);
 
 /* ---- differentiation (no reference counterpart: the reference is not differentiable) ----
  * The adjoint of one implicit step x_{n+1} = argmin G needs A = M/h^2 + K(x_{n+1}) (K = Hessian

@@ -365,6 +365,62 @@ def test_device_energy_matches_oracle():
     assert np.isclose(e, eo, rtol=1e-4), (e, eo)
 
 
+def _potential_fp64(m, x, y, mass, contact=None):
+    from oracle import diff_oracle as D
+    P = D.make_problem(m)
+    mu = np.full(m.num_tets, m.mu) if m.tet_mu is None else m.tet_mu.astype(np.float64)
+    lam = np.full(m.num_tets, m.lam) if m.tet_lambda is None else m.tet_lambda.astype(np.float64)
+    el = D.energy(P, x.astype(np.float64), mu, lam)
+    fr = m.fixed == 0
+    d = (x.astype(np.float64) - y)[fr]
+    ine = 0.5 * float(np.sum(mass.astype(np.float64)[fr] * np.sum(d * d, axis=1))) / (H * H)
+    if contact is not None:
+        ax, plane, k = contact
+        pen = np.maximum(plane - x.astype(np.float64)[fr, ax], 0.0)
+        ine += 0.5 * k * float(np.sum(pen * pen))
+    return el + ine, el
+
+
+@pytest.mark.parametrize("gen", ["cube", "sphere_contact"])
+def test_incremental_potential_matches_fp64(gen):
+    """G = inertia + stable NH elastic (+ penalty) on the device (double, fixed order) vs float64
+    numpy with the FP64 oracle's DmInv and volumes (1e-5 relative)."""
+    if gen == "cube":
+        m, contact = meshgen.noisy_cube(10, seed=2), None
+    else:
+        m, contact = meshgen.voxel_sphere(14, seed=1), (2, -0.9, 1e4)
+    rng = np.random.default_rng(7)
+    t, _, _, _, _ = prepare_host(m)
+    x = (m.rest + rng.normal(scale=0.02, size=m.rest.shape)).astype(np.float32)
+    y = (m.rest + rng.normal(scale=0.02, size=m.rest.shape)).astype(np.float32)
+    d = upload(m, params=params_for(m, 10, contact=contact))
+    d.set_positions(x)
+    d.set_inertia(y)
+    G, el = d.incremental_potential()
+    G64, el64 = _potential_fp64(m, x, y, t.vertexMass, contact)
+    print(f"{gen}: G {G:.9e} vs {G64:.9e}, elastic {el:.9e} vs {el64:.9e}", flush=True)
+    assert abs(G - G64) <= 1e-5 * abs(G64) and abs(el - el64) <= 1e-5 * abs(el64)
+    assert d.incremental_potential() == (G, el)  # fixed summation order: bit-reproducible
+    e1 = d.total_energy(m.mu, m.lam)
+    assert d.total_energy(m.mu, m.lam) == e1
+
+
+def test_sweeps_lower_the_incremental_potential():
+    """Each plain Gauss-Seidel sweep (no Chebyshev) lowers G from a perturbed state."""
+    m = meshgen.noisy_cube(16, seed=4)
+    d = upload(m, params=params_for(m, 10, accel=False))
+    y = m.x0.copy()
+    y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
+    d.set_inertia(y)
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    gs = [d.incremental_potential()[0]]
+    for _ in range(6):
+        d.solve_iteration(sdt, m.mu, m.lam)
+        gs.append(d.incremental_potential()[0])
+    print("G per sweep:", " ".join(f"{g:.6e}" for g in gs), flush=True)
+    assert all(b < a for a, b in zip(gs, gs[1:]))
+
+
 def test_solve_before_init_is_a_state_error():
     from vbd_tetmesh_amd import _capi as A
     d = DeviceMesh()

@@ -120,6 +120,7 @@ SIGNATURES = [
     ("vbd_precompute_tet_data", C.c_int, [f32p, C.c_int32, i32p, f32p, f32p]),
     ("vbd_accelerator_omega", C.c_float, [C.c_int32, C.c_float, C.c_float]),
     ("vbd_total_energy", C.c_int, [H, C.c_float, C.c_float, f64p]),
+    ("vbd_incremental_potential", C.c_int, [H, f64p, f64p]),
     ("vbd_set_materials", C.c_int, [H, f32p, f32p]),
     ("vbd_residual_norm", C.c_int, [H, f64p, f64p]),
     ("vbd_step_converged", C.c_int, [H, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, i32p, f64p]),

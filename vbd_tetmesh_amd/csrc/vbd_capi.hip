@@ -32,7 +32,8 @@ struct vbd_solver {
     bool graphEnabled = true;
     hipGraphExec_t graphExec = nullptr;
     bool graphValid = false;
-    double* d_energy = nullptr;
+    double* d_energy = nullptr;   // 2 doubles: energy results
+    double* d_epart = nullptr;    // per-block energy partial sums
     std::vector<int32_t> meshVertOffsets;  // batched uploads: per-mesh vertex offsets
     // differentiation scratch, allocated on first use (slot order, nV entries)
     f4* d_w = nullptr;
@@ -84,11 +85,12 @@ void drop_graph(vbd_solver* h) {
 }
 
 void free_scratch(vbd_solver* h) {
-    void* ps[] = {h->d_w, h->d_rhs, h->d_out, h->d_wsave[0], h->d_wsave[1], h->d_r2, h->d_sum, h->d_tgrad, h->d_x3};
+    void* ps[] = {h->d_w,  h->d_rhs, h->d_out,   h->d_wsave[0], h->d_wsave[1],
+                  h->d_r2, h->d_sum, h->d_tgrad, h->d_x3,       h->d_epart};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     h->d_w = h->d_rhs = h->d_out = h->d_wsave[0] = h->d_wsave[1] = nullptr;
-    h->d_r2 = h->d_sum = nullptr;
+    h->d_r2 = h->d_sum = h->d_epart = nullptr;
     h->d_tgrad = h->d_x3 = nullptr;
 }
 
@@ -247,6 +249,11 @@ int heuristic_lpv(const DevMesh& m) {
     return l;
 }
 
+hipError_t ensure_energy_scratch(vbd_solver* h) {
+    if (h->d_epart) return hipSuccess;
+    return hipMalloc(&h->d_epart, sizeof(double) * (size_t)energy_partials(h->m));
+}
+
 // Default launch configuration: the tiled kernel with the smallest tile (block 256 with 1 or 2
 // records per thread, block 512 with 2) that holds the largest valence; else the per-vertex
 // kernel with the histogram heuristic.
@@ -293,7 +300,7 @@ int vbd_create(vbd_solver** out, int32_t device) {
         delete h;
         return hip_fail(nullptr, "hipStreamCreateWithFlags", e);
     }
-    e = hipMalloc(&h->d_energy, sizeof(double));
+    e = hipMalloc(&h->d_energy, sizeof(double) * 2);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(h->stream);
         delete h;
@@ -980,9 +987,29 @@ int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy) {
     if (rc) return rc;
     if (!energy) return fail(h, VBD_ERR_ARG, "null output");
     DeviceGuard g(h->device);
-    HIP_CHECK(h, launch_energy(h->m, mu, lambda, h->d_energy, h->stream));
+    HIP_CHECK(h, ensure_energy_scratch(h));
+    HIP_CHECK(h, launch_energy(h->m, 0, mu, lambda, h->d_epart, h->d_energy, h->stream));
     HIP_CHECK(h, hipMemcpyAsync(energy, h->d_energy, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_incremental_potential(vbd_solver* h, double* potential, double* elastic) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    DeviceGuard g(h->device);
+    const vbd_step_params& p = h->params;
+    const float hstep = p.dt / (float)p.substeps;
+    double out[2] = {0.0, 0.0};
+    HIP_CHECK(h, ensure_energy_scratch(h));
+    HIP_CHECK(h, launch_energy(h->m, 1, p.mu, p.lambda, h->d_epart, h->d_energy, h->stream));
+    HIP_CHECK(h, launch_inertia_energy(h->m, 1.0 / ((double)hstep * hstep), contact_of(p), h->d_epart,
+                                       h->d_energy + 1, h->stream));
+    HIP_CHECK(h, hipMemcpyAsync(out, h->d_energy, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    if (elastic) *elastic = out[0];
+    if (potential) *potential = out[0] + out[1];
     return VBD_OK;
 }
 

@@ -98,7 +98,13 @@ hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t
 hipError_t launch_copy_inertia(const DevMesh& m, hipStream_t s);
 hipError_t launch_scatter_pos(int n, const float* src3, const int* inv, f4* dst, hipStream_t s);
 hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, hipStream_t s);
-hipError_t launch_energy(const DevMesh& m, float mu, float lambda, double* out, hipStream_t s);
+// energies in double, fixed summation order; partial holds energy_partials(m) doubles
+// mode 0: the reference's log-NH diagnostic, 1: the stable NH (J-1) elastic energy
+int energy_partials(const DevMesh& m);
+hipError_t launch_energy(const DevMesh& m, int mode, float mu, float lambda, double* partial, double* out,
+                         hipStream_t s);
+hipError_t launch_inertia_energy(const DevMesh& m, double sdt, const ContactArgs& ct, double* partial, double* out,
+                                 hipStream_t s);
 hipError_t launch_copy_f4(int n, const f4* src, f4* dst, hipStream_t s);
 // differentiation passes (mode: 0 adjoint GS update, 1 adjoint residual, 2 A*w, 3 forward residual)
 hipError_t launch_pass(int mode, const DevMesh& m, int slot0, int count, int lpv, int block, f4* w, const f4* rhs,

@@ -1218,15 +1218,32 @@ hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Diagnostic energy (VBD.cpp:258-288 with NeoHookean.cpp:6-20): sum over tets of
-// V0 [mu/2 (|F|^2 - 3) - mu ln J + lambda/2 (ln J)^2], J = |det F| (the product of the
-// non-negative singular values).  Accumulated in double.
+// Energies, summed in double with a fixed order (per-block partial sums, then k_sum_fixed):
+//   ENERGY_LOG_NH:    the reference's diagnostic (VBD.cpp:258-288 with NeoHookean.cpp:6-20),
+//                     V0 [mu/2 (|F|^2 - 3) - mu ln J + lambda/2 (ln J)^2], J = |det F| (the
+//                     product of the non-negative singular values);
+//   ENERGY_STABLE_NH: the elastic part of the incremental potential the solver minimises,
+//                     V0 [mu/2 (|F|^2 - 3) - mu (J - 1) + lambda/2 (J - 1)^2] (Energy.cpp:93-97);
+// and the inertia part sum over free vertices of m/(2h^2) |x - y|^2 (+ penalty k/2 d^2).
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_energy(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
-                                                const f4* __restrict__ pos, const float* __restrict__ dminv,
-                                                const float* __restrict__ vol, const float* __restrict__ tmu,
-                                                const float* __restrict__ tlam, float mu, float lambda,
-                                                double* __restrict__ out) {
+constexpr int ENERGY_LOG_NH = 0, ENERGY_STABLE_NH = 1;
+
+__device__ __forceinline__ double block_sum_256(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_tet_energy(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
+                                                    const f4* __restrict__ pos, const float* __restrict__ dminv,
+                                                    const float* __restrict__ vol, const float* __restrict__ tmu,
+                                                    const float* __restrict__ tlam, float mu, float lambda,
+                                                    double* __restrict__ partial) {
     __shared__ double red[256];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
@@ -1234,39 +1251,75 @@ __global__ __launch_bounds__(256) void k_energy(int nT, const i4* __restrict__ t
         const i4 tt = tets[t];
         const f4 x0 = pos[inv[tt.x]], x1 = pos[inv[tt.y]], x2 = pos[inv[tt.z]], x3 = pos[inv[tt.w]];
         const float* D = dminv + 9 * (size_t)t;
-        const float ds[3][3] = {{x1.x - x0.x, x2.x - x0.x, x3.x - x0.x},
-                                {x1.y - x0.y, x2.y - x0.y, x3.y - x0.y},
-                                {x1.z - x0.z, x2.z - x0.z, x3.z - x0.z}};
-        float F[3][3];
-        float tr = 0.f;
+        // F = Ds DmInv in double: near rest |F|^2 - 3 and J - 1 cancel to O(strain)
+        const double ds[3][3] = {{(double)x1.x - x0.x, (double)x2.x - x0.x, (double)x3.x - x0.x},
+                                 {(double)x1.y - x0.y, (double)x2.y - x0.y, (double)x3.y - x0.y},
+                                 {(double)x1.z - x0.z, (double)x2.z - x0.z, (double)x3.z - x0.z}};
+        double F[3][3];
+        double tr = 0.0;
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) {
                 F[r][c] = ds[r][0] * D[c] + ds[r][1] * D[3 + c] + ds[r][2] * D[6 + c];
                 tr += F[r][c] * F[r][c];
             }
-        const float det = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
-                          F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
-                          F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
-        const float J = fabsf(det);
-        const float m_ = tmu ? tmu[t] : mu, l_ = tlam ? tlam[t] : lambda;
-        const double lj = log((double)J);
-        e = (0.5 * m_ * (tr - 3.0) - m_ * lj + 0.5 * l_ * lj * lj) * vol[t];
+        const double det = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                           F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                           F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+        const double m_ = tmu ? tmu[t] : mu, l_ = tlam ? tlam[t] : lambda;
+        if constexpr (MODE == ENERGY_LOG_NH) {
+            const double lj = log(fabs(det));
+            e = (0.5 * m_ * (tr - 3.0) - m_ * lj + 0.5 * l_ * lj * lj) * vol[t];
+        } else {
+            e = (0.5 * m_ * (tr - 3.0) - m_ * (det - 1.0) + 0.5 * l_ * (det - 1.0) * (det - 1.0)) * vol[t];
+        }
     }
-    red[threadIdx.x] = e;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+    const double sum = block_sum_256(e, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sum;
 }
 
-hipError_t launch_energy(const DevMesh& m, float mu, float lambda, double* out, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(out, 0, sizeof(double), s);
-    if (e != hipSuccess || m.nT == 0) return e;
-    hipLaunchKernelGGL(k_energy, dim3((m.nT + 255) / 256), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos, m.dminv,
-                       m.vol, m.tmu, m.tlam, mu, lambda, out);
-    return hipGetLastError();
+__global__ __launch_bounds__(256) void k_inertia_energy(int nFree, const f4* __restrict__ pos,
+                                                        const f4* __restrict__ inert, const float* __restrict__ mass,
+                                                        double sdt, ContactArgs ct, double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (s < nFree) {
+        const f4 x = pos[s], y = inert[s];
+        const double dx = (double)x.x - y.x, dy = (double)x.y - y.y, dz = (double)x.z - y.z;
+        e = 0.5 * mass[s] * sdt * (dx * dx + dy * dy + dz * dz);
+        if (ct.on) {
+            const double xa = ct.axis == 0 ? x.x : (ct.axis == 1 ? x.y : x.z);
+            const double d = (double)ct.plane - xa;
+            if (d > 0.0) e += 0.5 * (double)ct.k * d * d;
+        }
+    }
+    const double sum = block_sum_256(e, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sum;
+}
+
+int energy_partials(const DevMesh& m) { return std::max(1, std::max((m.nT + 255) / 256, (m.nFree + 255) / 256)); }
+
+hipError_t launch_energy(const DevMesh& m, int mode, float mu, float lambda, double* partial, double* out,
+                         hipStream_t s) {
+    const int nb = (m.nT + 255) / 256;
+    if (nb == 0) return launch_sum_fixed(0, partial, out, s);
+    if (mode == ENERGY_LOG_NH)
+        hipLaunchKernelGGL(k_tet_energy<ENERGY_LOG_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
+                           m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
+    else
+        hipLaunchKernelGGL(k_tet_energy<ENERGY_STABLE_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
+                           m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s);
+}
+
+hipError_t launch_inertia_energy(const DevMesh& m, double sdt, const ContactArgs& ct, double* partial, double* out,
+                                 hipStream_t s) {
+    const int nb = (m.nFree + 255) / 256;
+    if (nb == 0) return launch_sum_fixed(0, partial, out, s);
+    hipLaunchKernelGGL(k_inertia_energy, dim3(nb), dim3(256), 0, s, m.nFree, m.pos, m.inert, m.mass, sdt, ct, partial);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s);
 }
 
 // ---------------------------------------------------------------------------------------

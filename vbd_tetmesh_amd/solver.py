@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import Callable, List, Optional, Sequence
+from typing import Callable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -352,9 +352,16 @@ class DeviceMesh(_Handle):
         return cb.value, bb.value
 
     def total_energy(self, mu: float, lam: float) -> float:
+        """VBDsolver::computeTotalEnergy's log-Neo-Hookean diagnostic (VBD.cpp:258-288)."""
         e = C.c_double()
         check(lib.vbd_total_energy(self.h, mu, lam, C.byref(e)), self.h)
         return e.value
+
+    def incremental_potential(self) -> Tuple[float, float]:
+        """(G, elastic part) of the incremental potential at the current positions and inertia."""
+        g, e = C.c_double(), C.c_double()
+        check(lib.vbd_incremental_potential(self.h, C.byref(g), C.byref(e)), self.h)
+        return g.value, e.value
 
     # -- differentiation
     def set_materials(self, tet_mu, tet_lambda):
