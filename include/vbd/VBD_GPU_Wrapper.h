@@ -42,6 +42,10 @@ class VBD_GPU_Solver {
         return *this;
     }
 
+    // Extension: VBD_MATERIAL_COROTATED_GPU reproduces the reference GPU kernel's corotated model
+    // (VBD_GPU_Wrapper.cu:181); the default is the CPU solver's stable Neo-Hookean.
+    void setMaterialModel(int model) { vbd_throw_if(vbd_set_material_model(h_, model), h_, "setMaterialModel"); }
+
     void initFromCPU(const TetMesh& mesh, const std::vector<std::vector<int>>& vertexColors,
                      const std::vector<Matrix3r>& tetDmInv, const std::vector<Real>& tetRestVol) {
         std::vector<uint8_t> fixed(mesh.numVerts);

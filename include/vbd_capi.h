@@ -156,6 +156,17 @@ int vbd_synchronize(vbd_solver* h);
 /* Enables (1, default) or disables (0) hipGraph capture of vbd_step. */
 int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
 
+/* ---- material model ------------------------------------------------------------------ */
+/* VBD_MATERIAL_STABLE_NEO_HOOKEAN (default): the CPU solver's material, VBDsolver / Energy.cpp.
+ * VBD_MATERIAL_COROTATED_GPU: what the reference GPU kernel runs (VBD_GPU_Wrapper.cu:181 with
+ *   VBD_DeviceMath.cuh:127-198): corotated linear elasticity with R from 5 polar iterations,
+ *   H = V [2 mu |q|^2 I + lambda (R q)(R q)^T], and the wrapper's m/h^2 * 1.000001 Hessian
+ *   diagonal (.cu:208).  Runs the per-vertex kernel (lanes 8/16/32, else 16).  The
+ *   differentiation entry points reject it. */
+#define VBD_MATERIAL_STABLE_NEO_HOOKEAN 0
+#define VBD_MATERIAL_COROTATED_GPU 1
+int vbd_set_material_model(vbd_solver* h, int32_t model);
+
 /* ---- launch configuration, autotuning and timing (no reference counterpart) ------- */
 /* Color-kernel mapping.
  * lanes_per_vertex > 0: per-vertex kernel on the full records (F and cof F formed as in
@@ -205,8 +216,9 @@ float vbd_accelerator_omega(int32_t order, float rho, float prev_omega);
 int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy);
 /* The incremental potential the step minimises (no reference counterpart; the consistent
  * objective for convergence monitoring): G = sum_free m/(2h^2) |x - y|^2 + sum_tets V0 Psi(F)
- * (+ the penalty contact energy), Psi the stable Neo-Hookean (J-1) energy of Energy.cpp:93-97,
- * h = dt / substeps and materials from the parameters (or per-tet).  elastic = the Psi sum. */
+ * (+ the penalty contact energy), Psi the stable Neo-Hookean (J-1) energy of Energy.cpp:93-97
+ * (or, with VBD_MATERIAL_COROTATED_GPU, mu |F - R|^2 + lambda/2 tr(R^T F - I)^2), h = dt /
+ * substeps and materials from the parameters (or per-tet).  elastic = the Psi sum. */
 int vbd_incremental_potential(vbd_solver* h, double* potential, double* elastic);
 
 /* ---- differentiation (no reference counterpart: the reference is not differentiable) ----

@@ -39,6 +39,7 @@ class Params(C.Structure):
         ("mode", C.c_int32),
         ("threads", C.c_int32),
         ("energyEvery", C.c_int32),
+        ("material", C.c_int32),
     ]
 
 
@@ -47,7 +48,7 @@ MODE_CLOSED, MODE_FAITHFUL, MODE_FAITHFUL_COST = 0, 1, 2
 
 def make_params(dt=1.0 / 60.0, iterations=100, substeps=1, gravity=(0.0, -9.8, 0.0), accel=True, rho=0.95,
                 mu=1e5, lam=4e5, init_from_inertia=False, contact=None, mode=MODE_CLOSED, threads=1,
-                energy_every=0) -> Params:
+                energy_every=0, material=0) -> Params:
     p = Params()
     p.dt = dt
     p.numIterations = iterations
@@ -64,6 +65,7 @@ def make_params(dt=1.0 / 60.0, iterations=100, substeps=1, gravity=(0.0, -9.8, 0
     p.mode = mode
     p.threads = threads
     p.energyEvery = energy_every
+    p.material = material
     return p
 
 
@@ -90,6 +92,8 @@ def lib():
         L.orc_tet_full_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float, f32p, f32p]
         L.orc_tet_block_f64.argtypes = [f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, f64p, f64p]
         L.orc_tet_block_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, f32p, f32p]
+        L.orc_tet_block_corot_f64.argtypes = [f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_int, f64p, f64p]
+        L.orc_polar_rotation_f64.argtypes = [f64p, f64p]
         L.orc_tet_energy_f64.argtypes = [f64p, f64p, C.c_double, C.c_double, C.c_double]
         L.orc_tet_energy_f64.restype = C.c_double
         L.orc_omega_f32.argtypes = [C.c_int, C.c_float, C.c_float]
@@ -272,6 +276,28 @@ def tet_block(x12, dminv9, mu, lam, V, k, faithful, f64=True):
     fn = L.orc_tet_block_f64 if f64 else L.orc_tet_block_f32
     fn(_p(x12, ct), _p(d9, ct), mu, lam, V, int(k), int(bool(faithful)), _p(H, ct), _p(g, ct))
     return H, g
+
+
+def tet_block_corotated(x12, dminv9, mu, lam, V, k):
+    """(k,k) block and gradient of corner k for the GPU wrapper's corotated model
+    (VBD_DeviceMath.cuh:127-198), in double."""
+    L = lib()
+    x12 = np.ascontiguousarray(x12, np.float64).reshape(12)
+    d9 = np.ascontiguousarray(dminv9, np.float64).reshape(9)
+    H = np.zeros((3, 3))
+    g = np.zeros(3)
+    L.orc_tet_block_corot_f64(_p(x12, C.c_double), _p(d9, C.c_double), mu, lam, V, int(k), _p(H, C.c_double),
+                              _p(g, C.c_double))
+    return H, g
+
+
+def polar_rotation(F):
+    """The corotated model's rotation: 5 steps of R <- (R + cof(R)/det R) / 2 from R = F."""
+    L = lib()
+    F9 = np.ascontiguousarray(F, np.float64).reshape(9)
+    R = np.zeros(9)
+    L.orc_polar_rotation_f64(_p(F9, C.c_double), _p(R, C.c_double))
+    return R.reshape(3, 3)
 
 
 def tet_energy(x12, dminv9, mu, lam, V) -> float:

@@ -172,6 +172,45 @@ static inline void block_closed(const M3<R>& F, const M3<R>& DmInv, R mu, R lamb
         for (int c = 0; c < 3; ++c) H[r][c] = V * ((r == c ? mu * qq : R(0)) + lambda * Cq[r] * Cq[c]);
 }
 
+// Corotated linear elasticity of the reference GPU kernel (VBD_DeviceMath.cuh:127-198):
+// R from 5 steps of R <- (R + cof(R)/det R) / 2 (stopping if |det R| < 1e-8),
+// P = 2 mu (F - R) + lambda tr(R^T F - I) R,  g_k = V P q_k,
+// H_kk = V [2 mu |q_k|^2 I + lambda (R q_k)(R q_k)^T] (dR/dF dropped, always SPD).
+template <class R>
+static inline M3<R> polar_rotation(const M3<R>& F) {
+    M3<R> Rm = F;
+    for (int it = 0; it < 5; ++it) {
+        R d = det3(Rm);
+        if (std::abs((double)d) < 1e-8) break;
+        M3<R> C = cofactor(Rm);
+        R inv = R(1) / d;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) Rm(r, c) = R(0.5) * (Rm(r, c) + C(r, c) * inv);
+    }
+    return Rm;
+}
+
+template <class R>
+static inline void block_corotated(const M3<R>& F, const M3<R>& DmInv, R mu, R lambda, R V, int k, R H[3][3],
+                                   R g[3]) {
+    M3<R> Rm = polar_rotation(F);
+    R tr = 0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) tr += Rm(r, c) * F(r, c);
+    tr -= R(3);
+    M3<R> P;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) P(r, c) = R(2) * mu * (F(r, c) - Rm(r, c)) + lambda * tr * Rm(r, c);
+    R q[3];  // Qn as the kernel forms it: (-D0 - D1) - D2 for corner 0
+    for (int b = 0; b < 3; ++b) q[b] = k == 0 ? -DmInv(0, b) - DmInv(1, b) - DmInv(2, b) : DmInv(k - 1, b);
+    for (int r = 0; r < 3; ++r) g[r] = V * (P(r, 0) * q[0] + P(r, 1) * q[1] + P(r, 2) * q[2]);
+    R qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    R rq[3];
+    for (int r = 0; r < 3; ++r) rq[r] = Rm(r, 0) * q[0] + Rm(r, 1) * q[1] + Rm(r, 2) * q[2];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) H[r][c] = V * ((r == c ? R(2) * mu * qq : R(0)) + lambda * rq[r] * rq[c]);
+}
+
 // Energy.cpp:114-136: 9x9 dP/dF, column i + 3j is dP for dF = e_i e_j^T, flattened as vi + 3 vj.
 template <class R>
 static inline void build_HF(R HF[9][9], const M3<R>& F, const M3<R>& C, R J, R mu, R lambda) {
@@ -265,6 +304,7 @@ struct Params {
     int32_t mode;       // 0 closed, 1 faithful block, 2 faithful with reference cost structure
     int32_t threads;    // OpenMP threads inside one color (race-free; result independent of it)
     int32_t energyEvery;  // 10 in the reference; 0 disables the diagnostic
+    int32_t material;   // 0 stable Neo-Hookean (Energy.cpp), 1 the GPU wrapper's corotated model
 };
 
 template <class R> struct Mesh {
@@ -353,8 +393,14 @@ static inline void solve_vertex(const Mesh<R>& M, const Params& p, R* x, const R
     R H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     R f[3];
     R ms = M.mass[v] * s;
-    for (int c = 0; c < 3; ++c) H[c][c] += ms;
-    for (int c = 0; c < 3; ++c) f[c] = (M.mass[v] * (y[3 * v + c] - x[3 * v + c])) * s;
+    if (p.material == 1) {
+        // VBD_GPU_Wrapper.cu:203-213: coeff = m s; H diagonal coeff * 1.000001, f = coeff (y - x)
+        for (int c = 0; c < 3; ++c) H[c][c] += ms * (R)1.000001f;
+        for (int c = 0; c < 3; ++c) f[c] = ms * (y[3 * v + c] - x[3 * v + c]);
+    } else {
+        for (int c = 0; c < 3; ++c) H[c][c] += ms;
+        for (int c = 0; c < 3; ++c) f[c] = (M.mass[v] * (y[3 * v + c] - x[3 * v + c])) * s;
+    }
     for (int e = M.adjOff[v]; e < M.adjOff[v + 1]; ++e) {
         int t = M.adj[e];
         const int32_t* tet = M.tets + 4 * t;
@@ -369,7 +415,9 @@ static inline void solve_vertex(const Mesh<R>& M, const Params& p, R* x, const R
         std::memcpy(&DmInv, M.dminv + 9 * t, sizeof(DmInv));
         matmul(F, Ds, DmInv);
         R Hl[3][3], gl[3];
-        if (p.mode == 0) {
+        if (p.material == 1) {
+            block_corotated(F, DmInv, mu, lam, M.vol[t], k, Hl, gl);
+        } else if (p.mode == 0) {
             block_closed(F, DmInv, mu, lam, M.vol[t], k, Hl, gl);
         } else if (p.mode == 1) {
             block_faithful(F, DmInv, mu, lam, M.vol[t], k, Hl, gl);
@@ -658,6 +706,27 @@ void orc_tet_block_f64(const double* x12, const double* dminv9, double mu, doubl
     else block_closed(F, DmInv, mu, lambda, V, k, H, g);
     std::memcpy(H9, H, sizeof(H));
     std::memcpy(g3, g, sizeof(g));
+}
+
+// Corotated block (material 1) and the polar factor it uses, in double.
+void orc_tet_block_corot_f64(const double* x12, const double* dminv9, double mu, double lambda, double V, int k,
+                             double* H9, double* g3) {
+    int32_t tet[4] = {0, 1, 2, 3};
+    M3<double> Ds, DmInv, F;
+    compute_Dm(Ds, x12, tet);
+    std::memcpy(&DmInv, dminv9, sizeof(DmInv));
+    matmul(F, Ds, DmInv);
+    double H[3][3], g[3];
+    block_corotated(F, DmInv, mu, lambda, V, k, H, g);
+    std::memcpy(H9, H, sizeof(H));
+    std::memcpy(g3, g, sizeof(g));
+}
+
+void orc_polar_rotation_f64(const double* F9, double* R9) {
+    M3<double> F;
+    std::memcpy(&F, F9, sizeof(F));
+    M3<double> Rm = polar_rotation(F);
+    std::memcpy(R9, &Rm, sizeof(Rm));
 }
 
 void orc_tet_block_f32(const float* x12, const float* dminv9, float mu, float lambda, float V, int k, int faithful,

@@ -1,0 +1,146 @@
+"""GPU parity of the corotated material of the reference GPU kernel (VBD_GPU_Wrapper.cu:181,
+VBD_DeviceMath.cuh:127-198) against the oracle's restatement (material 1), which
+tests/test_oracle.py pins (polar factor vs SVD, gradient vs FD, block at a rotated rest state,
+equivariance).  Tolerance: positions within 1e-5 relative max-norm per frame."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vbd_tetmesh_amd import VBDParams, meshgen
+from vbd_tetmesh_amd import _capi as A
+from vbd_tetmesh_amd.solver import VBD_GPU_Solver, prepare_host, upload
+
+pytestmark = pytest.mark.gpu
+
+H = 1.0 / 60.0
+TOL = 1e-5
+COROT = A.VBD_MATERIAL_COROTATED_GPU
+
+
+def relmax(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def params(m, iterations, contact=None, rho=0.95):
+    return VBDParams(dt=H, numIterations=iterations, gravity=m.gravity, useAcceleration=True, accelerationRho=rho,
+                     mu=m.mu, lambda_=m.lam, contact=contact, material=COROT)
+
+
+def test_corotated_beam_matches_oracle():
+    """Beam 20x4x4, 100 iterations, Chebyshev 0.95, 30 frames, corotated material."""
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=100, rho=0.95, mu=m.mu, lam=m.lam, material=1),
+                             m.x0, None, 30)
+    d = upload(m, params=params(m, 100))
+    rel = 0.0
+    for f in range(30):
+        d.step(1)
+        rel = max(rel, relmax(d.positions(), of[f]))
+    sag = m.x0[:, 1].min() - of[-1][:, 1].min()
+    print(f"corotated beam: {rel:.3e} (sag {sag:.3e})", flush=True)
+    assert sag > 0 and rel <= TOL
+
+
+def test_corotated_reference_gpu_loop():
+    """The reference main.cpp loop on VBD_GPU_Solver with its own GPU material: host forwardStep,
+    updateInertiaFromCPU, solveVBD (x <- y), copyPositionsToCPU, host updateVelocity."""
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    frames = 20
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=30, rho=0.95, mu=m.mu, lam=m.lam,
+                                                 init_from_inertia=True, material=1), m.x0, None, frames)
+    t, off, verts, dminv, vol = prepare_host(m)
+    g = VBD_GPU_Solver(material=COROT)
+    t.computeInertia(H)
+    g.initFromCPU(t, (off, verts), dminv, vol)
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    rel = 0.0
+    for f in range(frames):
+        t.savePreviousPosition()
+        t.applyGravity(m.gravity, H)
+        t.computeInertia(H)
+        g.updateInertiaFromCPU(t)
+        g.solveVBD(30, sdt, m.mu, m.lam, True, 0.95)
+        g.copyPositionsToCPU(t)
+        t.updateVelocity(H)
+        rel = max(rel, relmax(t.mVertPos, of[f]))
+    print(f"corotated GPU-solver loop: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32, 0])
+def test_corotated_sphere_contact_lanes(lanes):
+    """Shuffled-id sphere with penalty contact; the corotated kernel at each lane width (a tiled
+    configuration falls back to 16 lanes)."""
+    m = meshgen.voxel_sphere(16, seed=2)
+    contact = (2, -1.05, 1e4)
+    prep = O.prepare(m)
+    frames = 12
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=25, rho=0.95, mu=m.mu, lam=m.lam,
+                                                 gravity=m.gravity, contact=contact, material=1), m.x0, None, frames)
+    d = upload(m, params=params(m, 25, contact=contact))
+    d.set_launch_config(lanes, 128, 1)
+    rel = 0.0
+    for f in range(frames):
+        d.step(1)
+        rel = max(rel, relmax(d.positions(), of[f]))
+    print(f"corotated sphere lanes={lanes}: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_corotated_batch_with_per_tet_materials():
+    ms = meshgen.batch_beams(8, seed=3)
+    from vbd_tetmesh_amd.solver import DeviceMesh
+    descs = []
+    for m in ms:
+        t, off, verts, dminv, vol = prepare_host(m)
+        descs.append(dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                          rest_volume=vol, color_off=off, color_verts=verts, tet_mu=t.tetMu, tet_lambda=t.tetLambda))
+    b = DeviceMesh().init_batch(descs)
+    b.set_params(VBDParams(dt=H, numIterations=20, useAcceleration=True, accelerationRho=0.95, gravity=(0, -9.8, 0),
+                           material=COROT))
+    b.step(5)
+    got = b.positions()
+    m = ms[2]
+    prep = O.prepare(m)
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=20, rho=0.95, material=1), m.x0, None, 5)
+    o = sum(x.num_verts for x in ms[:2])
+    rel = relmax(got[o:o + m.num_verts], of[-1])
+    print(f"corotated batch mesh 2: {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+def test_corotated_incremental_potential_matches_fp64():
+    """G with the corotated energy mu |F - R|^2 + lambda/2 tr(R^T F - I)^2 (R = polar factor)
+    vs float64 numpy with the SVD polar factor."""
+    m = meshgen.noisy_cube(8, seed=5)
+    rng = np.random.default_rng(3)
+    x = (m.rest + rng.normal(scale=0.01, size=m.rest.shape)).astype(np.float32)
+    d = upload(m, params=params(m, 10))
+    d.set_positions(x)
+    d.set_inertia(x)  # inertia term zero: G = elastic
+    G, el = d.incremental_potential()
+    from oracle import diff_oracle as D
+    P = D.make_problem(m)
+    _, F, _, _ = D.kinematics(P, x.astype(np.float64))
+    U, _, Vt = np.linalg.svd(F)
+    R = U @ Vt
+    fr = np.sum((F - R) ** 2, axis=(1, 2))
+    tr = np.einsum("tij,tij->t", R, F) - 3.0
+    ref = float(np.sum(P.vol * (m.mu * fr + 0.5 * m.lam * tr * tr)))
+    print(f"corotated G {G:.9e} vs {ref:.9e}", flush=True)
+    assert G == el and abs(G - ref) <= 1e-5 * abs(ref)
+
+
+def test_corotated_material_rejects_differentiation_and_bad_models():
+    m = meshgen.diff_cube(3)
+    d = upload(m, params=params(m, 10))
+    with pytest.raises(A.VBDError) as ei:
+        d.step_converged(50)
+    assert ei.value.code == A.VBD_ERR_STATE
+    with pytest.raises(A.VBDError) as ei:
+        d.set_material_model(7)
+    assert ei.value.code == A.VBD_ERR_ARG
+    d.set_material_model(A.VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+    d.step_converged(50)

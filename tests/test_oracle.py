@@ -311,3 +311,109 @@ def test_contact_term_is_fd_of_penalty():
             assert np.isclose(xo[i, 2], expect, rtol=1e-5)
         else:
             assert xo[i, 2] == x[i, 2]
+
+
+# ---- corotated model of the reference GPU kernel (VBD_DeviceMath.cuh:127-198) ----------------
+
+def exact_polar(F):
+    U, s, Vt = np.linalg.svd(F)
+    R = U @ Vt
+    if np.linalg.det(R) < 0:
+        U[:, -1] *= -1
+        R = U @ Vt
+    return R
+
+
+def corotated_energy(x, dminv, mu, lam, V):
+    F = (x[1:] - x[0]).T @ dminv
+    R = exact_polar(F)
+    return V * (mu * np.sum((F - R) ** 2) + 0.5 * lam * (np.trace(R.T @ F) - 3) ** 2)
+
+
+def random_rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q @ np.diag(np.sign(np.diag(r)))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_corotated_rotation_is_the_polar_factor(seed):
+    """Five Higham steps reach the SVD polar factor for stretches in [0.7, 1.4]."""
+    rng = np.random.default_rng(300 + seed)
+    F = random_rotation(rng) @ np.diag(rng.uniform(0.7, 1.4, 3)) @ random_rotation(rng)
+    R = O.polar_rotation(F)
+    assert np.abs(R - exact_polar(F)).max() < 1e-9
+    assert np.abs(R.T @ R - np.eye(3)).max() < 1e-9
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_corotated_gradient_is_fd_of_corotated_energy(seed):
+    """g_k = V [2 mu (F - R) + lambda tr(R^T F - I) R] q_k == central FD of
+    V [mu |F - R|^2 + lambda/2 tr(R^T F - I)^2] with the exact polar factor."""
+    rng = np.random.default_rng(400 + seed)
+    _, x, dminv, V = random_tet(rng, deform=0.08)
+    mu, lam = 1e5, 4e5
+    eps = 1e-6
+    for k in range(4):
+        _, g = O.tet_block_corotated(x, dminv, mu, lam, V, k)
+        fd = np.zeros(3)
+        for c in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[k, c] += eps
+            xm[k, c] -= eps
+            fd[c] = (corotated_energy(xp, dminv, mu, lam, V) - corotated_energy(xm, dminv, mu, lam, V)) / (2 * eps)
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max()), (k, g, fd)
+
+
+def test_corotated_block_at_a_rotated_rest_state():
+    """At F = Q (rigid rotation) the force vanishes and the block H = V [2 mu |q|^2 I +
+    lambda (Qq)(Qq)^T] equals the exact directional Hessian along Qq, and twice it along
+    directions orthogonal to Qq (the kernel drops dR/dF)."""
+    rng = np.random.default_rng(7)
+    rest, _, dminv, V = random_tet(rng)
+    Q = random_rotation(rng)
+    x = rest @ Q.T + rng.normal(size=3)
+    mu, lam = 1e5, 4e5
+    eps = 1e-5
+    for k in range(4):
+        H, g = O.tet_block_corotated(x, dminv, mu, lam, V, k)
+        assert np.abs(g).max() < 1e-6 * V * mu
+        q = -dminv.sum(axis=0) if k == 0 else dminv[k - 1]
+        a = Q @ q / np.linalg.norm(q)
+        b = np.cross(a, rng.normal(size=3))
+        b /= np.linalg.norm(b)
+        for d, factor in ((a, 1.0), (b, 2.0)):
+            xp, xm = x.copy(), x.copy()
+            xp[k] += eps * d
+            xm[k] -= eps * d
+            dg = (O.tet_block_corotated(xp, dminv, mu, lam, V, k)[1] -
+                  O.tet_block_corotated(xm, dminv, mu, lam, V, k)[1]) / (2 * eps)
+            assert np.allclose(H @ d, factor * dg, rtol=1e-5, atol=1e-6 * np.abs(H).max())
+        assert np.all(np.linalg.eigvalsh(H) > 0)
+
+
+def test_corotated_block_is_rotation_equivariant():
+    rng = np.random.default_rng(11)
+    _, x, dminv, V = random_tet(rng)
+    Q = random_rotation(rng)
+    for k in range(4):
+        H, g = O.tet_block_corotated(x, dminv, 1e5, 4e5, V, k)
+        H2, g2 = O.tet_block_corotated(x @ Q.T, dminv, 1e5, 4e5, V, k)
+        assert np.allclose(g2, Q @ g, rtol=1e-9, atol=1e-9 * np.abs(g).max())
+        assert np.allclose(H2, Q @ H @ Q.T, rtol=1e-9, atol=1e-9 * np.abs(H).max())
+
+
+def test_corotated_simulation_settles_and_differs_from_neo_hookean():
+    """The material switch reaches the solve: a sagging beam settles under both models, to
+    different shapes."""
+    m = meshgen.beam(10, 2, 2)
+    prep = O.prepare(m)
+    outs = []
+    for material in (0, 1):
+        p = O.make_params(iterations=60, rho=0.95, material=material)
+        x, v, _, _ = O.simulate(prep, p, m.x0, None, 40, record=False)
+        assert np.all(np.isfinite(x))
+        outs.append(x)
+    sag = [m.x0[:, 1].min() - o[:, 1].min() for o in outs]
+    assert all(s > 0 for s in sag)
+    assert np.abs(outs[0] - outs[1]).max() > 1e-4 * np.abs(outs[0]).max()

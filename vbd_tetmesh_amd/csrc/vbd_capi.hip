@@ -29,6 +29,7 @@ struct vbd_solver {
     int lpv = 8;
     int unroll = 1;
     int block = 256;
+    int materialModel = VBD_MATERIAL_STABLE_NEO_HOOKEAN;
     bool graphEnabled = true;
     hipGraphExec_t graphExec = nullptr;
     bool graphValid = false;
@@ -125,6 +126,12 @@ bool finite_array(const float* a, size_t n) {
 // One color launch with the handle's configuration: lpv > 0 selects the per-vertex lane-group
 // kernel, lpv == 0 the tiled kernel on compact records (unroll = records per thread).
 hipError_t launch_color_cfg(vbd_solver* h, int c, float sdt, float mu, float lambda, const ContactArgs& ct, f4* save) {
+    if (h->materialModel == VBD_MATERIAL_COROTATED_GPU) {
+        // per-vertex lane groups on the full records; the configured lanes when 8/16/32
+        const bool own = h->lpv == 8 || h->lpv == 16 || h->lpv == 32;
+        return launch_color_corotated(h->m, c, own ? h->lpv : 16, own ? h->block : 256, sdt, mu, lambda, ct, save,
+                                      h->stream);
+    }
     if (h->lpv == 0) return launch_color_tiled(h->m, c, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
     return launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
 }
@@ -214,7 +221,7 @@ int download_vertex_array(vbd_solver* h, const f4* src, float* dst) {
 
 // Valence-histogram key of the autotuner cache: free-vertex count bucket (log2), mean
 // valence, and the share of vertices above 16 tets, quantised.
-std::string tune_key(const DevMesh& m, bool contact) {
+std::string tune_key(const DevMesh& m, bool contact, int material) {
     double sum = 0, n = 0, big = 0;
     for (size_t d = 0; d < m.valenceHist.size(); ++d) {
         sum += (double)d * m.valenceHist[d];
@@ -223,9 +230,9 @@ std::string tune_key(const DevMesh& m, bool contact) {
     }
     const double mean = n > 0 ? sum / n : 0;
     char buf[128];
-    snprintf(buf, sizeof(buf), "n%d_c%d_mean%d_big%d_mat%d_ct%d", (int)std::log2(std::max(1.0, n)),
+    snprintf(buf, sizeof(buf), "n%d_c%d_mean%d_big%d_mat%d_ct%d_m%d", (int)std::log2(std::max(1.0, n)),
              m.nColors, (int)std::lround(mean * 2), (int)std::lround(n > 0 ? 10 * big / n : 0), (int)m.perTetMat,
-             (int)contact);
+             (int)contact, material);
     return buf;
 }
 
@@ -774,6 +781,16 @@ int vbd_synchronize(vbd_solver* h) {
     return VBD_OK;
 }
 
+int vbd_set_material_model(vbd_solver* h, int32_t model) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (model != VBD_MATERIAL_STABLE_NEO_HOOKEAN && model != VBD_MATERIAL_COROTATED_GPU)
+        return fail(h, VBD_ERR_ARG, "unknown material model");
+    h->materialModel = model;
+    DeviceGuard g(h->device);
+    drop_graph(h);
+    return VBD_OK;
+}
+
 int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane) {
     if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
     if (lanes_per_vertex == 0) {
@@ -873,7 +890,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     if (rc) return rc;
     if (iterations <= 0) return fail(h, VBD_ERR_ARG, "iterations must be positive");
     DeviceGuard g(h->device);
-    const std::string key = tune_key(h->m, h->paramsSet && h->params.contact_on);
+    const std::string key = tune_key(h->m, h->paramsSet && h->params.contact_on, h->materialModel);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -910,6 +927,9 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
         for (int b : cand_b) {
             if (e != hipSuccess) break;
             if (cd.l > 0 && (b > 256 || b % cd.l)) continue;
+            // the corotated model runs the per-vertex kernel with lanes 8/16/32 only
+            if (h->materialModel == VBD_MATERIAL_COROTATED_GPU && (cd.u != 1 || (cd.l != 8 && cd.l != 16 && cd.l != 32)))
+                continue;
             if (cd.l == 0) {
                 if (b < 128 || !tiled_config_supported(cd.u, b) || !h->m.tiledOk || h->m.maxValence > cd.u * b)
                     continue;
@@ -1003,7 +1023,8 @@ int vbd_incremental_potential(vbd_solver* h, double* potential, double* elastic)
     const float hstep = p.dt / (float)p.substeps;
     double out[2] = {0.0, 0.0};
     HIP_CHECK(h, ensure_energy_scratch(h));
-    HIP_CHECK(h, launch_energy(h->m, 1, p.mu, p.lambda, h->d_epart, h->d_energy, h->stream));
+    HIP_CHECK(h, launch_energy(h->m, h->materialModel == VBD_MATERIAL_COROTATED_GPU ? 2 : 1, p.mu, p.lambda,
+                               h->d_epart, h->d_energy, h->stream));
     HIP_CHECK(h, launch_inertia_energy(h->m, 1.0 / ((double)hstep * hstep), contact_of(p), h->d_epart,
                                        h->d_energy + 1, h->stream));
     HIP_CHECK(h, hipMemcpyAsync(out, h->d_energy, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
@@ -1095,6 +1116,8 @@ int vbd_set_materials(vbd_solver* h, const float* tet_mu, const float* tet_lambd
 int vbd_residual_norm(vbd_solver* h, double* norm, double* gravity_load) {
     int rc = check_ready(h);
     if (rc) return rc;
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     DeviceGuard g(h->device);
     HIP_CHECK(h, ensure_scratch(h));
@@ -1111,6 +1134,8 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
                        int32_t stall_checks, int32_t* iterations, double* final_norm) {
     int rc = check_ready(h);
     if (rc) return rc;
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
     if (h->params.substeps != 1) return fail(h, VBD_ERR_ARG, "converged steps support substeps == 1");
@@ -1164,6 +1189,8 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
 int vbd_hessian_vector(vbd_solver* h, const float* w, float* out) {
     int rc = check_ready(h);
     if (rc) return rc;
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if ((!w || !out) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
     DeviceGuard g(h->device);
@@ -1180,6 +1207,8 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
                       float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual) {
     int rc = check_ready(h);
     if (rc) return rc;
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if ((!w || !rhs) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
     if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
@@ -1240,6 +1269,8 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
 int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dlambda) {
     int rc = check_ready(h);
     if (rc) return rc;
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if ((!w || !dmu || !dlambda) && h->m.nT) return fail(h, VBD_ERR_ARG, "null argument");
     DeviceGuard g(h->device);
     HIP_CHECK(h, ensure_scratch(h));

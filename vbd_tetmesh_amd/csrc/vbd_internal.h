@@ -84,6 +84,9 @@ struct ContactArgs {
 bool launch_config_supported(int lpv, int unroll, int block);
 hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int block, float sdt, float mu, float lambda,
                         const ContactArgs& ct, f4* save, hipStream_t s);
+// corotated material of the reference GPU kernel (per-vertex lane groups, lpv 8/16/32)
+hipError_t launch_color_corotated(const DevMesh& m, int color, int lpv, int block, float sdt, float mu, float lambda,
+                                  const ContactArgs& ct, f4* save, hipStream_t s);
 // tiled variant: per_thread records per thread, tiles of per_thread*block records and fewer than
 // block/8 vertices (build_tiles first)
 bool tiled_config_supported(int per_thread, int block);
@@ -99,7 +102,8 @@ hipError_t launch_copy_inertia(const DevMesh& m, hipStream_t s);
 hipError_t launch_scatter_pos(int n, const float* src3, const int* inv, f4* dst, hipStream_t s);
 hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, hipStream_t s);
 // energies in double, fixed summation order; partial holds energy_partials(m) doubles
-// mode 0: the reference's log-NH diagnostic, 1: the stable NH (J-1) elastic energy
+// mode 0: the reference's log-NH diagnostic, 1: the stable NH (J-1) elastic energy,
+// 2: the corotated model's energy
 int energy_partials(const DevMesh& m);
 hipError_t launch_energy(const DevMesh& m, int mode, float mu, float lambda, double* partial, double* out,
                          hipStream_t s);

@@ -698,6 +698,170 @@ hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int bl
 }
 
 // ---------------------------------------------------------------------------------------
+// Corotated linear elasticity of the reference GPU kernel (VBD_GPU_Wrapper.cu:127-233 with
+// VBD_DeviceMath.cuh:127-198), on the full records:
+//   R = 5 steps of R <- (R + cof(R) / det R) / 2 from R = F (stop if |det R| < 1e-8),
+//   P = 2 mu (F - R) + lambda tr(R^T F - I) R,   g = V P q,
+//   H = V [2 mu |q|^2 I + lambda (R q)(R q)^T]   (dR/dF dropped),
+// and the wrapper's vertex update: H diagonal m/h^2 * 1.000001, force (m/h^2)(y - x)
+// (VBD_GPU_Wrapper.cu:203-213), adjugate solve skipped when |det H| < 1e-8.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void corot_accumulate(const bool corner0, const f4 a, const f4 b, const float c,
+                                                 const float V, const float tmu, const float tlam, const f4 xs,
+                                                 const f4 ya, const f4 yb, const f4 yc, Acc& A) {
+    const Kin k = pair_kinematics(corner0, a, b, c, xs, ya, yb, yc);
+    // the first Higham step reuses cof(F)
+    float R00 = k.F00, R01 = k.F01, R02 = k.F02, R10 = k.F10, R11 = k.F11, R12 = k.F12;
+    float R20 = k.F20, R21 = k.F21, R22 = k.F22;
+    float C00 = k.C00, C01 = k.C01, C02 = k.C02, C10 = k.C10, C11 = k.C11, C12 = k.C12;
+    float C20 = k.C20, C21 = k.C21, C22 = k.C22;
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        if (it > 0) {
+            C00 = R11 * R22 - R12 * R21; C01 = R12 * R20 - R10 * R22; C02 = R10 * R21 - R11 * R20;
+            C10 = R21 * R02 - R22 * R01; C11 = R22 * R00 - R20 * R02; C12 = R20 * R01 - R21 * R00;
+            C20 = R01 * R12 - R02 * R11; C21 = R02 * R10 - R00 * R12; C22 = R00 * R11 - R01 * R10;
+        }
+        const float det = R00 * (R11 * R22 - R12 * R21) - R01 * (R10 * R22 - R12 * R20) + R02 * (R10 * R21 - R11 * R20);
+        if (fabsf(det) < 1e-8f) break;
+        const float inv = 1.0f / det;
+        R00 = 0.5f * (R00 + C00 * inv); R01 = 0.5f * (R01 + C01 * inv); R02 = 0.5f * (R02 + C02 * inv);
+        R10 = 0.5f * (R10 + C10 * inv); R11 = 0.5f * (R11 + C11 * inv); R12 = 0.5f * (R12 + C12 * inv);
+        R20 = 0.5f * (R20 + C20 * inv); R21 = 0.5f * (R21 + C21 * inv); R22 = 0.5f * (R22 + C22 * inv);
+    }
+    const float tr = R00 * k.F00 + R01 * k.F01 + R02 * k.F02 + R10 * k.F10 + R11 * k.F11 + R12 * k.F12 +
+                     R20 * k.F20 + R21 * k.F21 + R22 * k.F22 - 3.0f;
+    const float m2 = 2.0f * tmu, lt = tlam * tr;
+    const float q0 = k.q0, q1 = k.q1, q2 = k.q2;
+    const float P00 = m2 * (k.F00 - R00) + lt * R00, P01 = m2 * (k.F01 - R01) + lt * R01;
+    const float P02 = m2 * (k.F02 - R02) + lt * R02, P10 = m2 * (k.F10 - R10) + lt * R10;
+    const float P11 = m2 * (k.F11 - R11) + lt * R11, P12 = m2 * (k.F12 - R12) + lt * R12;
+    const float P20 = m2 * (k.F20 - R20) + lt * R20, P21 = m2 * (k.F21 - R21) + lt * R21;
+    const float P22 = m2 * (k.F22 - R22) + lt * R22;
+    const float rq0 = R00 * q0 + R01 * q1 + R02 * q2;
+    const float rq1 = R10 * q0 + R11 * q1 + R12 * q2;
+    const float rq2 = R20 * q0 + R21 * q1 + R22 * q2;
+    const float d = m2 * (q0 * q0 + q1 * q1 + q2 * q2);
+    A.h00 += V * (d + tlam * rq0 * rq0);
+    A.h01 += V * (tlam * rq0 * rq1);
+    A.h02 += V * (tlam * rq0 * rq2);
+    A.h11 += V * (d + tlam * rq1 * rq1);
+    A.h12 += V * (tlam * rq1 * rq2);
+    A.h22 += V * (d + tlam * rq2 * rq2);
+    A.r0 -= V * (P00 * q0 + P01 * q1 + P02 * q2);
+    A.r1 -= V * (P10 * q0 + P11 * q1 + P12 * q2);
+    A.r2 -= V * (P20 * q0 + P21 * q1 + P22 * q2);
+}
+
+template <int LPV, bool MAT, bool CONTACT, bool SAVE>
+__global__ __launch_bounds__(256) void k_color_corot(int slot0, int count, int nslots, f4* __restrict__ pos,
+                                                     const f4* __restrict__ inert, const float* __restrict__ mass,
+                                                     const int* __restrict__ row, const i4* __restrict__ ptet,
+                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
+                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
+                                                     ContactArgs ct) {
+    const int gpb = blockDim.x / LPV;
+    const int g = xcd_block() * gpb + (int)threadIdx.x / LPV;
+    const int lane = (int)threadIdx.x % LPV;
+    const bool valid = g < count;
+    const int s = slot0 + g;
+    Acc A = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    f4 xs = {0.f, 0.f, 0.f, 0.f};
+    if (valid) {
+        const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, nslots);
+        xs = pos[s];
+        const int p1 = row[s + 1];
+        for (int p = row[s] + lane; p < p1; p += LPV) {
+            const i4 t = __builtin_nontemporal_load(&ptet[p]);
+            const f4 a = __builtin_nontemporal_load(&pa[p]);
+            const f4 b = __builtin_nontemporal_load(&pb[p]);
+            const float c = __builtin_nontemporal_load(&pc[p]);
+            float tmu = mu, tlam = lambda;
+            if constexpr (MAT) {
+                const f2 mm = __builtin_nontemporal_load(&pmat[p]);
+                tmu = mm.x;
+                tlam = mm.y;
+            }
+            const f4 ya = gather_pos(prs, t.x & SLOT_MASK), yb = gather_pos(prs, t.y & SLOT_MASK);
+            const f4 yc = gather_pos(prs, t.z & SLOT_MASK);
+            corot_accumulate(t.x < 0, a, b, c, __int_as_float(t.w), tmu, tlam, xs, ya, yb, yc, A);
+        }
+    }
+    float h00 = gsum<LPV>(A.h00), h01 = gsum<LPV>(A.h01), h02 = gsum<LPV>(A.h02);
+    float h11 = gsum<LPV>(A.h11), h12 = gsum<LPV>(A.h12), h22 = gsum<LPV>(A.h22);
+    float r0 = gsum<LPV>(A.r0), r1 = gsum<LPV>(A.r1), r2 = gsum<LPV>(A.r2);
+    if (!valid || lane != LPV - 1) return;
+    const float coeff = mass[s] * sdt;
+    const f4 ys = inert[s];
+    const float reg = coeff * 1.000001f;
+    h00 += reg; h11 += reg; h22 += reg;
+    r0 += coeff * (ys.x - xs.x);
+    r1 += coeff * (ys.y - xs.y);
+    r2 += coeff * (ys.z - xs.z);
+    if constexpr (CONTACT) {
+        const float xa = ct.axis == 0 ? xs.x : (ct.axis == 1 ? xs.y : xs.z);
+        const float dd = ct.plane - xa;
+        if (dd > 0.f) {
+            if (ct.axis == 0) { h00 += ct.k; r0 += ct.k * dd; }
+            else if (ct.axis == 1) { h11 += ct.k; r1 += ct.k * dd; }
+            else { h22 += ct.k; r2 += ct.k * dd; }
+        }
+    }
+    const float det = h00 * (h11 * h22 - h12 * h12) - h01 * (h01 * h22 - h12 * h02) + h02 * (h01 * h12 - h11 * h02);
+    f4 xn = xs;
+    if (!(fabsf(det) < 1e-8f)) {
+        const float inv = 1.0f / det;
+        const float a00 = h11 * h22 - h12 * h12, a01 = h02 * h12 - h01 * h22, a02 = h01 * h12 - h02 * h11;
+        const float a11 = h00 * h22 - h02 * h02, a12 = h02 * h01 - h00 * h12, a22 = h00 * h11 - h01 * h01;
+        xn.x = xs.x + inv * (a00 * r0 + a01 * r1 + a02 * r2);
+        xn.y = xs.y + inv * (a01 * r0 + a11 * r1 + a12 * r2);
+        xn.z = xs.z + inv * (a02 * r0 + a12 * r1 + a22 * r2);
+    }
+    if constexpr (SAVE) save[s] = xs;
+    pos[s] = xn;
+}
+
+template <int LPV, bool MAT, bool CONTACT, bool SAVE>
+static hipError_t launch_corot_t(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+    const int slot0 = m.colorSlotOff[color];
+    const int count = m.colorSlotOff[color + 1] - slot0;
+    if (count <= 0) return hipSuccess;
+    const int gpb = block / LPV;
+    int nb = (count + gpb - 1) / gpb;
+    nb = (nb + 7) & ~7;
+    hipLaunchKernelGGL((k_color_corot<LPV, MAT, CONTACT, SAVE>), dim3(nb), dim3(block), 0, s, slot0, count, m.nV,
+                       m.pos, m.inert, m.mass, m.row, m.ptet, m.pa, m.pb, m.pc, m.pmat, save, mu, lambda, sdt, ct);
+    return hipGetLastError();
+}
+
+template <int LPV>
+static hipError_t launch_corot_l(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
+                                 const ContactArgs& ct, f4* save, hipStream_t s) {
+    const bool mat = m.perTetMat, con = ct.on != 0, sv = save != nullptr;
+#define VBD_DISPATCH(MT, CT, SV) \
+    if (mat == MT && con == CT && sv == SV) return launch_corot_t<LPV, MT, CT, SV>(m, color, block, sdt, mu, lambda, ct, save, s);
+    VBD_DISPATCH(false, false, false) VBD_DISPATCH(false, false, true)
+    VBD_DISPATCH(false, true, false) VBD_DISPATCH(false, true, true)
+    VBD_DISPATCH(true, false, false) VBD_DISPATCH(true, false, true)
+    VBD_DISPATCH(true, true, false) VBD_DISPATCH(true, true, true)
+#undef VBD_DISPATCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_color_corotated(const DevMesh& m, int color, int lpv, int block, float sdt, float mu, float lambda,
+                                  const ContactArgs& ct, f4* save, hipStream_t s) {
+    if (block < 64 || block > 256 || (block % 64) != 0) return hipErrorInvalidValue;
+    switch (lpv) {
+        case 8: return launch_corot_l<8>(m, color, block, sdt, mu, lambda, ct, save, s);
+        case 16: return launch_corot_l<16>(m, color, block, sdt, mu, lambda, ct, save, s);
+        case 32: return launch_corot_l<32>(m, color, block, sdt, mu, lambda, ct, save, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Differentiation kernels.
 //
 // With A = M/h^2 + K(x) (K = Hessian of E, plus the penalty-contact term k e_a e_a^T where a
@@ -1224,9 +1388,11 @@ hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, 
 //                     product of the non-negative singular values);
 //   ENERGY_STABLE_NH: the elastic part of the incremental potential the solver minimises,
 //                     V0 [mu/2 (|F|^2 - 3) - mu (J - 1) + lambda/2 (J - 1)^2] (Energy.cpp:93-97);
+//   ENERGY_COROTATED: V0 [mu |F - R|^2 + lambda/2 tr(R^T F - I)^2], R from the corotated
+//                     kernel's 5 polar steps, whose gradient is that kernel's P;
 // and the inertia part sum over free vertices of m/(2h^2) |x - y|^2 (+ penalty k/2 d^2).
 // ---------------------------------------------------------------------------------------
-constexpr int ENERGY_LOG_NH = 0, ENERGY_STABLE_NH = 1;
+constexpr int ENERGY_LOG_NH = 0, ENERGY_STABLE_NH = 1, ENERGY_COROTATED = 2;
 
 __device__ __forceinline__ double block_sum_256(double v, double* red) {
     red[threadIdx.x] = v;
@@ -1269,6 +1435,29 @@ __global__ __launch_bounds__(256) void k_tet_energy(int nT, const i4* __restrict
         if constexpr (MODE == ENERGY_LOG_NH) {
             const double lj = log(fabs(det));
             e = (0.5 * m_ * (tr - 3.0) - m_ * lj + 0.5 * l_ * lj * lj) * vol[t];
+        } else if constexpr (MODE == ENERGY_COROTATED) {
+            double R[3][3], C[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) R[r][c] = F[r][c];
+            for (int it = 0; it < 5; ++it) {
+                const double d = R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1]) -
+                                 R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0]) +
+                                 R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]);
+                if (fabs(d) < 1e-8) break;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c)
+                        C[r][c] = R[(r + 1) % 3][(c + 1) % 3] * R[(r + 2) % 3][(c + 2) % 3] -
+                                  R[(r + 1) % 3][(c + 2) % 3] * R[(r + 2) % 3][(c + 1) % 3];
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) R[r][c] = 0.5 * (R[r][c] + C[r][c] / d);
+            }
+            double fr = 0.0, rf = 0.0;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    fr += (F[r][c] - R[r][c]) * (F[r][c] - R[r][c]);
+                    rf += R[r][c] * F[r][c];
+                }
+            e = (m_ * fr + 0.5 * l_ * (rf - 3.0) * (rf - 3.0)) * vol[t];
         } else {
             e = (0.5 * m_ * (tr - 3.0) - m_ * (det - 1.0) + 0.5 * l_ * (det - 1.0) * (det - 1.0)) * vol[t];
         }
@@ -1305,6 +1494,9 @@ hipError_t launch_energy(const DevMesh& m, int mode, float mu, float lambda, dou
     if (nb == 0) return launch_sum_fixed(0, partial, out, s);
     if (mode == ENERGY_LOG_NH)
         hipLaunchKernelGGL(k_tet_energy<ENERGY_LOG_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
+                           m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
+    else if (mode == ENERGY_COROTATED)
+        hipLaunchKernelGGL(k_tet_energy<ENERGY_COROTATED>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
                            m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
     else
         hipLaunchKernelGGL(k_tet_energy<ENERGY_STABLE_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,

@@ -44,6 +44,7 @@ class VBDParams:
     velocityDamping: float = 0.999   # TetMesh.h:308 hard-codes 0.999
     startFromInertia: bool = False   # True reproduces VBD_GPU_Wrapper.cu:334 (x <- y)
     contact: Optional[tuple] = None  # (axis, plane, stiffness) penalty half-space x[axis] >= plane
+    material: int = A.VBD_MATERIAL_STABLE_NEO_HOOKEAN  # or A.VBD_MATERIAL_COROTATED_GPU
 
     def to_c(self) -> A.StepParams:
         p = A.StepParams()
@@ -315,6 +316,12 @@ class DeviceMesh(_Handle):
     def set_params(self, p: VBDParams):
         cp = p.to_c()
         check(lib.vbd_set_params(self.h, C.byref(cp)), self.h)
+        self.set_material_model(p.material)
+
+    def set_material_model(self, model: int):
+        """VBD_MATERIAL_STABLE_NEO_HOOKEAN (the CPU solver's material) or VBD_MATERIAL_COROTATED_GPU
+        (the reference GPU kernel's corotated model and regularised inertia term)."""
+        check(lib.vbd_set_material_model(self.h, int(model)), self.h)
 
     def step(self, frames: int = 1):
         check(lib.vbd_step(self.h, frames), self.h)
@@ -412,8 +419,11 @@ class DeviceMesh(_Handle):
 class VBD_GPU_Solver:
     """VBD_GPU_Solver (VBD_GPU_Wrapper.h:7-45) on the gfx950 C ABI."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, material: int = A.VBD_MATERIAL_STABLE_NEO_HOOKEAN):
+        """material=A.VBD_MATERIAL_COROTATED_GPU runs the reference GPU kernel's corotated model
+        (VBD_GPU_Wrapper.cu:181); the default is the CPU solver's stable Neo-Hookean."""
         self._dev = DeviceMesh(device)
+        self._material = material
 
     @property
     def device_mesh(self) -> DeviceMesh:
@@ -425,6 +435,7 @@ class VBD_GPU_Solver:
         off, verts = _colors_to_csr(vertexColors)
         self._dev.init(mesh.mVertPos, mesh.vertexMass, mesh.isFixed, mesh.tets, tetDmInv, tetRestVol, off, verts,
                        inertia=mesh.mInertia)
+        self._dev.set_material_model(self._material)
 
     def solveIteration(self, dtSqrReciprocal, mu, lambda_):
         """VBD_GPU_Wrapper.cu:317-329: one sweep over all colors."""
