@@ -9,7 +9,9 @@ v0, 32 seeded parameter indices and their central finite differences (float64 fo
 warm-started from the base trajectory; step 1e-3 of the parameter for mu and lambda, 1e-3 of
 max |v0| for v0 components, some of which are near zero).
 
-usage: python tests/golden/make_diffcube_golden.py [n] [steps] [out.npz]
+usage: python tests/golden/make_diffcube_golden.py [n] [steps] [out.npz] [--only mu,lam,v0]
+  --only recomputes the finite differences of the listed parameter families and keeps the
+  others from an existing out.npz (same n, steps and seeds).
 """
 import os
 import sys
@@ -25,9 +27,20 @@ from vbd_tetmesh_amd import meshgen  # noqa: E402
 
 
 def main():
-    n = int(sys.argv[1]) if len(sys.argv) > 1 else 16
-    steps = int(sys.argv[2]) if len(sys.argv) > 2 else 30
-    out = sys.argv[3] if len(sys.argv) > 3 else os.path.join(ROOT, "tests", "golden", f"diffcube{n}_fp64.npz")
+    argv = list(sys.argv[1:])
+    only = None
+    if "--only" in argv:
+        i = argv.index("--only")
+        only = set(argv[i + 1].split(","))
+        del argv[i:i + 2]
+    n = int(argv[0]) if len(argv) > 0 else 16
+    steps = int(argv[1]) if len(argv) > 1 else 30
+    out = argv[2] if len(argv) > 2 else os.path.join(ROOT, "tests", "golden", f"diffcube{n}_fp64.npz")
+    old_fd = None
+    if only is not None:
+        old = np.load(out)
+        assert int(old["n"]) == n and int(old["steps"]) == steps, "existing fixture has other sizes"
+        old_fd = list(old["fd"])
     m, mu_true, lam_true, v0 = meshgen.diff_workload(n, seed=0)
     P = D.make_problem(m)
     mu = m.tet_mu.astype(np.float64)
@@ -54,8 +67,13 @@ def main():
         d = xs_[-1] - x_target
         return float(np.sum(d * d))
 
+    j = -1
     for kind, ids in (("mu", idx_mu), ("lam", idx_lam), ("v0", idx_v)):
         for i in ids:
+            j += 1
+            if only is not None and kind not in only:
+                fd.append(old_fd[j])
+                continue
             mp, lp, vp = mu.copy(), lam.copy(), v0d.copy()
             mm, lm, vm = mu.copy(), lam.copy(), v0d.copy()
             if kind == "mu":

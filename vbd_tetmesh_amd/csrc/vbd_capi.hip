@@ -31,6 +31,9 @@ struct vbd_solver {
     int block = 256;
     int materialModel = VBD_MATERIAL_STABLE_NEO_HOOKEAN;
     bool graphEnabled = true;
+    // graphs of the converged-step and adjoint-solve iteration blocks, keyed by their content
+    std::map<std::string, hipGraphExec_t> blockGraphs;
+    double* h_res = nullptr;  // pinned host copy of a block's residual
     hipGraphExec_t graphExec = nullptr;
     bool graphValid = false;
     double* d_energy = nullptr;   // 2 doubles: energy results
@@ -83,9 +86,14 @@ void drop_graph(vbd_solver* h) {
     if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
     h->graphExec = nullptr;
     h->graphValid = false;
+    for (auto& kv : h->blockGraphs) (void)hipGraphExecDestroy(kv.second);
+    h->blockGraphs.clear();
 }
 
 void free_scratch(vbd_solver* h) {
+    drop_graph(h);  // block graphs reference the scratch buffers
+    if (h->h_res) (void)hipHostFree(h->h_res);
+    h->h_res = nullptr;
     void* ps[] = {h->d_w,  h->d_rhs, h->d_out,   h->d_wsave[0], h->d_wsave[1],
                   h->d_r2, h->d_sum, h->d_tgrad, h->d_x3,       h->d_epart};
     for (void* p : ps)
@@ -1052,6 +1060,7 @@ hipError_t ensure_scratch(vbd_solver* h) {
     if (!h->d_sum && (e = hipMalloc(&h->d_sum, sizeof(double) * 2)) != hipSuccess) return e;
     if (!h->d_tgrad && (e = hipMalloc(&h->d_tgrad, sizeof(float) * 2 * t)) != hipSuccess) return e;
     if (!h->d_x3 && (e = hipMalloc(&h->d_x3, sizeof(float) * 3 * n)) != hipSuccess) return e;
+    if (!h->h_res && (e = hipHostMalloc(&h->h_res, sizeof(double) * 2, hipHostMallocDefault)) != hipSuccess) return e;
     return e;
 }
 
@@ -1093,6 +1102,59 @@ hipError_t forward_residual(vbd_solver* h, double* norm2) {
     if (e == hipSuccess) e = sum_r2(h, h->m.nFree, norm2);
     return e;
 }
+
+// Enqueues |r|^2 of the per-vertex residual already in d_r2 into the pinned h_res[0] (no wait).
+hipError_t enqueue_sum_to_host(vbd_solver* h) {
+    hipError_t e = launch_sum_fixed(h->m.nFree, h->d_r2, h->d_sum, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->h_res, h->d_sum, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    return e;
+}
+
+extern "C++" {
+// Runs the work `enqueue` puts on the handle's stream, replaying a graph captured on first use
+// under `key` (graphs enabled) or launching it directly, then waits; the same kernels run either
+// way, so results are identical.  The key must name everything the work bakes into arguments
+// that can change between calls (the buffers are fixed per handle; set_params, launch
+// configuration, material changes and re-initialisation drop the cache).
+template <class Fn>
+hipError_t run_block(vbd_solver* h, const std::string& key, Fn&& enqueue) {
+    hipError_t e = hipSuccess;
+    if (!h->graphEnabled) {
+        e = enqueue();
+    } else {
+        auto it = h->blockGraphs.find(key);
+        if (it == h->blockGraphs.end()) {
+            hipGraph_t graph = nullptr;
+            e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return e;
+            e = enqueue();
+            hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+            if (e == hipSuccess) e = e2;
+            hipGraphExec_t exec = nullptr;
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return e;
+            it = h->blockGraphs.emplace(key, exec).first;
+        }
+        e = hipGraphLaunch(it->second, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e;
+}
+
+// Key part for a block of n iterations starting at iteration it0 with the given omegas.
+std::string block_key(char kind, int it0, const std::vector<float>& omegas, float extra) {
+    std::string k(1, kind);
+    char buf[32];
+    snprintf(buf, sizeof(buf), "%d:%d:%08x:", (int)omegas.size(), it0 & 1, (unsigned)__builtin_bit_cast(uint32_t, extra));
+    k += buf;
+    for (float w : omegas) {
+        snprintf(buf, sizeof(buf), "%08x", (unsigned)__builtin_bit_cast(uint32_t, w));
+        k += buf;
+    }
+    return k;
+}
+}  // extern "C++"
 
 }  // namespace
 
@@ -1154,31 +1216,37 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
     float omega = 1.0f;
     int it = 0, stalled = 0;
     double r2 = 0, best = 1e300;
-    bool checked_last = false;
-    for (it = 0; it < max_iterations;) {
-        f4* save = p.use_acceleration ? h->m.save[it & 1] : nullptr;
-        HIP_CHECK(h, enqueue_sweep(h, sdt, p.mu, p.lambda, ct, save));
-        if (p.use_acceleration) {
-            omega = omega_ref(it + 1, p.acceleration_rho, omega);
-            if (omega > 1.0f) HIP_CHECK(h, launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream));
-        }
-        ++it;
-        checked_last = false;
-        if (it % check_every == 0 || it == max_iterations) {
-            HIP_CHECK(h, forward_residual(h, &r2));
-            checked_last = true;
-            const double r = std::sqrt(r2);
-            if (r <= tol) break;
-            // stagnation at the float32 floor: no 1% gain on the best residual for stall_checks checks
-            if (r < 0.99 * best) {
-                best = r;
-                stalled = 0;
-            } else if (stall_checks > 0 && ++stalled >= stall_checks) {
-                break;
+    const bool accel = p.use_acceleration != 0;
+    std::vector<float> om;
+    while (it < max_iterations) {
+        // one block: the iterations up to the next check, then the residual
+        const int n = std::min(check_every - it % check_every, max_iterations - it);
+        om.assign(n, 1.0f);
+        for (int j = 0; j < n && accel; ++j) om[j] = omega = omega_ref(it + j + 1, p.acceleration_rho, omega);
+        const int it0 = it;
+        HIP_CHECK(h, run_block(h, block_key('F', it0, om, 0.f), [&]() -> hipError_t {
+            for (int j = 0; j < n; ++j) {
+                const int k = it0 + j;
+                hipError_t e = enqueue_sweep(h, sdt, p.mu, p.lambda, ct, accel ? h->m.save[k & 1] : nullptr);
+                if (e == hipSuccess && accel && om[j] > 1.0f) e = launch_blend(h->m, om[j], h->m.save[(k + 1) & 1], h->stream);
+                if (e != hipSuccess) return e;
             }
+            hipError_t e = launch_pass(3, h->m, 0, h->m.nFree, pass_lpv(h), 256, nullptr, nullptr, h->d_out, h->d_r2,
+                                       nullptr, p.mu, p.lambda, sdt, ct, h->stream);
+            return e == hipSuccess ? enqueue_sum_to_host(h) : e;
+        }));
+        it += n;
+        r2 = h->h_res[0];
+        const double r = std::sqrt(r2);
+        if (r <= tol) break;
+        // stagnation at the float32 floor: no 1% gain on the best residual for stall_checks checks
+        if (r < 0.99 * best) {
+            best = r;
+            stalled = 0;
+        } else if (stall_checks > 0 && ++stalled >= stall_checks) {
+            break;
         }
     }
-    if (!checked_last) HIP_CHECK(h, forward_residual(h, &r2));
     HIP_CHECK(h, launch_velocity(h->m, hstep, p.velocity_damping, h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
     if (iterations) *iterations = it;
@@ -1226,39 +1294,42 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
     float omega = 1.0f;
     int it = 0, stalled = 0;
     double best = 1e300;
-    bool checked_last = false;
-    for (it = 0; it < max_iterations && bn > 0;) {
-        f4* save = rho > 0.f ? h->d_wsave[it & 1] : nullptr;
-        for (int c = 0; c < h->m.nColors; ++c) {
-            const int s0 = h->m.colorSlotOff[c], cnt = h->m.colorSlotOff[c + 1] - s0;
-            HIP_CHECK(h, launch_pass(0, h->m, s0, cnt, lpv, 256, h->d_w, h->d_rhs, nullptr, nullptr, save, mu, lam, sdt,
-                                     ct, h->stream));
-        }
-        if (rho > 0.f) {
-            omega = omega_ref(it + 1, rho, omega);
-            if (omega > 1.0f) HIP_CHECK(h, launch_blend_vec(h->m.nFree, h->d_w, h->d_wsave[(it + 1) & 1], omega, h->stream));
-        }
-        ++it;
-        checked_last = false;
-        if (it % check_every == 0 || it == max_iterations) {
-            HIP_CHECK(h, launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2, nullptr, mu,
-                                     lam, sdt, ct, h->stream));
-            HIP_CHECK(h, sum_r2(h, h->m.nFree, &r2));
-            checked_last = true;
-            const double r = std::sqrt(r2);
-            if (r <= rel_tol * bn) break;
-            if (r < 0.99 * best) {
-                best = r;
-                stalled = 0;
-            } else if (stall_checks > 0 && ++stalled >= stall_checks) {
-                break;
+    const bool accel = rho > 0.f;
+    std::vector<float> om;
+    while (it < max_iterations && bn > 0) {
+        const int n = std::min(check_every - it % check_every, max_iterations - it);
+        om.assign(n, 1.0f);
+        for (int j = 0; j < n && accel; ++j) om[j] = omega = omega_ref(it + j + 1, rho, omega);
+        const int it0 = it;
+        HIP_CHECK(h, run_block(h, block_key('A', it0, om, rho), [&]() -> hipError_t {
+            for (int j = 0; j < n; ++j) {
+                const int k = it0 + j;
+                f4* save = accel ? h->d_wsave[k & 1] : nullptr;
+                for (int c = 0; c < h->m.nColors; ++c) {
+                    const int s0 = h->m.colorSlotOff[c], cnt = h->m.colorSlotOff[c + 1] - s0;
+                    hipError_t e = launch_pass(0, h->m, s0, cnt, lpv, 256, h->d_w, h->d_rhs, nullptr, nullptr, save, mu,
+                                               lam, sdt, ct, h->stream);
+                    if (e != hipSuccess) return e;
+                }
+                if (accel && om[j] > 1.0f) {
+                    hipError_t e = launch_blend_vec(h->m.nFree, h->d_w, h->d_wsave[(k + 1) & 1], om[j], h->stream);
+                    if (e != hipSuccess) return e;
+                }
             }
+            hipError_t e = launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2, nullptr,
+                                       mu, lam, sdt, ct, h->stream);
+            return e == hipSuccess ? enqueue_sum_to_host(h) : e;
+        }));
+        it += n;
+        r2 = h->h_res[0];
+        const double r = std::sqrt(r2);
+        if (r <= rel_tol * bn) break;
+        if (r < 0.99 * best) {
+            best = r;
+            stalled = 0;
+        } else if (stall_checks > 0 && ++stalled >= stall_checks) {
+            break;
         }
-    }
-    if (!checked_last && bn > 0) {
-        HIP_CHECK(h, launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2, nullptr, mu, lam,
-                                 sdt, ct, h->stream));
-        HIP_CHECK(h, sum_r2(h, h->m.nFree, &r2));
     }
     HIP_CHECK(h, download_vec(h, h->d_w, w));
     if (iterations) *iterations = it;
