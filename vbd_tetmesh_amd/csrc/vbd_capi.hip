@@ -90,6 +90,50 @@ void drop_graph(vbd_solver* h) {
     h->blockGraphs.clear();
 }
 
+// Runs the work `enqueue` puts on the handle's stream, replaying a graph captured on first use
+// under `key` (graphs enabled) or launching it directly, then waits (if `wait`); the same kernels run either
+// way, so results are identical.  The key must name everything the work bakes into arguments
+// that can change between calls (the buffers are fixed per handle; set_params, launch
+// configuration, material changes and re-initialisation drop the cache).
+template <class Fn>
+hipError_t run_block(vbd_solver* h, const std::string& key, Fn&& enqueue, bool wait = true) {
+    hipError_t e = hipSuccess;
+    if (!h->graphEnabled) {
+        e = enqueue();
+    } else {
+        auto it = h->blockGraphs.find(key);
+        if (it == h->blockGraphs.end()) {
+            hipGraph_t graph = nullptr;
+            e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return e;
+            e = enqueue();
+            hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+            if (e == hipSuccess) e = e2;
+            hipGraphExec_t exec = nullptr;
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return e;
+            it = h->blockGraphs.emplace(key, exec).first;
+        }
+        e = hipGraphLaunch(it->second, h->stream);
+    }
+    if (e == hipSuccess && wait) e = hipStreamSynchronize(h->stream);
+    return e;
+}
+
+// Key part for a block of n iterations starting at iteration it0 with the given omegas.
+std::string block_key(char kind, int it0, const std::vector<float>& omegas, float extra) {
+    std::string k(1, kind);
+    char buf[32];
+    snprintf(buf, sizeof(buf), "%d:%d:%08x:", (int)omegas.size(), it0 & 1, (unsigned)__builtin_bit_cast(uint32_t, extra));
+    k += buf;
+    for (float w : omegas) {
+        snprintf(buf, sizeof(buf), "%08x", (unsigned)__builtin_bit_cast(uint32_t, w));
+        k += buf;
+    }
+    return k;
+}
+
 void free_scratch(vbd_solver* h) {
     drop_graph(h);  // block graphs reference the scratch buffers
     if (h->h_res) (void)hipHostFree(h->h_res);
@@ -707,8 +751,17 @@ int vbd_solve(vbd_solver* h, int32_t num_iterations, float dt_sqr_reciprocal, fl
     DeviceGuard g(h->device);
     ContactArgs ct{};
     if (h->paramsSet) ct = contact_of(h->params);
-    if (start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
-    HIP_CHECK(h, enqueue_iterations(h, num_iterations, dt_sqr_reciprocal, mu, lambda, use_acceleration != 0, rho, ct));
+    if (num_iterations == 0 && !start_from_inertia) return VBD_OK;
+    // the whole call is one cached graph (asynchronous, like the reference solveVBD)
+    const std::vector<float> args = {(float)num_iterations, dt_sqr_reciprocal, mu, lambda, (float)use_acceleration,
+                                     rho, (float)start_from_inertia};
+    HIP_CHECK(h, run_block(h, block_key('S', 0, args, 0.f), [&]() -> hipError_t {
+        if (start_from_inertia) {
+            hipError_t e = launch_copy_inertia(h->m, h->stream);
+            if (e != hipSuccess) return e;
+        }
+        return enqueue_iterations(h, num_iterations, dt_sqr_reciprocal, mu, lambda, use_acceleration != 0, rho, ct);
+    }, false));
     return VBD_OK;
 }
 
@@ -1110,51 +1163,6 @@ hipError_t enqueue_sum_to_host(vbd_solver* h) {
     return e;
 }
 
-extern "C++" {
-// Runs the work `enqueue` puts on the handle's stream, replaying a graph captured on first use
-// under `key` (graphs enabled) or launching it directly, then waits; the same kernels run either
-// way, so results are identical.  The key must name everything the work bakes into arguments
-// that can change between calls (the buffers are fixed per handle; set_params, launch
-// configuration, material changes and re-initialisation drop the cache).
-template <class Fn>
-hipError_t run_block(vbd_solver* h, const std::string& key, Fn&& enqueue) {
-    hipError_t e = hipSuccess;
-    if (!h->graphEnabled) {
-        e = enqueue();
-    } else {
-        auto it = h->blockGraphs.find(key);
-        if (it == h->blockGraphs.end()) {
-            hipGraph_t graph = nullptr;
-            e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-            if (e != hipSuccess) return e;
-            e = enqueue();
-            hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
-            if (e == hipSuccess) e = e2;
-            hipGraphExec_t exec = nullptr;
-            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (graph) (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return e;
-            it = h->blockGraphs.emplace(key, exec).first;
-        }
-        e = hipGraphLaunch(it->second, h->stream);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    return e;
-}
-
-// Key part for a block of n iterations starting at iteration it0 with the given omegas.
-std::string block_key(char kind, int it0, const std::vector<float>& omegas, float extra) {
-    std::string k(1, kind);
-    char buf[32];
-    snprintf(buf, sizeof(buf), "%d:%d:%08x:", (int)omegas.size(), it0 & 1, (unsigned)__builtin_bit_cast(uint32_t, extra));
-    k += buf;
-    for (float w : omegas) {
-        snprintf(buf, sizeof(buf), "%08x", (unsigned)__builtin_bit_cast(uint32_t, w));
-        k += buf;
-    }
-    return k;
-}
-}  // extern "C++"
 
 }  // namespace
 
