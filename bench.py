@@ -89,12 +89,20 @@ def cpu_baseline(m, target_seconds: float = 10.0):
         sw += s
         cw += c
     sweeps = reps + 1
+    # the reference's own scheme, OpenMP over colors (racy; at most one thread per color)
+    pr = O.make_params(dt=H, iterations=1, rho=0.95, mu=m.mu, lam=m.lam, gravity=m.gravity,
+                       mode=O.MODE_FAITHFUL_COST, threads=cores)
+    _, sr = O.sweep_over_colors(prep, pr, m.x0, y)
+    nfree = int((m.fixed == 0).sum())
     return {"value": cw / sw / 1e6, "unit": "M vertex-iters/s", "cores": cores, "kind": "port",
             "sample": f"{sweeps} whole sweeps of the same 128^3 cube ({cw} vertex solves in {sw:.1f} s), reference "
                       f"cost structure (full 12x12 element Hessian per (vertex, tet) call, per-call std::vector), "
                       f"{cores} threads inside each color; energy diagnostic excluded",
             "single_thread_value": c1 / s1 / 1e6,
-            "single_thread_sample": f"first {n1} free vertices of each color, 1 thread ({c1} solves in {s1:.1f} s)"}
+            "single_thread_sample": f"first {n1} free vertices of each color, 1 thread ({c1} solves in {s1:.1f} s)",
+            "reference_scheme_value": nfree / sr / 1e6,
+            "reference_scheme_sample": f"one whole sweep with OpenMP over the {prep.n_colors} colors as VBD.cpp:119-121 "
+                                       f"({cores} threads requested, racy; {nfree} solves in {sr:.1f} s)"}
 
 
 def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, iterations: int = 20):

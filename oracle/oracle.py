@@ -106,6 +106,9 @@ def lib():
         L.orc_solve_subset_f32.argtypes = [C.POINTER(Params), C.c_int, C.c_int, i32p, i32p, i32p, u8p] + \
             [f32p] * 5 + [f32p, f32p, C.c_int, i32p, f32p]
         L.orc_solve_subset_f32.restype = C.c_double
+        L.orc_sweep_over_colors_f32.argtypes = [C.POINTER(Params), C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, i32p,
+                                                i32p, u8p] + [f32p] * 5 + [f32p, f32p]
+        L.orc_sweep_over_colors_f32.restype = C.c_double
         L.orc_total_energy_f32.argtypes = [C.POINTER(Params), C.c_int, i32p, f32p, f32p, f32p, f32p, f32p]
         L.orc_total_energy_f32.restype = C.c_double
         assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
@@ -250,6 +253,23 @@ def solve_subset(prep: Prepared, params: Params, x: np.ndarray, y: np.ndarray, v
                                   _p(tmu, C.c_float), _p(tlam, C.c_float), _p(x, C.c_float), _p(y, C.c_float),
                                   verts.size, _p(verts, C.c_int32), _p(out, C.c_float))
     return out, secs
+
+
+def sweep_over_colors(prep: Prepared, params: Params, x: np.ndarray, y: np.ndarray):
+    """One sweep in the reference's scheme (VBD.cpp:119-121: OpenMP over colors, params.threads
+    threads; racy, timing only).  Returns (positions after the sweep, seconds)."""
+    L = lib()
+    x = np.ascontiguousarray(x, np.float32).copy()
+    y = np.ascontiguousarray(y, np.float32)
+    tmu = None if prep.tet_mu is None else np.ascontiguousarray(prep.tet_mu, np.float32)
+    tlam = None if prep.tet_lambda is None else np.ascontiguousarray(prep.tet_lambda, np.float32)
+    secs = L.orc_sweep_over_colors_f32(C.byref(params), prep.nV, prep.nT, _p(prep.tets, C.c_int32),
+                                       _p(prep.adj_off, C.c_int32), _p(prep.adj, C.c_int32), prep.n_colors,
+                                       _p(prep.color_off, C.c_int32), _p(prep.color_verts, C.c_int32),
+                                       _p(prep.fixed, C.c_uint8), _p(prep.mass, C.c_float), _p(prep.dminv, C.c_float),
+                                       _p(prep.vol, C.c_float), _p(tmu, C.c_float), _p(tlam, C.c_float),
+                                       _p(x, C.c_float), _p(y, C.c_float))
+    return x, secs
 
 
 def tet_full(x12, dminv9, mu, lam, V, f64=True):

@@ -819,6 +819,27 @@ double orc_solve_subset_f32(const Params* p, int nV, int nT, const int32_t* tets
     return el;
 }
 
+// One sweep in the reference's own scheme (VBD.cpp:119-121): OpenMP over the colors, each color
+// swept by one thread while other colors run concurrently.  Vertices of different colors that
+// share a tet race, so the result is not deterministic; for timing only.  Returns seconds.
+double orc_sweep_over_colors_f32(const Params* p, int nV, int nT, const int32_t* tets, const int32_t* adjOff,
+                                 const int32_t* adj, int nColors, const int32_t* colorOff, const int32_t* colorVerts,
+                                 const uint8_t* fixed, const float* mass, const float* dminv, const float* vol,
+                                 const float* tetMu, const float* tetLambda, float* x, const float* y) {
+    Mesh<float> M = make_mesh<float>(nV, nT, tets, adjOff, adj, nColors, colorOff, colorVerts, fixed, mass, dminv, vol,
+                                     tetMu, tetLambda);
+    float h = p->dt / (float)p->substeps;
+    float s = inv_h2<float>(h);
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for num_threads(p->threads > 0 ? p->threads : 1) schedule(dynamic)
+    for (int c = 0; c < nColors; ++c)
+        for (int i = colorOff[c]; i < colorOff[c + 1]; ++i) {
+            int v = colorVerts[i];
+            if (!fixed[v]) solve_vertex(M, *p, x, y, v, s);
+        }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // Total diagnostic energy (VBD.cpp:258-288).
 double orc_total_energy_f32(const Params* p, int nT, const int32_t* tets, const float* dminv, const float* vol,
                             const float* tetMu, const float* tetLambda, const float* x) {

@@ -417,3 +417,21 @@ def test_corotated_simulation_settles_and_differs_from_neo_hookean():
     sag = [m.x0[:, 1].min() - o[:, 1].min() for o in outs]
     assert all(s > 0 for s in sag)
     assert np.abs(outs[0] - outs[1]).max() > 1e-4 * np.abs(outs[0]).max()
+
+
+def test_reference_over_colors_sweep_with_one_thread_is_the_sequential_sweep():
+    """The timing-only restatement of VBD.cpp:119-121 (OpenMP over colors) with one thread is the
+    ordinary colour-by-colour Gauss-Seidel sweep, bit for bit."""
+    m = meshgen.noisy_cube(6, seed=3)
+    prep = O.prepare(m)
+    y = m.x0.copy()
+    y[m.fixed == 0] += np.float32(1e-3)
+    p = O.make_params(iterations=1, mu=m.mu, lam=m.lam, threads=1)
+    got, _ = O.sweep_over_colors(prep, p, m.x0, y)
+    x = m.x0.copy()
+    for c in range(prep.n_colors):
+        vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
+        vs = vs[m.fixed[vs] == 0]
+        xo, _ = O.solve_subset(prep, p, x, y, vs)
+        x[vs] = xo
+    assert np.array_equal(got, x)
