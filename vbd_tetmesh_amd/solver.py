@@ -118,6 +118,33 @@ class TetMesh:
     def savePreviousPosition(self):
         self.mVertPrevPos[:] = self.mVertPos
 
+    # surface output, TetMesh.h:45-186
+    def extractSurfaceFaces(self) -> np.ndarray:
+        """Boundary triangles (faces used by exactly one tet), oriented as the tet's outward face
+        ((0,2,1), (0,1,3), (0,3,2), (1,2,3) of a positively oriented tet), ordered by sorted ids."""
+        if self.numTets == 0:
+            return np.zeros((0, 3), np.int32)
+        t = self.tets
+        tri = np.concatenate([t[:, [0, 2, 1]], t[:, [0, 1, 3]], t[:, [0, 3, 2]], t[:, [1, 2, 3]]])
+        key = np.sort(tri, axis=1)
+        _, first, counts = np.unique(key, axis=0, return_index=True, return_counts=True)
+        return np.ascontiguousarray(tri[first[counts == 1]], np.int32)
+
+    def exportToObj(self, filepath: str) -> bool:
+        """Wavefront OBJ of the current positions and the surface triangles (1-based)."""
+        try:
+            faces = self.extractSurfaceFaces()
+            with open(filepath, "w") as f:
+                f.write(f"# TetMesh OBJ Export\n# Vertices: {self.numVerts}\n# Tetrahedra: {self.numTets}\n\n")
+                for v in self.mVertPos:
+                    f.write(f"v {v[0]:g} {v[1]:g} {v[2]:g}\n")
+                f.write(f"\n# Surface Triangles: {len(faces)}\n")
+                for a, b, c in faces + 1:
+                    f.write(f"f {a} {b} {c}\n")
+        except OSError:
+            return False
+        return True
+
 
 def mesh_from_generator(m, mu=None, lam=None) -> TetMesh:
     """TetMesh from a vbd_tetmesh_amd.meshgen.Mesh (per-tet materials carried over)."""
