@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
     ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
+    ap.add_argument("--only", choices=["batched", "sphere", "adjoint"], default=None,
+                    help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -270,6 +272,12 @@ def main():
 
     from vbd_tetmesh_amd import VBDParams, meshgen
     from vbd_tetmesh_amd.solver import upload
+
+    if args.only:
+        fn = {"batched": bench_batch, "sphere": lambda: bench_sphere(slot_order=args.slot_order),
+              "adjoint": bench_adjoint}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
 
     t0 = time.time()
     m = meshgen.noisy_cube(args.n, seed=0)
