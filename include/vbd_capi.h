@@ -153,6 +153,11 @@ int vbd_update_velocity(vbd_solver* h);
  * replayed.  Asynchronous: returns after enqueueing; transfers or vbd_synchronize wait. */
 int vbd_step(vbd_solver* h, int32_t num_frames);
 int vbd_synchronize(vbd_solver* h);
+/* Enqueues all further work of the handle on `stream` (a hipStream_t of the handle's device,
+ * e.g. a framework's current stream); NULL returns to the handle's own non-blocking stream.
+ * Waits for the work already enqueued on the previous stream.  The caller keeps the stream
+ * alive while the handle uses it. */
+int vbd_set_stream(vbd_solver* h, void* stream);
 /* Enables (1, default) or disables (0) hipGraph capture of vbd_step. */
 int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
 

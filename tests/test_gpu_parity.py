@@ -147,6 +147,22 @@ def test_main_cpp_hybrid_loop_with_gpu_solver():
     assert rel <= TOL
 
 
+def test_caller_stream_gives_identical_results():
+    """vbd_set_stream: the same frames on a torch stream (graphs replayed there) and back on the
+    handle's own stream."""
+    import torch
+    m = meshgen.beam(12, 3, 3)
+    a, _ = run_gpu_frames(m, params_for(m, 20), 4)
+    d = upload(m, params=params_for(m, 20))
+    s = torch.cuda.Stream()
+    d.set_stream(s)
+    for _ in range(3):
+        d.step(1)
+    d.set_stream(None)
+    d.step(1)
+    assert np.array_equal(d.positions(), a[-1])
+
+
 def test_graph_replay_is_bitwise_identical_to_eager():
     m = meshgen.beam(12, 3, 3)
     a, _ = run_gpu_frames(m, params_for(m, 20), 6, graph=True)

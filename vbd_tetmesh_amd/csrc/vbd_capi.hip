@@ -20,7 +20,8 @@ using namespace vbd;
 
 struct vbd_solver {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // the stream all work is enqueued on
+    hipStream_t ownStream = nullptr;  // the handle's own non-blocking stream
     std::string err;
     bool inited = false;
     DevMesh m;
@@ -354,14 +355,15 @@ int vbd_create(vbd_solver** out, int32_t device) {
     if (!h) return fail(nullptr, VBD_ERR_ALLOC, "host allocation failed");
     h->device = device;
     DeviceGuard g(device);
-    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    e = hipStreamCreateWithFlags(&h->ownStream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete h;
         return hip_fail(nullptr, "hipStreamCreateWithFlags", e);
     }
+    h->stream = h->ownStream;
     e = hipMalloc(&h->d_energy, sizeof(double) * 2);
     if (e != hipSuccess) {
-        (void)hipStreamDestroy(h->stream);
+        (void)hipStreamDestroy(h->ownStream);
         delete h;
         return hip_fail(nullptr, "hipMalloc", e);
     }
@@ -377,8 +379,25 @@ void vbd_destroy(vbd_solver* h) {
     free_scratch(h);
     free_device_mesh(h->m);
     if (h->d_energy) (void)hipFree(h->d_energy);
-    (void)hipStreamDestroy(h->stream);
+    (void)hipStreamSynchronize(h->ownStream);
+    (void)hipStreamDestroy(h->ownStream);
     delete h;
+}
+
+int vbd_set_stream(vbd_solver* h, void* stream) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    DeviceGuard g(h->device);
+    hipStream_t next = stream ? (hipStream_t)stream : h->ownStream;
+    if (stream) {
+        int dev = -1;
+        hipError_t e = hipStreamGetDevice(next, &dev);
+        if (e != hipSuccess) return hip_fail(h, "hipStreamGetDevice", e);
+        if (dev != h->device) return fail(h, VBD_ERR_ARG, "the stream belongs to another device");
+    }
+    // work already enqueued on the previous stream finishes before anything on the new one
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    h->stream = next;
+    return VBD_OK;
 }
 
 int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {

@@ -362,6 +362,12 @@ class DeviceMesh(_Handle):
     def synchronize(self):
         check(lib.vbd_synchronize(self.h), self.h)
 
+    def set_stream(self, stream=None):
+        """Run on a caller's stream: a torch.cuda.Stream, a raw hipStream_t (int), or None for the
+        handle's own stream."""
+        raw = getattr(stream, "cuda_stream", stream)
+        check(lib.vbd_set_stream(self.h, C.c_void_p(raw) if raw else None), self.h)
+
     def set_graph(self, enabled: bool):
         check(lib.vbd_set_graph_enabled(self.h, int(bool(enabled))), self.h)
 
