@@ -47,7 +47,6 @@ struct DevMesh {
     // compact records of the tiled kernel (with ptet): F q = beta (x_s - y_a) + alpha1 (y_b - o)
     // + alpha2 (y_c - o) with o = corner 0, C q = d (y_b - y_a) x (y_c - y_a), J = C q . (x_s - y_a)
     f4* pq = nullptr;       // nPairs  beta, alpha1, alpha2, d
-    float* pqq = nullptr;   // nPairs  V0 |q|^2
     // GPU-built structures kept in the caller's numbering (exposed for verification)
     int* adjOff = nullptr;    // nV+1
     int* adjEntry = nullptr;  // 4*nT, tet*4 + corner

@@ -320,16 +320,19 @@ __device__ __forceinline__ f4 vertex_solve(const f4 xs, const f4 ys, const float
 //   J   = (C q) . delta                                         (J is affine in x_s and
 //                                                                vanishes on the face a b c)
 // and then, exactly as in the full form,
-//   g = V [mu (F q - C q) + lambda (J-1) C q],   H = mu V |q|^2 I + lambda V (C q)(C q)^T.
-// The record is ptet (3 slots, V) + pq (beta, alpha1, alpha2, d) + pqq (V |q|^2): 36 B
-// instead of 52 B, and about half the arithmetic of forming F and cof(F).
+//   g = V [mu (F q - C q) + lambda (J-1) C q],   H = mu V |q|^2 I + lambda V (C q)(C q)^T,
+// with |q|^2 = alpha0 = beta (s not corner 0) or -(alpha0 + alpha1 + alpha2) = beta - alpha1
+// - alpha2 (s corner 0, q = -(R0 + R1 + R2)).  The record is ptet (3 slots, V) + pq (beta,
+// alpha1, alpha2, d): 32 B instead of 52 B, and about half the arithmetic of forming F and
+// cof(F).
 //
 // The color's free vertices are cut into tiles of consecutive slots holding at most
 // RT = K * blockDim.x records and fewer than blockDim.x / 8 vertices (tileStart / tileRec,
 // built on the host).  One block per tile; XCD x takes the x-th eighth of the tiles.
 //   1. thread i takes records r0+i, r0+i+T, ... (K of them): the whole block streams one
 //      contiguous record range, every lane busy.  The record's owner slot comes from
-//      chunkBase[p >> 12] + the owner offset bits.  The 9 contribution values (6 Hessian
+//      chunkBase[p >> 12] + the owner offset bits; a tile spans at most two 4096-record
+//      chunks, whose bases are loaded once per block.  The 9 contribution values (6 Hessian
 //      entries, 3 force entries) go to LDS, one array per value; the tile's CSR row offsets,
 //      positions, inertia targets and masses are staged in LDS too.
 //   2. after a barrier, TG = 8 lanes per vertex sum its records (lane l: records l, l+TG, ...
@@ -345,13 +348,11 @@ template <int K, bool MAT>
 struct RecRegs {
     i4 t[K];
     f4 q[K];
-    float qq[K];
     f2 mm[K];
-    int cb[K];
 };
 
 struct RecSrc {  // buffer resources over one color's records (byte offsets from its first record)
-    __amdgpu_buffer_rsrc_t t, q, qq, mm;
+    __amdgpu_buffer_rsrc_t t, q, mm;
 };
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const void* base, int64_t bytes) {
@@ -361,17 +362,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const void* base, int
 // Loads are unconditional (index clamped to the color's last record): no branch around
 // them, so the compiler's wait counts can keep the gathers in flight with the vertex loads.
 template <int K, bool MAT>
-__device__ __forceinline__ void load_records(RecRegs<K, MAT>& R, const RecSrc& src, int r0, int pmax, int rec0,
-                                             int tid, int T, const int* __restrict__ chunkBase) {
+__device__ __forceinline__ void load_records(RecRegs<K, MAT>& R, const RecSrc& src, int r0, int pmax, int tid, int T) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = min(r0 + tid + k * T, pmax);  // relative to the color's first record
         R.t[k] = __builtin_bit_cast(i4, __builtin_amdgcn_raw_buffer_load_b128(src.t, p << 4, 0, BUF_NT));
         R.q[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.q, p << 4, 0, BUF_NT));
-        R.qq[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(src.qq, p << 2, 0, BUF_NT));
         if constexpr (MAT)
             R.mm[k] = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(src.mm, p << 3, 0, BUF_NT));
-        R.cb[k] = chunkBase[(p + rec0) >> CHUNK_SHIFT];
     }
 }
 
@@ -385,10 +383,11 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Explicit fmaf and no contraction: every instantiation (K, MAT, ...) rounds alike, so the
 // tiled configurations give bitwise identical results.
-__device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, const float vqq, const float V,
-                                               const float tmu, const float tlam, const f4 xs, const f4 ya,
-                                               const f4 yb, const f4 yc) {
+__device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, const float V, const float tmu,
+                                               const float tlam, const f4 xs, const f4 ya, const f4 yb,
+                                               const f4 yc) {
 #pragma clang fp contract(off)
+    const float qq = corner0 ? (qv.x - qv.y) - qv.z : qv.x;  // |q|^2
     const float dx = xs.x - ya.x, dy = xs.y - ya.y, dz = xs.z - ya.z;
     const float ux = yb.x - ya.x, uy = yb.y - ya.y, uz = yb.z - ya.z;
     const float wx = yc.x - ya.x, wy = yc.y - ya.y, wz = yc.z - ya.z;
@@ -407,7 +406,7 @@ __device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, 
     const float g0 = __builtin_fmaf(tmu, f0 - c0, aj * c0);
     const float g1 = __builtin_fmaf(tmu, f1 - c1, aj * c1);
     const float g2 = __builtin_fmaf(tmu, f2v - c2, aj * c2);
-    const float vm = tmu * vqq;
+    const float vm = tmu * (V * qq);
     const float vl = V * tlam;
     const float v0 = vl * c0, v1 = vl * c1, v2 = vl * c2;
     Acc A;
@@ -429,9 +428,8 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
                                                      const f4* __restrict__ inert, const float* __restrict__ mass,
                                                      const int* __restrict__ row, const int* __restrict__ chunkBase,
                                                      const i4* __restrict__ ptet, const f4* __restrict__ pq,
-                                                     const float* __restrict__ pqq, const f2* __restrict__ pmat,
-                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
-                                                     ContactArgs ct) {
+                                                     const f2* __restrict__ pmat, f4* __restrict__ save, float mu,
+                                                     float lambda, float sdt, ContactArgs ct) {
     // LDS: 9 * RT contribution values, then VM = T/TG entries (a tile has at most VM - 1
     // vertices): row offsets (nv + 1 of them), x, y, m
     extern __shared__ float red[];
@@ -454,12 +452,14 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     RecSrc src;
     src.t = rec_rsrc(ptet + rec0, 16 * (int64_t)nrec);
     src.q = rec_rsrc(pq + rec0, 16 * (int64_t)nrec);
-    src.qq = rec_rsrc(pqq + rec0, 4 * (int64_t)nrec);
-    src.mm = MAT ? rec_rsrc(pmat + rec0, 8 * (int64_t)nrec) : src.qq;
+    src.mm = MAT ? rec_rsrc(pmat + rec0, 8 * (int64_t)nrec) : src.q;
 
     const int r0 = tileRec[tile] - rec0, r1 = tileRec[tile + 1] - rec0;
     RecRegs<K, MAT> R;
-    load_records<K, MAT>(R, src, r0, pmax, rec0, tid, T, chunkBase);
+    load_records<K, MAT>(R, src, r0, pmax, tid, T);
+    // owner bases of the (at most two) 4096-record chunks the tile touches
+    const int chunkA = (rec0 + r0) >> CHUNK_SHIFT;
+    const int cbA = chunkBase[chunkA], cbB = chunkBase[(rec0 + max(r1 - 1, r0)) >> CHUNK_SHIFT];
     const int v0 = tileStart[tile], v1 = tileStart[tile + 1];
     const int nv = v1 - v0;
     // the tile's vertex data (clamped indices; lanes past nv load and ignore a copy)
@@ -471,7 +471,9 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     f4 xs[K], ya[K], yb[K], yc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        xs[k] = gather_pos(prs, R.cb[k] + owner_offset(R.t[k]));
+        const int p = min(r0 + tid + k * T, pmax);
+        const int cb = ((rec0 + p) >> CHUNK_SHIFT) == chunkA ? cbA : cbB;
+        xs[k] = gather_pos(prs, cb + owner_offset(R.t[k]));
         ya[k] = gather_pos(prs, R.t[k].x & SLOT_MASK);
         yb[k] = gather_pos(prs, R.t[k].y & SLOT_MASK);
         yc[k] = gather_pos(prs, R.t[k].z & SLOT_MASK);
@@ -484,8 +486,8 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
             tmu = R.mm[k].x;
             tlam = R.mm[k].y;
         }
-        const Acc A = compact_contrib(R.t[k].x < 0, R.q[k], R.qq[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k],
-                                      ya[k], yb[k], yc[k]);
+        const Acc A = compact_contrib(R.t[k].x < 0, R.q[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k], ya[k], yb[k],
+                                      yc[k]);
         if (r0 + q < r1) {
             red[0 * RT + q] = A.h00;
             red[1 * RT + q] = A.h01;
@@ -546,7 +548,7 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
     const int per = (ntiles + 7) / 8;
     hipLaunchKernelGGL((k_color_tiled<K, MAT, CONTACT, SAVE>), dim3(8 * per), dim3(block), lds, s,
                        m.tileStart + m.colorTileBase[color], m.tileRec + m.colorTileBase[color], ntiles, m.nV, m.pos,
-                       m.inert, m.mass, m.row, m.chunkBase, m.ptet, m.pq, m.pqq, m.pmat, save, mu, lambda, sdt, ct);
+                       m.inert, m.mass, m.row, m.chunkBase, m.ptet, m.pq, m.pmat, save, mu, lambda, sdt, ct);
     return hipGetLastError();
 }
 
@@ -1660,7 +1662,7 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
                                                      const int* __restrict__ row, const int* __restrict__ chunkBase,
                                                      i4* __restrict__ ptet, f4* __restrict__ pa, f4* __restrict__ pb,
                                                      float* __restrict__ pc, f2* __restrict__ pmat,
-                                                     f4* __restrict__ pq, float* __restrict__ pqq) {
+                                                     f4* __restrict__ pq) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nFree) return;
     const int v = perm[s];
@@ -1707,7 +1709,7 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
         {
             // compact record (tiled kernel), in double from the FP32 DmInv rows R_n:
             // q = -(R0+R1+R2) (corner 0) or R0, alpha_n = R_n . q, beta = -alpha_0 (corner 0) or
-            // alpha_0, d = det(DmInv) * (-1, +1, -1, +1)[k], V |q|^2
+            // alpha_0, d = det(DmInv) * (-1, +1, -1, +1)[k]
             double R[3][3], q[3], al[3];
             for (int n = 0; n < 3; ++n)
                 for (int j = 0; j < 3; ++j) R[n][j] = (double)Q[3 * n + j];
@@ -1723,7 +1725,6 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
             cq.z = (float)al[2];
             cq.w = (float)((k == 0 || k == 2) ? -det : det);
             pq[p] = cq;
-            pqq[p] = (float)((double)vol[t] * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]));
         }
         if (pmat) {
             f2 mm;
@@ -1912,11 +1913,10 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
     VBD_TRY(dalloc(m, &m.pc, (size_t)m.nPairs));
     if (m.perTetMat) VBD_TRY(dalloc(m, &m.pmat, (size_t)m.nPairs));
     VBD_TRY(dalloc(m, &m.pq, (size_t)m.nPairs));
-    VBD_TRY(dalloc(m, &m.pqq, (size_t)m.nPairs));
     if (m.nFree > 0) {
         hipLaunchKernelGGL(k_build_pairs, grid_for(m.nFree), dim3(256), 0, s, m.nFree, m.perm, m.inv, m.adjOff,
                            m.adjEntry, d_tets, d_dminv, d_vol, d_mu, d_lam, m.row, m.chunkBase, m.ptet, m.pa,
-                           m.pb, m.pc, m.perTetMat ? m.pmat : nullptr, m.pq, m.pqq);
+                           m.pb, m.pc, m.perTetMat ? m.pmat : nullptr, m.pq);
         VBD_TRY(hipGetLastError());
     }
 
@@ -1984,7 +1984,7 @@ fail:
 void free_device_mesh(DevMesh& m) {
     void* ptrs[] = {m.pos, m.inert, m.vel, m.xprev, m.save[0], m.save[1], m.mass, m.row, m.perm, m.inv,
                     m.ptet, m.pa, m.pb, m.pc, m.pmat, m.adjOff, m.adjEntry, m.colorList, m.tets, m.dminv,
-                    m.vol, m.tmu, m.tlam, m.chunkBase, m.tileStart, m.tileRec, m.pq, m.pqq};
+                    m.vol, m.tmu, m.tlam, m.chunkBase, m.tileStart, m.tileRec, m.pq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     m = DevMesh();
