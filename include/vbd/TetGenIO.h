@@ -6,7 +6,7 @@
 // nodes (empty name or unreadable file -> rest shape, main.cpp:67-72), and the .ele tets
 // ("<#tets> <4> <#attributes>", then "<index> <v0> <v1> <v2> <v3> ..."), shifts 1-based ids
 // to 0-based (main.cpp:112-114), pins every vertex whose coordinate on `pinAxis` is within
-// `pinFraction` of the extent from the minimum (main.cpp:146-164 pins y <= min + 5%, judged
+// `pinFraction` of the extent from the minimum (main.cpp:146-164 pins y <= min + 5%, tested
 // on the current positions as the reference does), and calls TetMesh::initialize.
 #pragma once
 

@@ -11,7 +11,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the C ABI on the device)")
-    # Build what is missing (CPU container only; on the GPU box the in-tree builds travel).
+    # Build what is missing (the in-tree builds are reused where they exist).
     lib = os.path.join(ROOT, "vbd_tetmesh_amd", "libvbd_hip.so")
     orc = os.path.join(ROOT, "oracle", "_oracle.so")
     if not os.path.exists(orc):
