@@ -223,6 +223,7 @@ def bench_adjoint(n: int = 16, steps: int = 30):
             "adjoint_over_forward": round(st.backward_seconds / st.forward_seconds, 2),
             "forward_iterations_mean": round(float(np.mean(st.forward_iterations)), 1),
             "adjoint_sweeps_mean": round(float(np.mean(st.adjoint_sweeps)), 1),
+            "adjoint_chebyshev_rho": sim.adj_rho, "adjoint_fallbacks": st.adjoint_fallbacks,
             "forward_value": round(vi / st.forward_seconds / 1e6, 2), "unit": "M vertex-iters/s"}
 
 

@@ -244,9 +244,11 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
 /* out = A w at the current positions (rows of free vertices). */
 int vbd_hessian_vector(vbd_solver* h, const float* w, float* out);
 /* Solves A w = rhs by colored block Gauss-Seidel (the forward colors, exact 3x3 diagonal blocks)
- * with Chebyshev acceleration when rho > 0.  w is the initial guess on entry and the solution
- * on exit; stops when |rhs - A w| <= rel_tol |rhs| (tested every check_every sweeps) or on
- * stagnation as in vbd_step_converged. */
+ * with Chebyshev acceleration when 0 < rho < 1.  w is the initial guess on entry and the
+ * solution on exit; stops when |rhs - A w| <= rel_tol |rhs| (tested every check_every sweeps)
+ * or on stagnation as in vbd_step_converged.  Returns VBD_ERR_NUMERIC, w unchanged, when the
+ * residual grows past 1e3 x max(|rhs|, first residual) or is non-finite (rho too large; plain
+ * Gauss-Seidel, rho = 0, converges for the SPD A). */
 int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_iterations, int32_t check_every,
                       float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual);
 /* Per-tet -(d^2 E / dx dmu)^T w and -(d^2 E / dx dlambda)^T w at the current positions:

@@ -78,6 +78,45 @@ def test_adjoint_solve_matches_direct_solve(small, rho):
     assert np.all(w[m.fixed == 1] == 0)
 
 
+def test_adjoint_solve_rejects_bad_rho_and_reports_divergence(small):
+    """rho outside [0, 1) is an argument error; a diverging Chebyshev solve returns
+    VBD_ERR_NUMERIC and leaves w unchanged; DiffSim then reruns plain Gauss-Seidel."""
+    from vbd_tetmesh_amd import _capi as A
+    m, _, _, _, _, sim, x = small
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    sim.dev.set_positions(x)
+    rhs = np.random.default_rng(5).normal(size=x.shape).astype(np.float32)
+    rhs[m.fixed == 1] = 0
+    for bad in (1.0, -0.1):
+        with pytest.raises(A.VBDError) as ei:
+            sim.dev.adjoint_solve(rhs, rel_tol=1e-6, rho=bad)
+        assert ei.value.code == A.VBD_ERR_ARG
+    outcomes = []
+    for rho in (0.99, 0.999, 0.9999):
+        try:
+            _, sweeps, rr = sim.dev.adjoint_solve(rhs, rel_tol=1e-6, rho=rho, max_iterations=4000)
+            outcomes.append(("converged", rho, sweeps, rr))
+            assert rr <= 1e-6 or sweeps == 4000
+        except A.VBDError as e:
+            assert e.code == A.VBD_ERR_NUMERIC
+            outcomes.append(("diverged", rho))
+    print("chebyshev rho outcomes:", outcomes, flush=True)
+
+
+def test_diverging_chebyshev_falls_back_to_gauss_seidel(small):
+    """A rollout whose Chebyshev rho diverges (slowly, so the stall test must not accept it)
+    still returns the plain Gauss-Seidel gradients."""
+    m, mu_t, lam_t, v0, _, _, _ = small
+    x_t = m.rest.copy()
+    ref = DiffSim(m, steps=4, adj_rho=0.0).loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    sim = DiffSim(m, steps=4, adj_rho=0.9999)
+    got = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    print(f"fallbacks {sim.stats.adjoint_fallbacks} of {len(sim.stats.adjoint_sweeps)} solves", flush=True)
+    assert sim.stats.adjoint_fallbacks > 0
+    for a, b in zip(got[1:], ref[1:]):
+        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b)
+
+
 def test_converged_step_matches_newton(small):
     m, _, _, v0, P, sim, _ = small
     sim.dev.set_materials(m.tet_mu, m.tet_lambda)

@@ -22,6 +22,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
+from . import _capi as A
 from .solver import DeviceMesh, VBDParams, prepare_host
 
 
@@ -31,6 +32,7 @@ class RolloutStats:
     forward_residuals: List[float] = field(default_factory=list)
     adjoint_sweeps: List[int] = field(default_factory=list)
     adjoint_residuals: List[float] = field(default_factory=list)
+    adjoint_fallbacks: int = 0  # solves rerun without Chebyshev after diverging
     forward_seconds: float = 0.0
     backward_seconds: float = 0.0
 
@@ -40,7 +42,7 @@ class DiffSim:
 
     def __init__(self, mesh, steps: int = 30, h: float = 1.0 / 60.0, rho: float = 0.95, max_iterations: int = 4000,
                  check_every: int = 10, rel_tol: float = 1e-6, stall_checks: int = 10, adj_max_iterations: int = 8000,
-                 adj_rel_tol: float = 1e-5, adj_rho: float = 0.9, adj_stall_checks: int = 5, checkpoint_every: int = 5,
+                 adj_rel_tol: float = 1e-5, adj_rho: float = 0.95, adj_stall_checks: int = 5, checkpoint_every: int = 5,
                  device: int = 0, damping: float = 0.999):
         if mesh.tet_mu is None:
             raise ValueError("the differentiable rollout needs per-tet materials (mesh.tet_mu / tet_lambda)")
@@ -117,9 +119,7 @@ class DiffSim:
             for n in range(end - 1, start - 1, -1):
                 ahat = a + (c / h) * b
                 self.dev.set_positions(xs[n + 1 - start])
-                w, sweeps, rr = self.dev.adjoint_solve(ahat, w0=w, max_iterations=self.adj_max_iterations,
-                                                       check_every=self.check_every, rel_tol=self.adj_rel_tol,
-                                                       rho=self.adj_rho, stall_checks=self.adj_stall_checks)
+                w, sweeps, rr = self._adjoint_solve(ahat, w)
                 self.stats.adjoint_sweeps.append(sweeps)
                 self.stats.adjoint_residuals.append(rr)
                 dm, dl = self.dev.parameter_gradients(w)
@@ -133,6 +133,19 @@ class DiffSim:
                 a, b = a_new, b_new
         self.stats.backward_seconds = time.perf_counter() - t0
         return gmu, glam, b
+
+    def _adjoint_solve(self, rhs, w0):
+        """Chebyshev-accelerated solve; plain Gauss-Seidel (rho = 0) if the accelerated one diverges."""
+        kw = dict(max_iterations=self.adj_max_iterations, check_every=self.check_every, rel_tol=self.adj_rel_tol,
+                  stall_checks=self.adj_stall_checks)
+        if self.adj_rho > 0:
+            try:
+                return self.dev.adjoint_solve(rhs, w0=w0, rho=self.adj_rho, **kw)
+            except A.VBDError as e:
+                if e.code != A.VBD_ERR_NUMERIC:
+                    raise
+                self.stats.adjoint_fallbacks += 1
+        return self.dev.adjoint_solve(rhs, w0=w0, rho=0.0, **kw)
 
     def loss_and_grads(self, mu, lam, v0, x_target):
         x, ck = self.rollout(mu, lam, v0)

@@ -1265,6 +1265,11 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
         it += n;
         r2 = h->h_res[0];
         const double r = std::sqrt(r2);
+        if (!std::isfinite(r)) {
+            if (iterations) *iterations = it;
+            if (final_norm) *final_norm = r;
+            return fail(h, VBD_ERR_NUMERIC, "converged step: the residual became non-finite");
+        }
         if (r <= tol) break;
         // stagnation at the float32 floor: no 1% gain on the best residual for stall_checks checks
         if (r < 0.99 * best) {
@@ -1307,6 +1312,7 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if ((!w || !rhs) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
     if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
+    if (!(rho >= 0.f && rho < 1.f)) return fail(h, VBD_ERR_ARG, "rho must be in [0, 1)");
     DeviceGuard g(h->device);
     HIP_CHECK(h, ensure_scratch(h));
     const ContactArgs ct = contact_of(h->params);
@@ -1323,6 +1329,7 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
     double best = 1e300;
     const bool accel = rho > 0.f;
     std::vector<float> om;
+    double first = 0.0;  // residual at the first check
     while (it < max_iterations && bn > 0) {
         const int n = std::min(check_every - it % check_every, max_iterations - it);
         om.assign(n, 1.0f);
@@ -1351,6 +1358,16 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
         r2 = h->h_res[0];
         const double r = std::sqrt(r2);
         if (r <= rel_tol * bn) break;
+        // Chebyshev with rho above the iteration's spectral radius diverges, fast (the residual
+        // explodes) or slowly (it creeps up until the stall test would end the solve): report both
+        const bool blowup = !std::isfinite(r) || r > 1e3 * std::max(bn, first) || (accel && r > 1e2 * best);
+        const bool stall = !(r < 0.99 * best) && stall_checks > 0 && stalled + 1 >= stall_checks;
+        if (blowup || (stall && accel && r > 10.0 * best)) {
+            if (iterations) *iterations = it;
+            if (rel_residual) *rel_residual = r / bn;
+            return fail(h, VBD_ERR_NUMERIC, "adjoint solve diverged (Chebyshev rho too large); w is unchanged");
+        }
+        if (first == 0.0) first = r;
         if (r < 0.99 * best) {
             best = r;
             stalled = 0;
