@@ -1,0 +1,123 @@
+"""GPU edge cases whose outcome the reference's semantics fix.
+
+* An empty mesh and a mesh whose vertices are all pinned: a step moves nothing (VBD.cpp:103-112,
+  TetMesh.h:305-325 touch free vertices only).
+* Tet-less vertices (mass 0) and the corners of a zero-stiffness tet of tiny mass: the 3x3
+  block has |det H| <= 1e-8, so the vertex is skipped (VBD.cpp:141-153) and never moves.
+* A color made only of tet-less vertices (a color with no records), placed last.
+
+Tolerance: positions within 1e-5 relative max-norm of the FP32 oracle; skipped vertices and
+the coloring-independence checks are exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vbd_tetmesh_amd import VBDParams, meshgen
+from vbd_tetmesh_amd.solver import DeviceMesh, mesh_from_generator, prepare_host, upload
+
+pytestmark = pytest.mark.gpu
+
+H = 1.0 / 60.0
+
+
+def relmax(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def params_for(m, iterations):
+    return VBDParams(dt=H, numIterations=iterations, substeps=1, gravity=m.gravity, useAcceleration=True,
+                     accelerationRho=0.95, mu=m.mu, lambda_=m.lam)
+
+
+def beam_with_loose_parts(n_loose=3):
+    """The 525-vertex beam plus `n_loose` tet-less vertices and a separate 1 mm tet with
+    mu = lambda = 0 (per-tet materials)."""
+    b = meshgen.beam()
+    nb = b.num_verts
+    loose = np.array([[2.0 + 0.1 * i, 0.5, 0.5] for i in range(n_loose)], np.float32)
+    tiny = np.array([[3, 0, 0], [3.001, 0, 0], [3, 0.001, 0], [3, 0, 0.001]], np.float32)
+    rest = np.concatenate([b.rest, loose, tiny])
+    tets = np.concatenate([b.tets, np.array([[nb + n_loose + k for k in range(4)]], np.int32)])
+    fixed = np.concatenate([b.fixed, np.zeros(n_loose + 4, np.uint8)])
+    tet_mu = np.full(len(tets), b.mu, np.float32)
+    tet_lam = np.full(len(tets), b.lam, np.float32)
+    tet_mu[-1] = tet_lam[-1] = 0.0
+    m = meshgen.Mesh(name="beam+loose", rest=rest, x0=rest.copy(), tets=tets, fixed=fixed, mu=b.mu, lam=b.lam,
+                     density=b.density, tet_mu=tet_mu, tet_lambda=tet_lam, gravity=b.gravity)
+    return m, np.arange(nb, len(rest))
+
+
+def test_empty_mesh_steps_as_a_noop():
+    print("[empty mesh]", flush=True)
+    d = DeviceMesh(0).init(np.zeros((0, 3)), np.zeros(0), np.zeros(0, np.uint8), np.zeros((0, 4)), np.zeros((0, 9)),
+                           np.zeros(0), np.zeros(1, np.int32), np.zeros(0, np.int32))
+    d.set_params(VBDParams(dt=H, numIterations=5))
+    d.step(3)
+    assert d.positions().shape == (0, 3)
+    assert d.total_energy(1e5, 4e5) == 0.0
+    assert d.incremental_potential() == (0.0, 0.0)
+
+
+def test_all_pinned_mesh_does_not_move():
+    m = meshgen.beam()
+    m.fixed[:] = 1
+    print(f"[all pinned] nV={m.num_verts}", flush=True)
+    d = upload(m, params=params_for(m, 20))
+    assert d.info().num_free == 0
+    d.step(5)
+    assert np.array_equal(d.positions(), m.x0)
+
+
+@pytest.mark.parametrize("lanes", [None, 8])
+def test_singular_blocks_are_skipped_like_the_oracle(lanes):
+    m, loose = beam_with_loose_parts()
+    t = mesh_from_generator(m)
+    det = (t.vertexMass[loose].astype(np.float64) / H ** 2) ** 3
+    assert np.all(det <= 1e-8)  # zero-stiffness corners: H = m/h^2 I is singular by the 1e-8 test
+    print(f"[loose parts] lanes={lanes} nV={m.num_verts}", flush=True)
+    prep = O.prepare(m)
+    op = O.make_params(dt=H, iterations=50, gravity=m.gravity, accel=True, rho=0.95, mu=m.mu, lam=m.lam)
+    _, _, of, _ = O.simulate(prep, op, m.x0, None, 10)
+    d = upload(m, params=params_for(m, 50))
+    if lanes:
+        d.set_launch_config(lanes, 256, 1)
+    got = []
+    for _ in range(10):
+        d.step(1)
+        got.append(d.positions())
+    rel = max(relmax(got[f], of[f]) for f in range(10))
+    print(f"loose parts: max rel vs fp32 oracle {rel:.2e}", flush=True)
+    assert rel <= 1e-5
+    for f in range(10):
+        assert np.array_equal(got[f][loose], m.x0[loose])
+        assert np.array_equal(of[f][loose], m.x0[loose])
+    assert np.abs(got[-1][:-7] - m.x0[:-7]).max() > 1e-3  # the beam itself sags
+
+
+@pytest.mark.parametrize("lanes", [None, 8])
+def test_a_color_of_tetless_vertices_changes_nothing(lanes):
+    """Tet-less vertices moved from color 0 into a color of their own, placed last: a color
+    with no records.  Same positions, bitwise."""
+    m, loose = beam_with_loose_parts(n_loose=5)
+    m.tets = m.tets[:-1]
+    m.tet_mu, m.tet_lambda = m.tet_mu[:-1], m.tet_lambda[:-1]
+    m.rest, m.x0, m.fixed = m.rest[:-4], m.x0[:-4], m.fixed[:-4]
+    loose = loose[:-4]
+    t, off, verts, dminv, vol = prepare_host(m)
+    keep = ~np.isin(verts, loose)
+    counts = np.array([np.count_nonzero(keep[off[c]:off[c + 1]]) for c in range(len(off) - 1)] + [len(loose)])
+    off2 = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    verts2 = np.concatenate([verts[keep], loose]).astype(np.int32)
+    runs = []
+    for o, v in ((off, verts), (off2, verts2)):
+        d = DeviceMesh(0).init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, o, v, velocity=t.mVelocity,
+                               tet_mu=t.tetMu, tet_lambda=t.tetLambda)
+        d.set_params(params_for(m, 30))
+        if lanes:
+            d.set_launch_config(lanes, 256, 1)
+        d.step(4)
+        runs.append(d.positions())
+    print(f"[tet-less color] lanes={lanes} colors {len(off) - 1} -> {len(off2) - 1}", flush=True)
+    assert np.array_equal(runs[0], runs[1])
+    assert np.array_equal(runs[1][loose], m.x0[loose])

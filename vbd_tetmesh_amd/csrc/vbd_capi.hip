@@ -827,6 +827,7 @@ int vbd_step(vbd_solver* h, int32_t num_frames) {
     if (rc) return rc;
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if (num_frames < 0) return fail(h, VBD_ERR_ARG, "negative frame count");
+    if (h->m.nFree == 0) return VBD_OK;  // every update of a step touches free vertices only
     DeviceGuard g(h->device);
     if (!h->graphEnabled) {
         for (int f = 0; f < num_frames; ++f) HIP_CHECK(h, enqueue_step(h));

@@ -1893,9 +1893,11 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
             m.hostRow[i + 1] = m.hostRow[i] + d;
         }
         // chunk c's base = owner slot of record c * 4096; every record's owner must lie within
-        // 4095 slots of its chunk's base (fails only with long runs of tet-less free vertices)
+        // 4095 slots of its chunk's base (fails only with long runs of tet-less free vertices).
+        // One spare entry: a tile of tet-less vertices after the last record reads the base
+        // of chunk nPairs >> 12.
         const int64_t nChunks = (m.nPairs + (1 << CHUNK_SHIFT) - 1) >> CHUNK_SHIFT;
-        std::vector<int> hcb((size_t)std::max<int64_t>(nChunks, 1), 0);
+        std::vector<int> hcb((size_t)nChunks + 1, 0);
         m.tiledOk = true;
         for (int i = 0; i < m.nFree; ++i) {
             const int a = m.hostRow[i], b = m.hostRow[i + 1];
