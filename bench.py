@@ -204,6 +204,54 @@ def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30, slot_order:
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
+def bench_energy(d, reps: int = 20):
+    """GPU energies on the resident mesh: the log-Neo-Hookean diagnostic (VBD.cpp:258-288) and
+    the incremental potential G, each a per-tet kernel plus a fixed-order double reduction,
+    timed with HIP events over `reps` back-to-back evaluations."""
+    res = {}
+    for mode, name in ((0, "log_neo_hookean_diagnostic"), (1, "incremental_potential")):
+        d.time_energy(mode, 2)
+        ms, nb = d.time_energy(mode, reps)
+        gbs = nb / ms / 1e6
+        res[name] = {"ms_per_eval": round(ms, 4), "alg_bytes": round(nb), "achieved": round(gbs, 1),
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return res
+
+
+def bench_corotated(d, m, iterations: int, steps: int = 5):
+    """The reference GPU kernel's material (corotated, VBD_GPU_Wrapper.cu:114-213, with its
+    x1.000001 inertia term) on the same resident mesh: autotuned lane-group kernel, whole steps
+    replayed from a hipGraph, and the color kernel against the same byte yardstick."""
+    from vbd_tetmesh_amd import _capi as A
+    info0 = d.info()
+    d.set_material_model(A.VBD_MATERIAL_COROTATED_GPU)
+    d.set_positions(m.x0)
+    tune = d.autotune(2)
+    d.set_positions(m.x0)
+    d.set_velocities(np.zeros_like(m.x0))
+    d.step(1)
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.step(steps)
+    d.synchronize()
+    secs = time.perf_counter() - t0
+    info = d.info()
+    x = d.positions()
+    cms, cn, _, _ = d.time_iterations(iterations)
+    cbytes, _ = d.algorithmic_bytes()
+    per_launch_s = cms / cn / 1e3
+    d.set_material_model(A.VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+    d.set_launch_config(info0.lanes_per_vertex, info0.block_size, info0.pairs_per_lane)
+    if not np.all(np.isfinite(x)):
+        raise SystemExit("non-finite positions after the corotated steps")
+    vi = float(info.num_verts) * iterations * steps
+    return {"workload": "same cube, corotated material of the reference GPU kernel", "value": round(vi / secs / 1e6, 2),
+            "unit": "M vertex-iters/s", "ms_per_step": round(secs * 1e3 / steps, 3), "steps": steps,
+            "kernel": "k_color_corot", "avg_launch_ms": round(per_launch_s * 1e3, 4),
+            "color_kernel_frac": round(cbytes / info.num_colors / per_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+            "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
+
+
 def bench_adjoint(n: int = 16, steps: int = 30):
     """Differentiable workload: 16^3 cube, per-tet mu field, lambda = 4 mu, 30 converged steps;
     forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation +
@@ -256,6 +304,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
     ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
+    ap.add_argument("--no-corotated", action="store_true", help="skip the corotated-material run of the cube")
     ap.add_argument("--only", choices=["batched", "sphere", "adjoint"], default=None,
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
@@ -352,6 +401,12 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(overall / HBM_PEAK_GBS, 4),
                           "blend_avg_launch_ms": round(bms / max(bn, 1), 4)},
     }
+    out["energy"] = bench_energy(d)
+    log(f"[bench] energies: " + ", ".join(f"{k} {v['ms_per_eval']:.3f} ms ({v['frac']})" for k, v in out["energy"].items()))
+    if not args.no_corotated:
+        cr = bench_corotated(d, m, args.iterations)
+        out["corotated"] = cr
+        log(f"[bench] corotated: {cr['value']:.1f} M vertex-iters/s, color kernel frac {cr['color_kernel_frac']}")
     if not args.no_batch:
         del d
         bb = bench_batch()

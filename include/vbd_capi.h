@@ -225,6 +225,16 @@ int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy);
  * (or, with VBD_MATERIAL_COROTATED_GPU, mu |F - R|^2 + lambda/2 tr(R^T F - I)^2), h = dt /
  * substeps and materials from the parameters (or per-tet).  elastic = the Psi sum. */
 int vbd_incremental_potential(vbd_solver* h, double* potential, double* elastic);
+/* Times `reps` back-to-back evaluations of an energy reduction with HIP events on the
+ * handle's stream (no host synchronisation in between; no reference counterpart).
+ * mode 0: the log-Neo-Hookean diagnostic of vbd_total_energy (mu, lambda from the step
+ * parameters, 1e5 / 4e5 before vbd_set_params); mode 1: the incremental potential of
+ * vbd_incremental_potential (needs vbd_set_params).  Outputs: milliseconds per evaluation and
+ * its algorithmic bytes -- per tet 56 (ids 16, DmInv 36, V0 4; +8 with per-tet mu, lambda),
+ * 12 per vertex (positions, read once; the per-tet corner gathers are cache-resident and not
+ * counted, as for the color launches), plus 28 per free vertex in mode 1 (x, inertia target,
+ * mass). */
+int vbd_time_energy(vbd_solver* h, int32_t mode, int32_t reps, double* ms_per_eval, double* bytes_per_eval);
 
 /* ---- differentiation (no reference counterpart: the reference is not differentiable) ----
  * The adjoint of one implicit step x_{n+1} = argmin G needs A = M/h^2 + K(x_{n+1}) (K = Hessian

@@ -381,6 +381,22 @@ def test_device_energy_matches_oracle():
     assert np.isclose(e, eo, rtol=1e-4), (e, eo)
 
 
+def test_energy_timing_leaves_the_energies_and_reports_the_yardstick():
+    m = meshgen.noisy_cube(10, seed=1)
+    d = upload(m, params=params_for(m, 5))
+    e0, g0 = d.total_energy(m.mu, m.lam), d.incremental_potential()
+    info = d.info()
+    for mode, extra in ((0, 0), (1, 28 * info.num_free)):
+        ms, nb = d.time_energy(mode, 5)
+        assert ms > 0 and nb == 56 * info.num_tets + 12 * info.num_verts + extra
+    assert d.total_energy(m.mu, m.lam) == e0 and d.incremental_potential() == g0
+    from vbd_tetmesh_amd import _capi as A
+    for bad in ((2, 5), (0, 0)):
+        with pytest.raises(A.VBDError) as ei:
+            d.time_energy(*bad)
+        assert ei.value.code == A.VBD_ERR_ARG
+
+
 def _potential_fp64(m, x, y, mass, contact=None):
     from oracle import diff_oracle as D
     P = D.make_problem(m)

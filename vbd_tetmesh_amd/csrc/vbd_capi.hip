@@ -1115,6 +1115,41 @@ int vbd_incremental_potential(vbd_solver* h, double* potential, double* elastic)
     return VBD_OK;
 }
 
+int vbd_time_energy(vbd_solver* h, int32_t mode, int32_t reps, double* ms_per_eval, double* bytes_per_eval) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (mode != 0 && mode != 1) return fail(h, VBD_ERR_ARG, "mode must be 0 (diagnostic) or 1 (incremental potential)");
+    if (reps <= 0) return fail(h, VBD_ERR_ARG, "reps must be positive");
+    if (mode == 1 && !h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_energy_scratch(h));
+    const vbd_step_params& p = h->params;
+    const float mu = h->paramsSet ? p.mu : 1e5f, lam = h->paramsSet ? p.lambda : 4e5f;
+    const int tmode = mode == 0 ? 0 : (h->materialModel == VBD_MATERIAL_COROTATED_GPU ? 2 : 1);
+    const double sdt = h->paramsSet ? 1.0 / ((double)(p.dt / (float)p.substeps) * (p.dt / (float)p.substeps)) : 0.0;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, h->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        e = launch_energy(h->m, tmode, mu, lam, h->d_epart, h->d_energy, h->stream);
+        if (e == hipSuccess && mode == 1)
+            e = launch_inertia_energy(h->m, sdt, contact_of(p), h->d_epart, h->d_energy + 1, h->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(h, "vbd_time_energy", e);
+    if (ms_per_eval) *ms_per_eval = (double)ms / reps;
+    if (bytes_per_eval)
+        *bytes_per_eval = (56.0 + (h->m.perTetMat ? 8.0 : 0.0)) * h->m.nT + 12.0 * h->m.nV +
+                          (mode == 1 ? 28.0 * h->m.nFree : 0.0);
+    return VBD_OK;
+}
+
 // ---------------------------------------------------------------------------------------
 // differentiation entry points
 // ---------------------------------------------------------------------------------------

@@ -52,7 +52,7 @@ struct DevMesh {
     int* adjEntry = nullptr;  // 4*nT, tet*4 + corner
     int* colorList = nullptr; // nV
     // per-tet arrays kept for the energy reduction
-    i4* tets = nullptr;     // nT (caller numbering)
+    i4* tets = nullptr;     // nT corner slots (caller tet order)
     float* dminv = nullptr; // 9*nT
     float* vol = nullptr;   // nT
     float* tmu = nullptr;   // nT or null

@@ -1165,14 +1165,13 @@ hipError_t launch_gravity_load2(const DevMesh& m, const float g[3], double* r2, 
 
 // Per-tet parameter gradients -(d^2E / dx dtheta)^T w in caller tet order:
 //   dmu_t = -V (F - C) : dF(w),   dlambda_t = -V (J - 1) C : dF(w)
-__global__ __launch_bounds__(256) void k_param_grad(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
-                                                    const f4* __restrict__ pos, const f4* __restrict__ w,
+__global__ __launch_bounds__(256) void k_param_grad(int nT, const i4* __restrict__ tets, const f4* __restrict__ pos, const f4* __restrict__ w,
                                                     const float* __restrict__ dminv, const float* __restrict__ vol,
                                                     float* __restrict__ dmu, float* __restrict__ dlam) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nT) return;
     const i4 tt = tets[t];
-    const int sl[4] = {inv[tt.x], inv[tt.y], inv[tt.z], inv[tt.w]};
+    const int sl[4] = {tt.x, tt.y, tt.z, tt.w};  // device slots
     const f4 x0 = pos[sl[0]], x1 = pos[sl[1]], x2 = pos[sl[2]], x3 = pos[sl[3]];
     const f4 w0 = w[sl[0]], w1 = w[sl[1]], w2 = w[sl[2]], w3 = w[sl[3]];
     const float* D = dminv + 9 * (size_t)t;
@@ -1211,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_param_grad(int nT, const i4* __restrict
 
 hipError_t launch_param_grad(const DevMesh& m, const f4* w, float* dmu, float* dlam, hipStream_t s) {
     if (m.nT == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_param_grad, dim3((m.nT + 255) / 256), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos, w, m.dminv,
+    hipLaunchKernelGGL(k_param_grad, dim3((m.nT + 255) / 256), dim3(256), 0, s, m.nT, m.tets, m.pos, w, m.dminv,
                        m.vol, dmu, dlam);
     return hipGetLastError();
 }
@@ -1407,18 +1406,37 @@ __device__ __forceinline__ double block_sum_256(double v, double* red) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_tet_energy(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
-                                                    const f4* __restrict__ pos, const float* __restrict__ dminv,
+__global__ __launch_bounds__(256) void k_tet_energy(int nT, const i4* __restrict__ tets, const f4* __restrict__ pos,
+                                                    const float* __restrict__ dminv,
                                                     const float* __restrict__ vol, const float* __restrict__ tmu,
                                                     const float* __restrict__ tlam, float mu, float lambda,
                                                     double* __restrict__ partial) {
     __shared__ double red[256];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // The block's 256 DmInv (36 B each, not 16-B aligned per tet) are staged through LDS with
+    // 16-byte coalesced loads (the block's segment starts at a multiple of 9216 bytes).  The
+    // corner gathers are issued first and stay in flight across the LDS-only barrier.
+    __shared__ f4 sD4[256 * 9 / 4];
+    float* sD = reinterpret_cast<float*>(sD4);
+    const int tid = (int)threadIdx.x;
+    const int b0 = (int)blockIdx.x * 256;
+    const int t = b0 + tid;
+    const int nf = 9 * min(256, nT - b0);
+    const float* Dg = dminv + 9 * (size_t)b0;
+    f4 st[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (tid + 256 * k < nf / 4) st[k] = reinterpret_cast<const f4*>(Dg)[tid + 256 * k];
+    const float tail = (nf / 4) * 4 + tid < nf ? Dg[(nf / 4) * 4 + tid] : 0.f;
+    const i4 tt = tets[min(t, nT - 1)];
+    const f4 x0 = pos[tt.x], x1 = pos[tt.y], x2 = pos[tt.z], x3 = pos[tt.w];  // device slots
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (tid + 256 * k < nf / 4) sD4[tid + 256 * k] = st[k];
+    if ((nf / 4) * 4 + tid < nf) sD[(nf / 4) * 4 + tid] = tail;
+    lds_barrier();
     double e = 0.0;
     if (t < nT) {
-        const i4 tt = tets[t];
-        const f4 x0 = pos[inv[tt.x]], x1 = pos[inv[tt.y]], x2 = pos[inv[tt.z]], x3 = pos[inv[tt.w]];
-        const float* D = dminv + 9 * (size_t)t;
+        const float* D = sD + 9 * tid;
         // F = Ds DmInv in double: near rest |F|^2 - 3 and J - 1 cancel to O(strain)
         const double ds[3][3] = {{(double)x1.x - x0.x, (double)x2.x - x0.x, (double)x3.x - x0.x},
                                  {(double)x1.y - x0.y, (double)x2.y - x0.y, (double)x3.y - x0.y},
@@ -1495,13 +1513,13 @@ hipError_t launch_energy(const DevMesh& m, int mode, float mu, float lambda, dou
     const int nb = (m.nT + 255) / 256;
     if (nb == 0) return launch_sum_fixed(0, partial, out, s);
     if (mode == ENERGY_LOG_NH)
-        hipLaunchKernelGGL(k_tet_energy<ENERGY_LOG_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
+        hipLaunchKernelGGL(k_tet_energy<ENERGY_LOG_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.pos,
                            m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
     else if (mode == ENERGY_COROTATED)
-        hipLaunchKernelGGL(k_tet_energy<ENERGY_COROTATED>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
+        hipLaunchKernelGGL(k_tet_energy<ENERGY_COROTATED>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.pos,
                            m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
     else
-        hipLaunchKernelGGL(k_tet_energy<ENERGY_STABLE_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.inv, m.pos,
+        hipLaunchKernelGGL(k_tet_energy<ENERGY_STABLE_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.pos,
                            m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s);
@@ -1646,6 +1664,15 @@ __global__ __launch_bounds__(256) void k_slot_key(int n, const int* __restrict__
 __global__ __launch_bounds__(256) void k_invert(int n, const int* __restrict__ perm, int* __restrict__ inv) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) inv[perm[s]] = s;
+}
+
+// Per-tet corner ids -> device slots (the per-tet kernels then gather positions directly).
+__global__ __launch_bounds__(256) void k_tets_to_slots(int nT, const i4* __restrict__ tets, const int* __restrict__ inv,
+                                                       i4* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const i4 c = tets[t];
+    out[t] = i4{inv[c.x], inv[c.y], inv[c.z], inv[c.w]};
 }
 
 __global__ __launch_bounds__(256) void k_slot_valence(int nFree, const int* __restrict__ perm,
@@ -1927,7 +1954,8 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
     VBD_TRY(dalloc(m, &m.dminv, (size_t)9 * nT));
     VBD_TRY(dalloc(m, &m.vol, (size_t)nT));
     if (nT > 0) {
-        VBD_TRY(hipMemcpyAsync(m.tets, d_tets, sizeof(i4) * nT, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_tets_to_slots, grid_for(nT), dim3(256), 0, s, nT, d_tets, m.inv, m.tets);
+        VBD_TRY(hipGetLastError());
         VBD_TRY(hipMemcpyAsync(m.dminv, d_dminv, sizeof(float) * 9 * nT, hipMemcpyDeviceToDevice, s));
         VBD_TRY(hipMemcpyAsync(m.vol, d_vol, sizeof(float) * nT, hipMemcpyDeviceToDevice, s));
     }
