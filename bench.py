@@ -247,7 +247,8 @@ def bench_corotated(d, m, iterations: int, steps: int = 5):
     vi = float(info.num_verts) * iterations * steps
     return {"workload": "same cube, corotated material of the reference GPU kernel", "value": round(vi / secs / 1e6, 2),
             "unit": "M vertex-iters/s", "ms_per_step": round(secs * 1e3 / steps, 3), "steps": steps,
-            "kernel": "k_color_corot", "avg_launch_ms": round(per_launch_s * 1e3, 4),
+            "kernel": "k_color_tiled (corotated)" if info.lanes_per_vertex == 0 else "k_color_corot",
+            "avg_launch_ms": round(per_launch_s * 1e3, 4),
             "color_kernel_frac": round(cbytes / info.num_colors / per_launch_s / 1e9 / HBM_PEAK_GBS, 4),
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 

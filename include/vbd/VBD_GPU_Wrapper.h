@@ -6,10 +6,11 @@
 //                                                           VBD_GPU_Wrapper.cu:332-362
 //   copyPositionsToCPU(mesh)                                VBD_GPU_Wrapper.cu:364-366
 //   updateInertiaFromCPU(mesh)                              VBD_GPU_Wrapper.cu:369-371
-// Differences, by design: the element model is the CPU solver's stable Neo-Hookean
-// (Energy.cpp:90-158), not the corotated model the reference kernel calls; there is no
-// 1.000001 inertia regularisation; failures throw std::runtime_error instead of being
-// ignored; the object owns its device memory and is movable but not copyable.
+// Differences, by design: the default element model is the CPU solver's stable Neo-Hookean
+// (Energy.cpp:90-158) without the 1.000001 inertia regularisation; setMaterialModel(
+// VBD_MATERIAL_COROTATED_GPU) selects the corotated model the reference kernel calls, with
+// that regularisation.  Failures throw std::runtime_error instead of being ignored; the
+// object owns its device memory and is movable but not copyable.
 #pragma once
 
 #include <string>

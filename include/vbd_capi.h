@@ -166,8 +166,9 @@ int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
  * VBD_MATERIAL_COROTATED_GPU: what the reference GPU kernel runs (VBD_GPU_Wrapper.cu:181 with
  *   VBD_DeviceMath.cuh:127-198): corotated linear elasticity with R from 5 polar iterations,
  *   H = V [2 mu |q|^2 I + lambda (R q)(R q)^T], and the wrapper's m/h^2 * 1.000001 Hessian
- *   diagonal (.cu:208).  Runs the per-vertex kernel (lanes 8/16/32, else 16).  The
- *   differentiation entry points reject it. */
+ *   diagonal (.cu:208).  Runs the tiled kernel on the full records when the launch
+ *   configuration is tiled (lanes_per_vertex 0, the default), else the per-vertex kernel
+ *   (lanes 8/16/32, else 16).  The differentiation entry points reject it. */
 #define VBD_MATERIAL_STABLE_NEO_HOOKEAN 0
 #define VBD_MATERIAL_COROTATED_GPU 1
 int vbd_set_material_model(vbd_solver* h, int32_t model);

@@ -71,8 +71,8 @@ def test_corotated_reference_gpu_loop():
 
 @pytest.mark.parametrize("lanes", [8, 16, 32, 0])
 def test_corotated_sphere_contact_lanes(lanes):
-    """Shuffled-id sphere with penalty contact; the corotated kernel at each lane width (a tiled
-    configuration falls back to 16 lanes)."""
+    """Shuffled-id sphere with penalty contact; the per-vertex corotated kernel at each lane
+    width, and the tiled kernel on the full records (lanes 0)."""
     m = meshgen.voxel_sphere(16, seed=2)
     contact = (2, -1.05, 1e4)
     prep = O.prepare(m)
@@ -87,6 +87,31 @@ def test_corotated_sphere_contact_lanes(lanes):
         rel = max(rel, relmax(d.positions(), of[f]))
     print(f"corotated sphere lanes={lanes}: {rel:.3e}", flush=True)
     assert rel <= TOL
+
+
+def test_corotated_tiled_configurations_are_bitwise_identical():
+    """Noisy 12^3 cube, corotated material: every tiled configuration gives the same bits
+    (summation order fixed by the vertex valence), within 1e-5 of the oracle."""
+    m = meshgen.noisy_cube(12, seed=4)
+    prep = O.prepare(m)
+    frames = 4
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=20, rho=0.95, mu=m.mu, lam=m.lam,
+                                                 gravity=m.gravity, material=1), m.x0, None, frames)
+    runs = {}
+    for cfg in [(0, 128, 1), (0, 256, 1), (0, 256, 2), (0, 512, 2), (0, 64, 2)]:
+        d = upload(m, params=params(m, 20))
+        d.set_launch_config(*cfg)
+        out = []
+        for _ in range(frames):
+            d.step(1)
+            out.append(d.positions())
+        runs[cfg] = np.stack(out)
+    ref = runs[(0, 256, 1)]
+    rel = max(relmax(ref[f], of[f]) for f in range(frames))
+    print(f"corotated tiled vs oracle {rel:.3e}", flush=True)
+    assert rel <= TOL
+    for cfg, r in runs.items():
+        assert np.array_equal(r, ref), cfg
 
 
 def test_corotated_batch_with_per_tet_materials():

@@ -177,15 +177,18 @@ bool finite_array(const float* a, size_t n) {
 }
 
 // One color launch with the handle's configuration: lpv > 0 selects the per-vertex lane-group
-// kernel, lpv == 0 the tiled kernel on compact records (unroll = records per thread).
+// kernel, lpv == 0 the tiled kernel (unroll = records per thread; compact records for the
+// stable Neo-Hookean material, full records for the corotated one).
 hipError_t launch_color_cfg(vbd_solver* h, int c, float sdt, float mu, float lambda, const ContactArgs& ct, f4* save) {
-    if (h->materialModel == VBD_MATERIAL_COROTATED_GPU) {
+    const bool corot = h->materialModel == VBD_MATERIAL_COROTATED_GPU;
+    if (h->lpv == 0)
+        return launch_color_tiled(h->m, c, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream, corot);
+    if (corot) {
         // per-vertex lane groups on the full records; the configured lanes when 8/16/32
         const bool own = h->lpv == 8 || h->lpv == 16 || h->lpv == 32;
         return launch_color_corotated(h->m, c, own ? h->lpv : 16, own ? h->block : 256, sdt, mu, lambda, ct, save,
                                       h->stream);
     }
-    if (h->lpv == 0) return launch_color_tiled(h->m, c, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
     return launch_color(h->m, c, h->lpv, h->unroll, h->block, sdt, mu, lambda, ct, save, h->stream);
 }
 
@@ -1008,8 +1011,9 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
         for (int b : cand_b) {
             if (e != hipSuccess) break;
             if (cd.l > 0 && (b > 256 || b % cd.l)) continue;
-            // the corotated model runs the per-vertex kernel with lanes 8/16/32 only
-            if (h->materialModel == VBD_MATERIAL_COROTATED_GPU && (cd.u != 1 || (cd.l != 8 && cd.l != 16 && cd.l != 32)))
+            // the corotated model runs the tiled kernel or the per-vertex kernel with lanes 8/16/32
+            if (h->materialModel == VBD_MATERIAL_COROTATED_GPU && cd.l != 0 &&
+                (cd.u != 1 || (cd.l != 8 && cd.l != 16 && cd.l != 32)))
                 continue;
             if (cd.l == 0) {
                 if (b < 128 || !tiled_config_supported(cd.u, b) || !h->m.tiledOk || h->m.maxValence > cd.u * b)

@@ -89,8 +89,9 @@ hipError_t launch_color_corotated(const DevMesh& m, int color, int lpv, int bloc
 // tiled variant: per_thread records per thread, tiles of per_thread*block records and fewer than
 // block/8 vertices (build_tiles first)
 bool tiled_config_supported(int per_thread, int block);
+// corotated: the reference GPU kernel's material on the full records (same tiles)
 hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int block, float sdt, float mu,
-                              float lambda, const ContactArgs& ct, f4* save, hipStream_t s);
+                              float lambda, const ContactArgs& ct, f4* save, hipStream_t s, bool corotated = false);
 // tile cut for the tiled kernel with (per_thread, block): records = per_thread * block,
 // max_verts = block / 8 - 1
 hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err);

@@ -307,6 +307,98 @@ __device__ __forceinline__ f4 vertex_solve(const f4 xs, const f4 ys, const float
 }
 
 // ---------------------------------------------------------------------------------------
+// Corotated linear elasticity of the reference GPU kernel (VBD_GPU_Wrapper.cu:127-233 with
+// VBD_DeviceMath.cuh:127-198), on the full records:
+//   R = 5 steps of R <- (R + cof(R) / det R) / 2 from R = F (stop if |det R| < 1e-8),
+//   P = 2 mu (F - R) + lambda tr(R^T F - I) R,   g = V P q,
+//   H = V [2 mu |q|^2 I + lambda (R q)(R q)^T]   (dR/dF dropped),
+// and the wrapper's vertex update: H diagonal m/h^2 * 1.000001, force (m/h^2)(y - x)
+// (VBD_GPU_Wrapper.cu:203-213), adjugate solve skipped when |det H| < 1e-8.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void corot_accumulate(const bool corner0, const f4 a, const f4 b, const float c,
+                                                 const float V, const float tmu, const float tlam, const f4 xs,
+                                                 const f4 ya, const f4 yb, const f4 yc, Acc& A) {
+#pragma clang fp contract(off)
+    const Kin k = pair_kinematics(corner0, a, b, c, xs, ya, yb, yc);
+    // the first Higham step reuses cof(F)
+    float R00 = k.F00, R01 = k.F01, R02 = k.F02, R10 = k.F10, R11 = k.F11, R12 = k.F12;
+    float R20 = k.F20, R21 = k.F21, R22 = k.F22;
+    float C00 = k.C00, C01 = k.C01, C02 = k.C02, C10 = k.C10, C11 = k.C11, C12 = k.C12;
+    float C20 = k.C20, C21 = k.C21, C22 = k.C22;
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        if (it > 0) {
+            C00 = R11 * R22 - R12 * R21; C01 = R12 * R20 - R10 * R22; C02 = R10 * R21 - R11 * R20;
+            C10 = R21 * R02 - R22 * R01; C11 = R22 * R00 - R20 * R02; C12 = R20 * R01 - R21 * R00;
+            C20 = R01 * R12 - R02 * R11; C21 = R02 * R10 - R00 * R12; C22 = R00 * R11 - R01 * R10;
+        }
+        const float det = R00 * (R11 * R22 - R12 * R21) - R01 * (R10 * R22 - R12 * R20) + R02 * (R10 * R21 - R11 * R20);
+        if (fabsf(det) < 1e-8f) break;
+        const float inv = 1.0f / det;
+        R00 = 0.5f * (R00 + C00 * inv); R01 = 0.5f * (R01 + C01 * inv); R02 = 0.5f * (R02 + C02 * inv);
+        R10 = 0.5f * (R10 + C10 * inv); R11 = 0.5f * (R11 + C11 * inv); R12 = 0.5f * (R12 + C12 * inv);
+        R20 = 0.5f * (R20 + C20 * inv); R21 = 0.5f * (R21 + C21 * inv); R22 = 0.5f * (R22 + C22 * inv);
+    }
+    const float tr = R00 * k.F00 + R01 * k.F01 + R02 * k.F02 + R10 * k.F10 + R11 * k.F11 + R12 * k.F12 +
+                     R20 * k.F20 + R21 * k.F21 + R22 * k.F22 - 3.0f;
+    const float m2 = 2.0f * tmu, lt = tlam * tr;
+    const float q0 = k.q0, q1 = k.q1, q2 = k.q2;
+    const float P00 = m2 * (k.F00 - R00) + lt * R00, P01 = m2 * (k.F01 - R01) + lt * R01;
+    const float P02 = m2 * (k.F02 - R02) + lt * R02, P10 = m2 * (k.F10 - R10) + lt * R10;
+    const float P11 = m2 * (k.F11 - R11) + lt * R11, P12 = m2 * (k.F12 - R12) + lt * R12;
+    const float P20 = m2 * (k.F20 - R20) + lt * R20, P21 = m2 * (k.F21 - R21) + lt * R21;
+    const float P22 = m2 * (k.F22 - R22) + lt * R22;
+    const float rq0 = R00 * q0 + R01 * q1 + R02 * q2;
+    const float rq1 = R10 * q0 + R11 * q1 + R12 * q2;
+    const float rq2 = R20 * q0 + R21 * q1 + R22 * q2;
+    const float d = m2 * (q0 * q0 + q1 * q1 + q2 * q2);
+    A.h00 += V * (d + tlam * rq0 * rq0);
+    A.h01 += V * (tlam * rq0 * rq1);
+    A.h02 += V * (tlam * rq0 * rq2);
+    A.h11 += V * (d + tlam * rq1 * rq1);
+    A.h12 += V * (tlam * rq1 * rq2);
+    A.h22 += V * (d + tlam * rq2 * rq2);
+    A.r0 -= V * (P00 * q0 + P01 * q1 + P02 * q2);
+    A.r1 -= V * (P10 * q0 + P11 * q1 + P12 * q2);
+    A.r2 -= V * (P20 * q0 + P21 * q1 + P22 * q2);
+}
+
+// The wrapper's vertex update (VBD_GPU_Wrapper.cu:203-213): H diagonal + m/h^2 * 1.000001,
+// force + (m/h^2)(y - x), penalty contact, adjugate solve unless |det H| < 1e-8.
+template <bool CONTACT>
+__device__ __forceinline__ f4 corot_vertex_solve(const f4 xs, const f4 ys, const float m, float h00, float h01,
+                                                 float h02, float h11, float h12, float h22, float r0, float r1,
+                                                 float r2, float sdt, const ContactArgs& ct) {
+#pragma clang fp contract(off)
+    const float coeff = m * sdt;
+    const float reg = coeff * 1.000001f;
+    h00 += reg; h11 += reg; h22 += reg;
+    r0 += coeff * (ys.x - xs.x);
+    r1 += coeff * (ys.y - xs.y);
+    r2 += coeff * (ys.z - xs.z);
+    if constexpr (CONTACT) {
+        const float xa = ct.axis == 0 ? xs.x : (ct.axis == 1 ? xs.y : xs.z);
+        const float dd = ct.plane - xa;
+        if (dd > 0.f) {
+            if (ct.axis == 0) { h00 += ct.k; r0 += ct.k * dd; }
+            else if (ct.axis == 1) { h11 += ct.k; r1 += ct.k * dd; }
+            else { h22 += ct.k; r2 += ct.k * dd; }
+        }
+    }
+    const float det = h00 * (h11 * h22 - h12 * h12) - h01 * (h01 * h22 - h12 * h02) + h02 * (h01 * h12 - h11 * h02);
+    f4 xn = xs;
+    if (!(fabsf(det) < 1e-8f)) {
+        const float inv = 1.0f / det;
+        const float a00 = h11 * h22 - h12 * h12, a01 = h02 * h12 - h01 * h22, a02 = h01 * h12 - h02 * h11;
+        const float a11 = h00 * h22 - h02 * h02, a12 = h02 * h01 - h00 * h12, a22 = h00 * h11 - h01 * h01;
+        xn.x = xs.x + inv * (a00 * r0 + a01 * r1 + a02 * r2);
+        xn.y = xs.y + inv * (a01 * r0 + a11 * r1 + a12 * r2);
+        xn.z = xs.z + inv * (a02 * r0 + a12 * r1 + a22 * r2);
+    }
+    return xn;
+}
+
+// ---------------------------------------------------------------------------------------
 // Tiled per-color vertex solve on compact records.
 //
 // Per (vertex s, tet) record the closed-form block needs only F q, C q and J (q = the
@@ -347,12 +439,14 @@ constexpr int BUF_NT = 2;  // nontemporal cache policy bit of the raw buffer loa
 template <int K, bool MAT>
 struct RecRegs {
     i4 t[K];
-    f4 q[K];
+    f4 q[K];     // compact coefficients (stable Neo-Hookean)
+    f4 a[K], b[K];  // full DmInv record (corotated): a, b, c as in the per-vertex kernels
+    float c[K];
     f2 mm[K];
 };
 
 struct RecSrc {  // buffer resources over one color's records (byte offsets from its first record)
-    __amdgpu_buffer_rsrc_t t, q, mm;
+    __amdgpu_buffer_rsrc_t t, q, a, b, c, mm;
 };
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const void* base, int64_t bytes) {
@@ -361,13 +455,19 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const void* base, int
 
 // Loads are unconditional (index clamped to the color's last record): no branch around
 // them, so the compiler's wait counts can keep the gathers in flight with the vertex loads.
-template <int K, bool MAT>
+template <int K, bool MAT, bool COROT>
 __device__ __forceinline__ void load_records(RecRegs<K, MAT>& R, const RecSrc& src, int r0, int pmax, int tid, int T) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = min(r0 + tid + k * T, pmax);  // relative to the color's first record
         R.t[k] = __builtin_bit_cast(i4, __builtin_amdgcn_raw_buffer_load_b128(src.t, p << 4, 0, BUF_NT));
-        R.q[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.q, p << 4, 0, BUF_NT));
+        if constexpr (COROT) {
+            R.a[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.a, p << 4, 0, BUF_NT));
+            R.b[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.b, p << 4, 0, BUF_NT));
+            R.c[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(src.c, p << 2, 0, BUF_NT));
+        } else {
+            R.q[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.q, p << 4, 0, BUF_NT));
+        }
         if constexpr (MAT)
             R.mm[k] = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(src.mm, p << 3, 0, BUF_NT));
     }
@@ -422,14 +522,18 @@ __device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, 
     return A;
 }
 
-template <int K, bool MAT, bool CONTACT, bool SAVE>
+// COROT: the corotated material of the reference GPU kernel on the full records (pa, pb, pc)
+// instead of the compact stable Neo-Hookean records (pq); same tiles, reduction and order.
+template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
 __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, const int* __restrict__ tileRec,
                                                      int ntiles, int nslots, f4* __restrict__ pos,
                                                      const f4* __restrict__ inert, const float* __restrict__ mass,
                                                      const int* __restrict__ row, const int* __restrict__ chunkBase,
                                                      const i4* __restrict__ ptet, const f4* __restrict__ pq,
-                                                     const f2* __restrict__ pmat, f4* __restrict__ save, float mu,
-                                                     float lambda, float sdt, ContactArgs ct) {
+                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
+                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
+                                                     ContactArgs ct) {
     // LDS: 9 * RT contribution values, then VM = T/TG entries (a tile has at most VM - 1
     // vertices): row offsets (nv + 1 of them), x, y, m
     extern __shared__ float red[];
@@ -451,12 +555,18 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     const int pmax = nrec - 1;
     RecSrc src;
     src.t = rec_rsrc(ptet + rec0, 16 * (int64_t)nrec);
-    src.q = rec_rsrc(pq + rec0, 16 * (int64_t)nrec);
-    src.mm = MAT ? rec_rsrc(pmat + rec0, 8 * (int64_t)nrec) : src.q;
+    if constexpr (COROT) {
+        src.a = rec_rsrc(pa + rec0, 16 * (int64_t)nrec);
+        src.b = rec_rsrc(pb + rec0, 16 * (int64_t)nrec);
+        src.c = rec_rsrc(pc + rec0, 4 * (int64_t)nrec);
+    } else {
+        src.q = rec_rsrc(pq + rec0, 16 * (int64_t)nrec);
+    }
+    src.mm = MAT ? rec_rsrc(pmat + rec0, 8 * (int64_t)nrec) : src.t;
 
     const int r0 = tileRec[tile] - rec0, r1 = tileRec[tile + 1] - rec0;
     RecRegs<K, MAT> R;
-    load_records<K, MAT>(R, src, r0, pmax, tid, T);
+    load_records<K, MAT, COROT>(R, src, r0, pmax, tid, T);
     // owner bases of the (at most two) 4096-record chunks the tile touches
     const int chunkA = (rec0 + r0) >> CHUNK_SHIFT;
     const int cbA = chunkBase[chunkA], cbB = chunkBase[(rec0 + max(r1 - 1, r0)) >> CHUNK_SHIFT];
@@ -486,8 +596,14 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
             tmu = R.mm[k].x;
             tlam = R.mm[k].y;
         }
-        const Acc A = compact_contrib(R.t[k].x < 0, R.q[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k], ya[k], yb[k],
-                                      yc[k]);
+        Acc A;
+        if constexpr (COROT) {
+            A = Acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            corot_accumulate(R.t[k].x < 0, R.a[k], R.b[k], R.c[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k], ya[k],
+                             yb[k], yc[k], A);
+        } else {
+            A = compact_contrib(R.t[k].x < 0, R.q[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k], ya[k], yb[k], yc[k]);
+        }
         if (r0 + q < r1) {
             red[0 * RT + q] = A.h00;
             red[1 * RT + q] = A.h01;
@@ -532,13 +648,17 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     s6 = gsum<TG>(s6); s7 = gsum<TG>(s7); s8 = gsum<TG>(s8);
     if (ok && lane == TG - 1) {
         const f4 x = lx[i];
-        const f4 xn = vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
+        f4 xn;
+        if constexpr (COROT)
+            xn = corot_vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
+        else
+            xn = vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
         if constexpr (SAVE) save[v0 + i] = x;
         pos[v0 + i] = xn;
     }
 }
 
-template <int K, bool MAT, bool CONTACT, bool SAVE>
+template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
 static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
                                  const ContactArgs& ct, f4* save, hipStream_t s) {
     const int ntiles = m.colorTileCount[color];
@@ -546,19 +666,20 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
     const size_t lds = sizeof(float) * 9 * (size_t)K * block +
                        (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / TG);
     const int per = (ntiles + 7) / 8;
-    hipLaunchKernelGGL((k_color_tiled<K, MAT, CONTACT, SAVE>), dim3(8 * per), dim3(block), lds, s,
+    hipLaunchKernelGGL((k_color_tiled<K, MAT, CONTACT, SAVE, COROT>), dim3(8 * per), dim3(block), lds, s,
                        m.tileStart + m.colorTileBase[color], m.tileRec + m.colorTileBase[color], ntiles, m.nV, m.pos,
-                       m.inert, m.mass, m.row, m.chunkBase, m.ptet, m.pq, m.pmat, save, mu, lambda, sdt, ct);
+                       m.inert, m.mass, m.row, m.chunkBase, m.ptet, m.pq, m.pa, m.pb, m.pc, m.pmat, save, mu, lambda,
+                       sdt, ct);
     return hipGetLastError();
 }
 
-template <int K>
+template <int K, bool COROT>
 static hipError_t launch_tiled_k(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
                                  const ContactArgs& ct, f4* save, hipStream_t s) {
     const bool mat = m.perTetMat, con = ct.on != 0, sv = save != nullptr;
 #define VBD_DISPATCH(MT, CT, SV)                                                                                   \
     if (mat == MT && con == CT && sv == SV)                                                                        \
-        return launch_tiled_t<K, MT, CT, SV>(m, color, block, sdt, mu, lambda, ct, save, s);
+        return launch_tiled_t<K, MT, CT, SV, COROT>(m, color, block, sdt, mu, lambda, ct, save, s);
     VBD_DISPATCH(false, false, false) VBD_DISPATCH(false, false, true)
     VBD_DISPATCH(false, true, false) VBD_DISPATCH(false, true, true)
     VBD_DISPATCH(true, false, false) VBD_DISPATCH(true, false, true)
@@ -573,12 +694,15 @@ bool tiled_config_supported(int per_thread, int block) {
 }
 
 hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int block, float sdt, float mu,
-                              float lambda, const ContactArgs& ct, f4* save, hipStream_t s) {
+                              float lambda, const ContactArgs& ct, f4* save, hipStream_t s, bool corotated) {
     if (!tiled_config_supported(per_thread, block) || m.tileRecords != per_thread * block ||
         m.tileVerts != block / TG - 1 || !m.tileStart)
         return hipErrorInvalidValue;
-    return per_thread == 1 ? launch_tiled_k<1>(m, color, block, sdt, mu, lambda, ct, save, s)
-                           : launch_tiled_k<2>(m, color, block, sdt, mu, lambda, ct, save, s);
+    if (corotated)
+        return per_thread == 1 ? launch_tiled_k<1, true>(m, color, block, sdt, mu, lambda, ct, save, s)
+                               : launch_tiled_k<2, true>(m, color, block, sdt, mu, lambda, ct, save, s);
+    return per_thread == 1 ? launch_tiled_k<1, false>(m, color, block, sdt, mu, lambda, ct, save, s)
+                           : launch_tiled_k<2, false>(m, color, block, sdt, mu, lambda, ct, save, s);
 }
 
 // Greedy cut of every color's slot range into tiles of at most `records` records and
@@ -699,62 +823,6 @@ hipError_t launch_color(const DevMesh& m, int color, int lpv, int unroll, int bl
     return hipErrorInvalidValue;
 }
 
-// ---------------------------------------------------------------------------------------
-// Corotated linear elasticity of the reference GPU kernel (VBD_GPU_Wrapper.cu:127-233 with
-// VBD_DeviceMath.cuh:127-198), on the full records:
-//   R = 5 steps of R <- (R + cof(R) / det R) / 2 from R = F (stop if |det R| < 1e-8),
-//   P = 2 mu (F - R) + lambda tr(R^T F - I) R,   g = V P q,
-//   H = V [2 mu |q|^2 I + lambda (R q)(R q)^T]   (dR/dF dropped),
-// and the wrapper's vertex update: H diagonal m/h^2 * 1.000001, force (m/h^2)(y - x)
-// (VBD_GPU_Wrapper.cu:203-213), adjugate solve skipped when |det H| < 1e-8.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void corot_accumulate(const bool corner0, const f4 a, const f4 b, const float c,
-                                                 const float V, const float tmu, const float tlam, const f4 xs,
-                                                 const f4 ya, const f4 yb, const f4 yc, Acc& A) {
-    const Kin k = pair_kinematics(corner0, a, b, c, xs, ya, yb, yc);
-    // the first Higham step reuses cof(F)
-    float R00 = k.F00, R01 = k.F01, R02 = k.F02, R10 = k.F10, R11 = k.F11, R12 = k.F12;
-    float R20 = k.F20, R21 = k.F21, R22 = k.F22;
-    float C00 = k.C00, C01 = k.C01, C02 = k.C02, C10 = k.C10, C11 = k.C11, C12 = k.C12;
-    float C20 = k.C20, C21 = k.C21, C22 = k.C22;
-#pragma unroll
-    for (int it = 0; it < 5; ++it) {
-        if (it > 0) {
-            C00 = R11 * R22 - R12 * R21; C01 = R12 * R20 - R10 * R22; C02 = R10 * R21 - R11 * R20;
-            C10 = R21 * R02 - R22 * R01; C11 = R22 * R00 - R20 * R02; C12 = R20 * R01 - R21 * R00;
-            C20 = R01 * R12 - R02 * R11; C21 = R02 * R10 - R00 * R12; C22 = R00 * R11 - R01 * R10;
-        }
-        const float det = R00 * (R11 * R22 - R12 * R21) - R01 * (R10 * R22 - R12 * R20) + R02 * (R10 * R21 - R11 * R20);
-        if (fabsf(det) < 1e-8f) break;
-        const float inv = 1.0f / det;
-        R00 = 0.5f * (R00 + C00 * inv); R01 = 0.5f * (R01 + C01 * inv); R02 = 0.5f * (R02 + C02 * inv);
-        R10 = 0.5f * (R10 + C10 * inv); R11 = 0.5f * (R11 + C11 * inv); R12 = 0.5f * (R12 + C12 * inv);
-        R20 = 0.5f * (R20 + C20 * inv); R21 = 0.5f * (R21 + C21 * inv); R22 = 0.5f * (R22 + C22 * inv);
-    }
-    const float tr = R00 * k.F00 + R01 * k.F01 + R02 * k.F02 + R10 * k.F10 + R11 * k.F11 + R12 * k.F12 +
-                     R20 * k.F20 + R21 * k.F21 + R22 * k.F22 - 3.0f;
-    const float m2 = 2.0f * tmu, lt = tlam * tr;
-    const float q0 = k.q0, q1 = k.q1, q2 = k.q2;
-    const float P00 = m2 * (k.F00 - R00) + lt * R00, P01 = m2 * (k.F01 - R01) + lt * R01;
-    const float P02 = m2 * (k.F02 - R02) + lt * R02, P10 = m2 * (k.F10 - R10) + lt * R10;
-    const float P11 = m2 * (k.F11 - R11) + lt * R11, P12 = m2 * (k.F12 - R12) + lt * R12;
-    const float P20 = m2 * (k.F20 - R20) + lt * R20, P21 = m2 * (k.F21 - R21) + lt * R21;
-    const float P22 = m2 * (k.F22 - R22) + lt * R22;
-    const float rq0 = R00 * q0 + R01 * q1 + R02 * q2;
-    const float rq1 = R10 * q0 + R11 * q1 + R12 * q2;
-    const float rq2 = R20 * q0 + R21 * q1 + R22 * q2;
-    const float d = m2 * (q0 * q0 + q1 * q1 + q2 * q2);
-    A.h00 += V * (d + tlam * rq0 * rq0);
-    A.h01 += V * (tlam * rq0 * rq1);
-    A.h02 += V * (tlam * rq0 * rq2);
-    A.h11 += V * (d + tlam * rq1 * rq1);
-    A.h12 += V * (tlam * rq1 * rq2);
-    A.h22 += V * (d + tlam * rq2 * rq2);
-    A.r0 -= V * (P00 * q0 + P01 * q1 + P02 * q2);
-    A.r1 -= V * (P10 * q0 + P11 * q1 + P12 * q2);
-    A.r2 -= V * (P20 * q0 + P21 * q1 + P22 * q2);
-}
-
 template <int LPV, bool MAT, bool CONTACT, bool SAVE>
 __global__ __launch_bounds__(256) void k_color_corot(int slot0, int count, int nslots, f4* __restrict__ pos,
                                                      const f4* __restrict__ inert, const float* __restrict__ mass,
@@ -794,32 +862,7 @@ __global__ __launch_bounds__(256) void k_color_corot(int slot0, int count, int n
     float h11 = gsum<LPV>(A.h11), h12 = gsum<LPV>(A.h12), h22 = gsum<LPV>(A.h22);
     float r0 = gsum<LPV>(A.r0), r1 = gsum<LPV>(A.r1), r2 = gsum<LPV>(A.r2);
     if (!valid || lane != LPV - 1) return;
-    const float coeff = mass[s] * sdt;
-    const f4 ys = inert[s];
-    const float reg = coeff * 1.000001f;
-    h00 += reg; h11 += reg; h22 += reg;
-    r0 += coeff * (ys.x - xs.x);
-    r1 += coeff * (ys.y - xs.y);
-    r2 += coeff * (ys.z - xs.z);
-    if constexpr (CONTACT) {
-        const float xa = ct.axis == 0 ? xs.x : (ct.axis == 1 ? xs.y : xs.z);
-        const float dd = ct.plane - xa;
-        if (dd > 0.f) {
-            if (ct.axis == 0) { h00 += ct.k; r0 += ct.k * dd; }
-            else if (ct.axis == 1) { h11 += ct.k; r1 += ct.k * dd; }
-            else { h22 += ct.k; r2 += ct.k * dd; }
-        }
-    }
-    const float det = h00 * (h11 * h22 - h12 * h12) - h01 * (h01 * h22 - h12 * h02) + h02 * (h01 * h12 - h11 * h02);
-    f4 xn = xs;
-    if (!(fabsf(det) < 1e-8f)) {
-        const float inv = 1.0f / det;
-        const float a00 = h11 * h22 - h12 * h12, a01 = h02 * h12 - h01 * h22, a02 = h01 * h12 - h02 * h11;
-        const float a11 = h00 * h22 - h02 * h02, a12 = h02 * h01 - h00 * h12, a22 = h00 * h11 - h01 * h01;
-        xn.x = xs.x + inv * (a00 * r0 + a01 * r1 + a02 * r2);
-        xn.y = xs.y + inv * (a01 * r0 + a11 * r1 + a12 * r2);
-        xn.z = xs.z + inv * (a02 * r0 + a12 * r1 + a22 * r2);
-    }
+    const f4 xn = corot_vertex_solve<CONTACT>(xs, inert[s], mass[s], h00, h01, h02, h11, h12, h22, r0, r1, r2, sdt, ct);
     if constexpr (SAVE) save[s] = xs;
     pos[s] = xn;
 }
