@@ -413,12 +413,15 @@ def _potential_fp64(m, x, y, mass, contact=None):
     return el + ine, el
 
 
-@pytest.mark.parametrize("gen", ["cube", "sphere_contact"])
+@pytest.mark.parametrize("gen", ["cube", "sphere_contact", "cube90"])
 def test_incremental_potential_matches_fp64(gen):
     """G = inertia + stable NH elastic (+ penalty) on the device (double, fixed order) vs float64
-    numpy with the FP64 oracle's DmInv and volumes (1e-5 relative)."""
+    numpy with the FP64 oracle's DmInv and volumes (1e-5 relative).  cube90 (4.37M tets) has
+    more than 16384 per-block partials, so its final sums take the two-stage path."""
     if gen == "cube":
         m, contact = meshgen.noisy_cube(10, seed=2), None
+    elif gen == "cube90":
+        m, contact = meshgen.noisy_cube(90, seed=2), None
     else:
         m, contact = meshgen.voxel_sphere(14, seed=1), (2, -0.9, 1e4)
     rng = np.random.default_rng(7)
@@ -435,6 +438,21 @@ def test_incremental_potential_matches_fp64(gen):
     assert d.incremental_potential() == (G, el)  # fixed summation order: bit-reproducible
     e1 = d.total_energy(m.mu, m.lam)
     assert d.total_energy(m.mu, m.lam) == e1
+
+
+def test_two_stage_residual_sum_matches_numpy():
+    """|M g| over 35k free vertices (the two-stage fixed-order sum) vs float64 numpy."""
+    m = meshgen.noisy_cube(32, seed=3)
+    t, _, _, _, _ = prepare_host(m)
+    d = upload(m, params=params_for(m, 10))
+    assert d.info().num_free > 16384
+    _, load = d.residual_norm()
+    fr = m.fixed == 0
+    g2 = sum(float(np.float32(c)) ** 2 for c in m.gravity)
+    ref = np.sqrt(np.sum(t.vertexMass[fr].astype(np.float64) ** 2) * g2)
+    print(f"|Mg| {load:.12e} vs {ref:.12e}", flush=True)
+    assert abs(load - ref) <= 1e-12 * ref
+    assert d.residual_norm()[1] == load
 
 
 def test_sweeps_lower_the_incremental_potential():

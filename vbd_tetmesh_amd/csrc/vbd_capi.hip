@@ -314,7 +314,7 @@ int heuristic_lpv(const DevMesh& m) {
 
 hipError_t ensure_energy_scratch(vbd_solver* h) {
     if (h->d_epart) return hipSuccess;
-    return hipMalloc(&h->d_epart, sizeof(double) * (size_t)energy_partials(h->m));
+    return hipMalloc(&h->d_epart, sizeof(double) * ((size_t)energy_partials(h->m) + SUM_STAGE));
 }
 
 // Default launch configuration: the tiled kernel with the smallest tile (block 256 with 1 or 2
@@ -1159,6 +1159,9 @@ int vbd_time_energy(vbd_solver* h, int32_t mode, int32_t reps, double* ms_per_ev
 // ---------------------------------------------------------------------------------------
 namespace {
 
+// d_r2 holds max(nV, 1) per-vertex values, then SUM_STAGE doubles for the two-stage sum
+double* r2_stage(vbd_solver* h) { return h->d_r2 + (h->m.nV > 0 ? h->m.nV : 1); }
+
 hipError_t ensure_scratch(vbd_solver* h) {
     const size_t n = (size_t)(h->m.nV > 0 ? h->m.nV : 1);
     const size_t t = (size_t)(h->m.nT > 0 ? h->m.nT : 1);
@@ -1168,7 +1171,7 @@ hipError_t ensure_scratch(vbd_solver* h) {
     if (!h->d_out && (e = hipMalloc(&h->d_out, sizeof(f4) * n)) != hipSuccess) return e;
     if (!h->d_wsave[0] && (e = hipMalloc(&h->d_wsave[0], sizeof(f4) * n)) != hipSuccess) return e;
     if (!h->d_wsave[1] && (e = hipMalloc(&h->d_wsave[1], sizeof(f4) * n)) != hipSuccess) return e;
-    if (!h->d_r2 && (e = hipMalloc(&h->d_r2, sizeof(double) * n)) != hipSuccess) return e;
+    if (!h->d_r2 && (e = hipMalloc(&h->d_r2, sizeof(double) * (n + SUM_STAGE))) != hipSuccess) return e;
     if (!h->d_sum && (e = hipMalloc(&h->d_sum, sizeof(double) * 2)) != hipSuccess) return e;
     if (!h->d_tgrad && (e = hipMalloc(&h->d_tgrad, sizeof(float) * 2 * t)) != hipSuccess) return e;
     if (!h->d_x3 && (e = hipMalloc(&h->d_x3, sizeof(float) * 3 * n)) != hipSuccess) return e;
@@ -1197,7 +1200,7 @@ hipError_t download_vec(vbd_solver* h, const f4* src, float* dst) {
 
 // sum of the first n entries of d_r2 -> host (deterministic)
 hipError_t sum_r2(vbd_solver* h, int n, double* out) {
-    hipError_t e = launch_sum_fixed(n, h->d_r2, h->d_sum, h->stream);
+    hipError_t e = launch_sum_fixed(n, h->d_r2, h->d_sum, h->stream, r2_stage(h));
     if (e == hipSuccess) e = hipMemcpyAsync(out, h->d_sum, sizeof(double), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     return e;
@@ -1217,7 +1220,7 @@ hipError_t forward_residual(vbd_solver* h, double* norm2) {
 
 // Enqueues |r|^2 of the per-vertex residual already in d_r2 into the pinned h_res[0] (no wait).
 hipError_t enqueue_sum_to_host(vbd_solver* h) {
-    hipError_t e = launch_sum_fixed(h->m.nFree, h->d_r2, h->d_sum, h->stream);
+    hipError_t e = launch_sum_fixed(h->m.nFree, h->d_r2, h->d_sum, h->stream, r2_stage(h));
     if (e == hipSuccess) e = hipMemcpyAsync(h->h_res, h->d_sum, sizeof(double), hipMemcpyDeviceToHost, h->stream);
     return e;
 }

@@ -101,7 +101,8 @@ hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t
 hipError_t launch_copy_inertia(const DevMesh& m, hipStream_t s);
 hipError_t launch_scatter_pos(int n, const float* src3, const int* inv, f4* dst, hipStream_t s);
 hipError_t launch_gather_pos(int n, const f4* src, const int* inv, float* dst3, hipStream_t s);
-// energies in double, fixed summation order; partial holds energy_partials(m) doubles
+// energies in double, fixed summation order; partial holds energy_partials(m) + SUM_STAGE
+// doubles (the tail is the second stage of the final sum)
 // mode 0: the reference's log-NH diagnostic, 1: the stable NH (J-1) elastic energy,
 // 2: the corotated model's energy
 int energy_partials(const DevMesh& m);
@@ -114,7 +115,12 @@ hipError_t launch_copy_f4(int n, const f4* src, f4* dst, hipStream_t s);
 hipError_t launch_pass(int mode, const DevMesh& m, int slot0, int count, int lpv, int block, f4* w, const f4* rhs,
                        f4* out, double* r2, f4* save, float mu, float lambda, float sdt, const ContactArgs& ct,
                        hipStream_t s);
-hipError_t launch_sum_fixed(int n, const double* v, double* out, hipStream_t s);
+// Sum of v[0, n) into out[0] in a fixed order (bit-reproducible).  Above SUM_TWO_STAGE
+// entries, with stage (SUM_STAGE doubles) given: SUM_STAGE blocks each sum one contiguous
+// chunk, then one block sums the chunk sums; otherwise one block strides the whole range.
+constexpr int SUM_STAGE = 256;
+constexpr int SUM_TWO_STAGE = 16384;
+hipError_t launch_sum_fixed(int n, const double* v, double* out, hipStream_t s, double* stage = nullptr);
 hipError_t launch_norm2_f4(int n, const f4* v, double* r2, hipStream_t s);
 hipError_t launch_gravity_load2(const DevMesh& m, const float g[3], double* r2, hipStream_t s);
 hipError_t launch_param_grad(const DevMesh& m, const f4* w, float* dmu, float* dlam, hipStream_t s);

@@ -1185,7 +1185,30 @@ hipError_t launch_norm2_f4(int n, const f4* v, double* r2, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_sum_fixed(int n, const double* v, double* out, hipStream_t s) {
+// Stage one of the two-stage fixed-order sum: block b sums chunk b of ceil(n / SUM_STAGE).
+__global__ __launch_bounds__(256) void k_sum_stage(int n, const double* __restrict__ v, double* __restrict__ stage) {
+    __shared__ double sh[256];
+    const int chunk = (n + SUM_STAGE - 1) / SUM_STAGE;
+    const int b0 = (int)blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+    double acc = 0.0;
+    for (int i = b0 + (int)threadIdx.x; i < b1; i += 256) acc += v[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) stage[blockIdx.x] = sh[0];
+}
+
+hipError_t launch_sum_fixed(int n, const double* v, double* out, hipStream_t s, double* stage) {
+    if (stage && n > SUM_TWO_STAGE) {
+        hipLaunchKernelGGL(k_sum_stage, dim3(SUM_STAGE), dim3(256), 0, s, n, v, stage);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, s, SUM_STAGE, stage, out);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, s, n, v, out);
     return hipGetLastError();
 }
@@ -1565,7 +1588,7 @@ hipError_t launch_energy(const DevMesh& m, int mode, float mu, float lambda, dou
         hipLaunchKernelGGL(k_tet_energy<ENERGY_STABLE_NH>, dim3(nb), dim3(256), 0, s, m.nT, m.tets, m.pos,
                            m.dminv, m.vol, m.tmu, m.tlam, mu, lambda, partial);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s);
+    return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s, partial + energy_partials(m));
 }
 
 hipError_t launch_inertia_energy(const DevMesh& m, double sdt, const ContactArgs& ct, double* partial, double* out,
@@ -1574,7 +1597,7 @@ hipError_t launch_inertia_energy(const DevMesh& m, double sdt, const ContactArgs
     if (nb == 0) return launch_sum_fixed(0, partial, out, s);
     hipLaunchKernelGGL(k_inertia_energy, dim3(nb), dim3(256), 0, s, m.nFree, m.pos, m.inert, m.mass, sdt, ct, partial);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s);
+    return e != hipSuccess ? e : launch_sum_fixed(nb, partial, out, s, partial + energy_partials(m));
 }
 
 // ---------------------------------------------------------------------------------------
