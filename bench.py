@@ -218,7 +218,7 @@ def bench_energy(d, reps: int = 20):
     return res
 
 
-def bench_corotated(d, m, iterations: int, steps: int = 5):
+def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
     """The reference GPU kernel's material (corotated, VBD_GPU_Wrapper.cu:114-213, with its
     x1.000001 inertia term) on the same resident mesh: autotuned lane-group kernel, whole steps
     replayed from a hipGraph, and the color kernel against the same byte yardstick."""
@@ -226,7 +226,11 @@ def bench_corotated(d, m, iterations: int, steps: int = 5):
     info0 = d.info()
     d.set_material_model(A.VBD_MATERIAL_COROTATED_GPU)
     d.set_positions(m.x0)
-    tune = d.autotune(2)
+    if cfg is None:
+        tune = d.autotune(2)
+    else:  # an explicit launch configuration (lanes, block, pairs per lane), as for the cube run
+        d.set_launch_config(*cfg)
+        tune = (cfg[0], cfg[2], cfg[1])
     d.set_positions(m.x0)
     d.set_velocities(np.zeros_like(m.x0))
     d.step(1)
@@ -405,7 +409,8 @@ def main():
     out["energy"] = bench_energy(d)
     log(f"[bench] energies: " + ", ".join(f"{k} {v['ms_per_eval']:.3f} ms ({v['frac']})" for k, v in out["energy"].items()))
     if not args.no_corotated:
-        cr = bench_corotated(d, m, args.iterations)
+        cr = bench_corotated(d, m, args.iterations,
+                             cfg=None if args.lanes is None else (args.lanes, args.block, args.unroll))
         out["corotated"] = cr
         log(f"[bench] corotated: {cr['value']:.1f} M vertex-iters/s, color kernel frac {cr['color_kernel_frac']}")
     if not args.no_batch:
