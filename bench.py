@@ -257,6 +257,50 @@ def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
+def bench_hybrid(m, iterations: int, frames: int = 5):
+    """The reference main.cpp:215-232 frame loop on the device solver, host buffers every frame:
+    host forwardStep (numpy), updateInertiaFromCPU (12 B per vertex H2D), solveVBD (x <- y,
+    Chebyshev 0.95), copyPositionsToCPU (12 B per vertex D2H, synchronising), host
+    updateVelocity.  Reports the PCIe-inclusive rate of the device part (upload + solve +
+    download) and of the whole loop; neither is the headline value."""
+    from vbd_tetmesh_amd.solver import VBD_GPU_Solver, prepare_host
+    t, off, verts, dminv, vol = prepare_host(m)
+    g = VBD_GPU_Solver()
+    t.computeInertia(H)
+    g.initFromCPU(t, (off, verts), dminv, vol)
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    dev = host = xfer = 0.0
+    for f in range(frames + 1):
+        h0 = time.perf_counter()
+        t.savePreviousPosition()
+        t.applyGravity(m.gravity, H)
+        t.computeInertia(H)
+        h1 = time.perf_counter()
+        g.updateInertiaFromCPU(t)
+        g.solveVBD(iterations, sdt, m.mu, m.lam, True, 0.95)
+        g.copyPositionsToCPU(t)
+        h2 = time.perf_counter()
+        t.updateVelocity(H)
+        h3 = time.perf_counter()
+        if f > 0:
+            dev += h2 - h1
+            host += (h1 - h0) + (h3 - h2)
+    # the transfers alone: one inertia upload and one position download per frame
+    for f in range(frames):
+        x0 = time.perf_counter()
+        g.updateInertiaFromCPU(t)
+        g.copyPositionsToCPU(t)
+        xfer += time.perf_counter() - x0
+    if not np.all(np.isfinite(t.mVertPos)):
+        raise SystemExit("non-finite positions in the hybrid loop")
+    vi = float(m.num_verts) * iterations * frames
+    return {"workload": "same cube, reference main.cpp frame loop (VBD_GPU_Solver, pageable host arrays)",
+            "device_value": round(vi / dev / 1e6, 2), "loop_value": round(vi / (dev + host) / 1e6, 2),
+            "unit": "M vertex-iters/s", "frames": frames,
+            "device_ms_per_frame": round(dev * 1e3 / frames, 3), "host_ms_per_frame": round(host * 1e3 / frames, 3),
+            "transfer_ms_per_frame": round(xfer * 1e3 / frames, 3), "pcie_bytes_per_frame": 24 * m.num_verts}
+
+
 def bench_adjoint(n: int = 16, steps: int = 30):
     """Differentiable workload: 16^3 cube, per-tet mu field, lambda = 4 mu, 30 converged steps;
     forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation +
@@ -310,6 +354,7 @@ def main():
     ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-corotated", action="store_true", help="skip the corotated-material run of the cube")
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the host-buffer (PCIe-inclusive) frame loop")
     ap.add_argument("--only", choices=["batched", "sphere", "adjoint"], default=None,
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
@@ -413,8 +458,15 @@ def main():
                              cfg=None if args.lanes is None else (args.lanes, args.block, args.unroll))
         out["corotated"] = cr
         log(f"[bench] corotated: {cr['value']:.1f} M vertex-iters/s, color kernel frac {cr['color_kernel_frac']}")
-    if not args.no_batch:
+    if not args.no_hybrid:
         del d
+        hy = bench_hybrid(m, args.iterations)
+        out["hybrid_pcie"] = hy
+        log(f"[bench] hybrid loop: device part {hy['device_value']:.1f} M vertex-iters/s "
+            f"({hy['device_ms_per_frame']:.2f} ms/frame, transfers {hy['transfer_ms_per_frame']:.2f} ms), whole loop "
+            f"{hy['loop_value']:.1f}")
+    if not args.no_batch:
+        d = None
         bb = bench_batch()
         out["batched"] = bb
         log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step; per-mesh plain "
