@@ -1,0 +1,94 @@
+"""GPU parity of the step parameters the headline workload leaves at their defaults: substeps
+(VBD.cpp:60-101 splits dt into substeps, each a forwardStep, iterations and updateVelocity),
+the velocity damping factor (TetMesh.h:308 hard-codes 0.999; a parameter here), and the
+penalty contact plane on every axis.
+
+Tolerance: positions within 1e-5 relative max-norm of the FP32 oracle per frame; the damped
+free fall against its float64 closed form within 1e-5 of the fall distance.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vbd_tetmesh_amd import VBDParams, meshgen
+from vbd_tetmesh_amd.solver import upload
+
+pytestmark = pytest.mark.gpu
+
+H = 1.0 / 60.0
+TOL = 1e-5
+
+
+def relmax(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("substeps", [2, 3])
+def test_substeps_match_oracle(substeps):
+    m = meshgen.beam()
+    prep = O.prepare(m)
+    frames = 10
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=30, substeps=substeps, rho=0.95, mu=m.mu,
+                                                 lam=m.lam, gravity=m.gravity), m.x0, None, frames)
+    p = VBDParams(dt=H, numIterations=30, substeps=substeps, gravity=m.gravity, useAcceleration=True,
+                  accelerationRho=0.95, mu=m.mu, lambda_=m.lam)
+    d = upload(m, params=p)
+    rel = 0.0
+    for f in range(frames):
+        d.step(1)
+        rel = max(rel, relmax(d.positions(), of[f]))
+    print(f"substeps={substeps}: max rel vs fp32 oracle {rel:.3e}", flush=True)
+    assert rel <= TOL
+
+
+@pytest.mark.parametrize("damping", [1.0, 0.95])
+def test_damped_free_fall_is_closed_form(damping):
+    """Zero stiffness, nothing pinned, plain sweeps: every sweep returns x = y, so
+    v_{n+1} = damping (v_n + h g) and x_{n+1} = x_n + h (v_n + h g)."""
+    m = meshgen.noisy_cube(6, seed=5, mu=0.0)
+    m.lam = 0.0
+    m.fixed[:] = 0
+    frames = 30
+    p = VBDParams(dt=H, numIterations=5, gravity=(0.0, -9.8, 0.0), useAcceleration=False, mu=0.0, lambda_=0.0,
+                  velocityDamping=damping)
+    d = upload(m, params=p)
+    for _ in range(frames):
+        d.step(1)
+    g = np.array([0.0, np.float32(-9.8), 0.0])
+    h = float(np.float32(H))
+    x = m.x0.astype(np.float64).copy()
+    v = np.zeros_like(x)
+    for _ in range(frames):
+        v = v + h * g
+        x = x + h * v
+        v = float(np.float32(damping)) * v
+    fall = np.abs(x - m.x0).max()
+    err = np.abs(d.positions() - x).max()
+    print(f"damping {damping}: fall {fall:.4f}, |gpu - closed form| {err:.2e}", flush=True)
+    assert err <= 1e-5 * fall
+    vg = d.velocities()
+    assert np.abs(vg - v).max() <= 1e-4 * np.abs(v).max()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_contact_plane_on_every_axis(axis):
+    """Shuffled-id sphere falling along -axis onto the plane x[axis] = -1.05."""
+    m = meshgen.voxel_sphere(14, seed=6)
+    grav = [0.0, 0.0, 0.0]
+    grav[axis] = -9.8
+    m.gravity = tuple(grav)
+    contact = (axis, -1.05, 1e4)
+    prep = O.prepare(m)
+    frames = 15
+    _, _, of, _ = O.simulate(prep, O.make_params(dt=H, iterations=25, rho=0.95, mu=m.mu, lam=m.lam,
+                                                 gravity=m.gravity, contact=contact), m.x0, None, frames)
+    p = VBDParams(dt=H, numIterations=25, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95, mu=m.mu,
+                  lambda_=m.lam, contact=contact)
+    d = upload(m, params=p)
+    rel = 0.0
+    for f in range(frames):
+        d.step(1)
+        rel = max(rel, relmax(d.positions(), of[f]))
+    low = of[-1][:, axis].min()
+    print(f"contact axis {axis}: max rel {rel:.3e}, lowest {low:.4f}", flush=True)
+    assert low < -1.05 + 0.05 and rel <= TOL
