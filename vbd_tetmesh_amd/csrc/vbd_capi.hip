@@ -84,6 +84,8 @@ struct DeviceGuard {
 };
 
 void drop_graph(vbd_solver* h) {
+    // a replay enqueued by an earlier asynchronous call may still be running
+    if ((h->graphExec || !h->blockGraphs.empty()) && h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
     h->graphExec = nullptr;
     h->graphValid = false;
