@@ -7,8 +7,9 @@
 // savePreviousPosition), and the boundary-surface OBJ export (TetMesh.h:45-186).
 #pragma once
 
+#include <algorithm>
+#include <cstdint>
 #include <fstream>
-#include <map>
 #include <string>
 #include <vector>
 
@@ -119,25 +120,45 @@ struct TetMesh {
     // Boundary faces: faces of the tets that occur exactly once, in ascending sorted-vertex
     // order, each with the winding of its (only) occurrence (TetMesh.h:80-141).
     std::vector<Vec3I> extractSurfaceFaces() const {
+        // all 4T faces keyed by their sorted vertex triple, sorted once (instead of a tree of
+        // 4T insertions); a key that occurs once is a boundary face
         struct Rec {
-            int count;
-            Vec3I oriented;
+            IdType k[3];
+            uint32_t face;  // 4 * tet + local face
         };
-        std::map<std::array<IdType, 3>, Rec> faces;
-        for (const auto& t : tets) {
-            const Vec3I tri[4] = {Vec3I(t[0], t[2], t[1]), Vec3I(t[0], t[1], t[3]), Vec3I(t[0], t[3], t[2]),
-                                  Vec3I(t[1], t[2], t[3])};
-            for (const Vec3I& f : tri) {
-                std::array<IdType, 3> key = {f[0], f[1], f[2]};
-                std::sort(key.begin(), key.end());
-                auto it = faces.find(key);
-                if (it == faces.end()) faces.emplace(key, Rec{1, f});
-                else it->second.count++;
+        std::vector<Rec> recs;
+        recs.reserve(4 * tets.size());
+        auto tri = [&](size_t t, int j) {
+            const TetVIdsArr& v = tets[t];
+            switch (j) {
+                case 0: return Vec3I(v[0], v[2], v[1]);
+                case 1: return Vec3I(v[0], v[1], v[3]);
+                case 2: return Vec3I(v[0], v[3], v[2]);
+                default: return Vec3I(v[1], v[2], v[3]);
             }
-        }
+        };
+        for (size_t t = 0; t < tets.size(); ++t)
+            for (int j = 0; j < 4; ++j) {
+                const Vec3I f = tri(t, j);
+                Rec r{{f[0], f[1], f[2]}, (uint32_t)(4 * t + j)};
+                std::sort(r.k, r.k + 3);
+                recs.push_back(r);
+            }
+        std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+            if (a.k[0] != b.k[0]) return a.k[0] < b.k[0];
+            if (a.k[1] != b.k[1]) return a.k[1] < b.k[1];
+            if (a.k[2] != b.k[2]) return a.k[2] < b.k[2];
+            return a.face < b.face;
+        });
         std::vector<Vec3I> out;
-        for (const auto& kv : faces)
-            if (kv.second.count == 1) out.push_back(kv.second.oriented);
+        for (size_t i = 0; i < recs.size();) {
+            size_t j = i + 1;
+            while (j < recs.size() && recs[j].k[0] == recs[i].k[0] && recs[j].k[1] == recs[i].k[1] &&
+                   recs[j].k[2] == recs[i].k[2])
+                ++j;
+            if (j == i + 1) out.push_back(tri(recs[i].face / 4, (int)(recs[i].face % 4)));
+            i = j;
+        }
         return out;
     }
 

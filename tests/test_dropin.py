@@ -30,6 +30,83 @@ def test_tetgen_round_trip(tmp_path, one_based):
     assert np.array_equal(pos, m.rest) and np.array_equal(tets, m.tets)
 
 
+def _build_tetgen_check(tmp_path):
+    exe = str(tmp_path / "tetgen_io_check")
+    lib = os.path.join(ROOT, "vbd_tetmesh_amd")
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "tetgen_io_check.cpp"), "-L", lib, "-lvbd_hip",
+                        f"-Wl,-rpath,{lib}", "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def _read_check(path):
+    b = open(path, "rb").read()
+    nv, nt = np.frombuffer(b, np.int32, 2)
+    o = 8
+    rest = np.frombuffer(b, np.float32, 3 * nv, o).reshape(-1, 3); o += 12 * nv
+    x = np.frombuffer(b, np.float32, 3 * nv, o).reshape(-1, 3); o += 12 * nv
+    tets = np.frombuffer(b, np.int32, 4 * nt, o).reshape(-1, 4); o += 16 * nt
+    fixed = np.frombuffer(b, np.uint8, nv, o); o += nv
+    mass = np.frombuffer(b, np.float32, nv, o); o += 4 * nv
+    nf = int(np.frombuffer(b, np.int32, 1, o)[0]); o += 4
+    faces = np.frombuffer(b, np.int32, 3 * nf, o).reshape(-1, 3)
+    return rest, x, tets, fixed, mass, faces
+
+
+def test_cpp_tetgen_loader_parses_the_format_and_pins_like_main_cpp(tmp_path):
+    """include/vbd/TetGenIO.h on hand-written files: comments, blank lines, attribute and
+    boundary-marker columns, 1-based ids, a separate current-shape node file.  Positions and
+    tets exactly as written (ids shifted to 0-based), vertices within 5% of the y extent of the
+    current shape pinned (main.cpp:146-164), lumped masses equal to the oracle's TetMesh.h:213-275
+    restatement, and the 12 outward boundary triangles of the one-cell Kuhn cube."""
+    m = meshgen.beam(1, 1, 1, cell=1.0)
+    rest = m.rest
+    cur = rest.copy()
+    cur[:, 1] += 0.5 * rest[:, 0]  # shear: the pin rule must use these y values
+    node = ["# rest shape", f"{rest.shape[0]}  3  1  1", ""]
+    node += [f"{i + 1}  {p[0]!r}  {p[1]!r}  {p[2]!r}  7.5  {i % 2}" for i, p in enumerate(rest.tolist())]
+    (tmp_path / "m.node").write_text("\n".join(node) + "\n")
+    curl = [f"{rest.shape[0]} 3 0 0"] + [f"{i + 1} {p[0]!r} {p[1]!r} {p[2]!r}" for i, p in enumerate(cur.tolist())]
+    (tmp_path / "cur.node").write_text("\n".join(curl) + "\n")
+    ele = ["# tets", f"{m.num_tets} 4 1"] + [f"{i + 1} {t[0] + 1} {t[1] + 1} {t[2] + 1} {t[3] + 1} 3"
+                                             for i, t in enumerate(m.tets.tolist())]
+    (tmp_path / "m.ele").write_text("\n".join(ele) + "\n")
+    exe = _build_tetgen_check(tmp_path)
+    out = str(tmp_path / "out.bin")
+    r = subprocess.run([exe, str(tmp_path / "m"), str(tmp_path / "cur.node"), "1", "0.05", out],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r_rest, r_x, r_tets, r_fixed, r_mass, faces = _read_check(out)
+    assert np.array_equal(r_rest, rest) and np.array_equal(r_x, cur.astype(np.float32))
+    assert np.array_equal(r_tets, m.tets)
+    y = cur[:, 1].astype(np.float32)
+    thr = np.float32(y.min() + (y.max() - y.min()) * np.float32(0.05))
+    assert np.array_equal(r_fixed, (y <= thr).astype(np.uint8)) and 0 < r_fixed.sum() < len(y)
+    mass, _, _, _ = O.mesh_init(rest, m.tets, 1000.0)
+    assert np.array_equal(r_mass, mass)
+    # the unit cube's boundary: 6 faces x 2 triangles, each outward
+    assert faces.shape == (12, 3)
+    c = rest.mean(axis=0)
+    for f in faces:
+        a, b, d = rest[f[0]], rest[f[1]], rest[f[2]]
+        assert np.dot(np.cross(b - a, d - a), (a + b + d) / 3 - c) > 0
+
+
+def test_cpp_tetgen_loader_rejects_bad_files(tmp_path):
+    exe = _build_tetgen_check(tmp_path)
+    (tmp_path / "bad.node").write_text("4 3 0 0\n1 0 0 0\n2 1 0 0\n3 0 1 0\n")  # one point short
+    (tmp_path / "bad.ele").write_text("1 4 0\n1 1 2 3 4\n")
+    r = subprocess.run([exe, str(tmp_path / "bad"), "-", "1", "0.05", str(tmp_path / "o.bin")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1
+    (tmp_path / "oob.node").write_text("4 3 0 0\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\n")
+    (tmp_path / "oob.ele").write_text("1 4 0\n1 1 2 3 5\n")  # vertex 5 does not exist
+    r = subprocess.run([exe, str(tmp_path / "oob"), "-", "1", "0.05", str(tmp_path / "o.bin")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1
+
+
 def test_demo_binary_is_built():
     assert os.path.exists(DEMO), "vbd_dropin_demo not built (run __graft_entry__.build())"
 

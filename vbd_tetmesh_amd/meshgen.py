@@ -253,19 +253,33 @@ def write_tetgen(prefix: str, positions: np.ndarray, tets: np.ndarray, one_based
             f.write(f"{i + base} {t[0] + base} {t[1] + base} {t[2] + base} {t[3] + base}\n")
 
 
+def _tetgen_rows(path: str, dtype):
+    """Header fields and the first n body rows of a TetGen file as an (n, columns) array; empty
+    lines and lines starting with '#' are skipped (as include/vbd/TetGenIO.h does)."""
+    with open(path) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.strip() and not ln.startswith("#")]
+    head = lines[0].split()
+    n = int(head[0])
+    body = lines[1:1 + n]
+    if len(body) != n:
+        raise ValueError(f"{path}: {len(body)} of {n} rows")
+    tokens = " ".join(body).split()
+    if n and len(tokens) % n == 0:  # the usual case: every row has the same column count
+        return head, np.array(tokens, dtype=dtype).reshape(n, -1)
+    width = min(len(r.split()) for r in body)
+    return head, np.array([r.split()[:width] for r in body], dtype=dtype)
+
+
 def read_tetgen(prefix: str):
     """Read TetGen .node/.ele files: (positions float32 (N,3), tets int32 (T,4)); 0- or 1-based
-    ids detected from the first node index, as main.cpp:38-39 and :112-114 do."""
-    with open(prefix + ".node") as f:
-        lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
-    n = int(lines[0][0])
-    rows = lines[1:1 + n]
-    start0 = int(rows[0][0]) == 0
-    pos = np.array([[float(r[1]), float(r[2]), float(r[3])] for r in rows], dtype=np.float32)
-    with open(prefix + ".ele") as f:
-        lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
-    m = int(lines[0][0])
-    tets = np.array([[int(v) for v in r[1:5]] for r in lines[1:1 + m]], dtype=np.int64)
+    ids detected from the first node index, as main.cpp:38-39 and :112-114 do.  Coordinates go
+    decimal -> float64 -> float32 (the C++ loader uses strtof; they can differ in the last bit
+    only for decimals within a float64 ulp of a float32 rounding midpoint)."""
+    _, nodes = _tetgen_rows(prefix + ".node", np.float64)
+    start0 = nodes.shape[0] > 0 and int(nodes[0, 0]) == 0
+    pos = nodes[:, 1:4].astype(np.float32)
+    _, ele = _tetgen_rows(prefix + ".ele", np.int64)
+    tets = ele[:, 1:5]
     if not start0:
-        tets -= 1
+        tets = tets - 1
     return pos, tets.astype(np.int32)
