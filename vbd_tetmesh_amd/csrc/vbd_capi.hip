@@ -317,6 +317,12 @@ int heuristic_lpv(const DevMesh& m) {
     return l;
 }
 
+// The tile cut (and its padded records) for a tiled configuration; the corotated material also
+// needs the full records padded.
+hipError_t tiles_for(vbd_solver* h, int records, int max_verts, std::string& err) {
+    return build_tiles(h->m, records, max_verts, h->stream, err, h->materialModel == VBD_MATERIAL_COROTATED_GPU);
+}
+
 hipError_t ensure_energy_scratch(vbd_solver* h) {
     if (h->d_epart) return hipSuccess;
     return hipMalloc(&h->d_epart, sizeof(double) * ((size_t)energy_partials(h->m) + SUM_STAGE));
@@ -340,7 +346,7 @@ void choose_default_config(vbd_solver* h) {
     for (const auto& c : wide ? wide_first : narrow_first) {
         std::string err;
         if (h->m.tiledOk && h->m.maxValence <= c[0] * c[1] &&
-            build_tiles(h->m, c[0] * c[1], c[0] / 8 - 1, h->stream, err) == hipSuccess) {
+            tiles_for(h, c[0] * c[1], c[0] / 8 - 1, err) == hipSuccess) {
             h->lpv = 0;
             h->unroll = c[1];
             h->block = c[0];
@@ -884,9 +890,13 @@ int vbd_set_material_model(vbd_solver* h, int32_t model) {
     if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
     if (model != VBD_MATERIAL_STABLE_NEO_HOOKEAN && model != VBD_MATERIAL_COROTATED_GPU)
         return fail(h, VBD_ERR_ARG, "unknown material model");
-    h->materialModel = model;
     DeviceGuard g(h->device);
     drop_graph(h);
+    h->materialModel = model;
+    if (h->inited && h->lpv == 0) {
+        std::string err;
+        if (tiles_for(h, h->unroll * h->block, h->block / 8 - 1, err) != hipSuccess) return fail(h, VBD_ERR_STATE, err);
+    }
     return VBD_OK;
 }
 
@@ -899,7 +909,7 @@ int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block
         if (!h->inited) return fail(h, VBD_ERR_STATE, "the tiled kernel needs an initialised mesh");
         DeviceGuard g(h->device);
         std::string err;
-        if (build_tiles(h->m, block_size * pairs_per_lane, block_size / 8 - 1, h->stream, err) != hipSuccess)
+        if (tiles_for(h, block_size * pairs_per_lane, block_size / 8 - 1, err) != hipSuccess)
             return fail(h, VBD_ERR_ARG, err);
     } else if (!launch_config_supported(lanes_per_vertex, pairs_per_lane, block_size) ||
                block_size % lanes_per_vertex) {
@@ -995,8 +1005,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
         auto it = g_tune_cache.find(key);
         std::string terr;
         if (it != g_tune_cache.end() &&
-            (it->second.l != 0 || build_tiles(h->m, it->second.u * it->second.b, it->second.b / 8 - 1, h->stream,
-                                                   terr) == hipSuccess)) {
+            (it->second.l != 0 || tiles_for(h, it->second.u * it->second.b, it->second.b / 8 - 1, terr) == hipSuccess)) {
             h->lpv = it->second.l;
             h->unroll = it->second.u;
             h->block = it->second.b;
@@ -1034,7 +1043,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
                 if (b < 128 || !tiled_config_supported(cd.u, b) || !h->m.tiledOk || h->m.maxValence > cd.u * b)
                     continue;
                 std::string terr;
-                if (build_tiles(h->m, cd.u * b, b / 8 - 1, h->stream, terr) != hipSuccess) continue;
+                if (tiles_for(h, cd.u * b, b / 8 - 1, terr) != hipSuccess) continue;
             }
             h->lpv = cd.l;
             h->unroll = cd.u;
@@ -1050,7 +1059,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
                 h->block = ob;
                 if (ol == 0) {
                     std::string terr;
-                    (void)build_tiles(h->m, ou * ob, ob / 8 - 1, h->stream, terr);
+                    (void)tiles_for(h, ou * ob, ob / 8 - 1, terr);
                 }
                 (void)launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
                 (void)hipStreamSynchronize(h->stream);
@@ -1071,7 +1080,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     if (e != hipSuccess) return hip_fail(h, "autotune", e);
     if (bl == 0) {
         std::string terr;
-        if (build_tiles(h->m, bu * bb, bb / 8 - 1, h->stream, terr) != hipSuccess) return fail(h, VBD_ERR_STATE, terr);
+        if (tiles_for(h, bu * bb, bb / 8 - 1, terr) != hipSuccess) return fail(h, VBD_ERR_STATE, terr);
     }
     h->lpv = bl;
     h->unroll = bu;

@@ -65,6 +65,16 @@ struct DevMesh {
     int* tileRec = nullptr;            // row[] at each tileStart entry (first record of the tile)
     int tileRecords = 0;               // record cap per tile of the current cut (0: none)
     int tileVerts = 0;                 // vertex cap per tile of the current cut
+    int tileEntries = 0;               // entries of tileStart / tileRec (tiles + one per color)
+    // the cut's records in tileRecords padded slots per tileStart entry (owner bits: the
+    // tile-local vertex index); tpa/tpb/tpc only when the corotated material needs them
+    i4* tpt = nullptr;
+    f4* tpq = nullptr;
+    f2* tpm = nullptr;
+    f4* tpa = nullptr;
+    f4* tpb = nullptr;
+    float* tpc = nullptr;
+    int64_t padBytes = 0;              // bytes of the padded records
     std::vector<int> colorTileBase;    // index of each color's first entry in tileStart
     std::vector<int> colorTileCount;   // tiles per color
     int64_t bytes = 0;
@@ -94,7 +104,7 @@ hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int b
                               float lambda, const ContactArgs& ct, f4* save, hipStream_t s, bool corotated = false);
 // tile cut for the tiled kernel with (per_thread, block): records = per_thread * block,
 // max_verts = block / 8 - 1
-hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err);
+hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err, bool full = false);
 hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
 hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s);
 hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t s);

@@ -419,12 +419,13 @@ __device__ __forceinline__ f4 corot_vertex_solve(const f4 xs, const f4 ys, const
 // cof(F).
 //
 // The color's free vertices are cut into tiles of consecutive slots holding at most
-// RT = K * blockDim.x records and fewer than blockDim.x / 8 vertices (tileStart / tileRec,
-// built on the host).  One block per tile; XCD x takes the x-th eighth of the tiles.
-//   1. thread i takes records r0+i, r0+i+T, ... (K of them): the whole block streams one
-//      contiguous record range, every lane busy.  The record's owner slot comes from
-//      chunkBase[p >> 12] + the owner offset bits; a tile spans at most two 4096-record
-//      chunks, whose bases are loaded once per block.  The 9 contribution values (6 Hessian
+// RT = K * blockDim.x records and fewer than blockDim.x / 8 vertices (tileStart, built on the
+// host).  Each tile's records are copied into RT padded slots at tile * RT (tpt, tpq, ...;
+// padding records are all zero), so a block finds its records from its index alone.  One
+// block per tile; XCD x takes the x-th eighth of the tiles.
+//   1. thread i takes records i, i+T, ... (K of them) of its tile: the whole block streams one
+//      contiguous range, every lane busy.  The record's owner is tileStart[tile] + the
+//      tile-local vertex index held in the owner bits.  The 9 contribution values (6 Hessian
 //      entries, 3 force entries) go to LDS, one array per value; the tile's CSR row offsets,
 //      positions, inertia targets and masses are staged in LDS too.
 //   2. after a barrier, TG = 8 lanes per vertex sum its records (lane l: records l, l+TG, ...
@@ -525,10 +526,9 @@ __device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, 
 // COROT: the corotated material of the reference GPU kernel on the full records (pa, pb, pc)
 // instead of the compact stable Neo-Hookean records (pq); same tiles, reduction and order.
 template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
-__global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, const int* __restrict__ tileRec,
-                                                     int ntiles, int nslots, f4* __restrict__ pos,
-                                                     const f4* __restrict__ inert, const float* __restrict__ mass,
-                                                     const int* __restrict__ row, const int* __restrict__ chunkBase,
+__global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, int ntiles, int nslots,
+                                                     f4* __restrict__ pos, const f4* __restrict__ inert,
+                                                     const float* __restrict__ mass, const int* __restrict__ row,
                                                      const i4* __restrict__ ptet, const f4* __restrict__ pq,
                                                      const f4* __restrict__ pa, const f4* __restrict__ pb,
                                                      const float* __restrict__ pc, const f2* __restrict__ pmat,
@@ -550,30 +550,25 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     const int tile = (int)(((int64_t)ntiles * xcd) >> 3) + (int)(blockIdx.x >> 3);
     if (tile >= (int)(((int64_t)ntiles * (xcd + 1)) >> 3)) return;  // whole block
     const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, nslots);
-    const int rec0 = tileRec[0];
-    const int nrec = tileRec[ntiles] - rec0;  // records of this color
-    const int pmax = nrec - 1;
+    // the tile's RT padded record slots: their address depends on the block index alone, so the
+    // record loads issue first, with nothing to wait for
+    const size_t base = (size_t)tile * RT;
     RecSrc src;
-    src.t = rec_rsrc(ptet + rec0, 16 * (int64_t)nrec);
+    src.t = rec_rsrc(ptet + base, 16 * (int64_t)RT);
     if constexpr (COROT) {
-        src.a = rec_rsrc(pa + rec0, 16 * (int64_t)nrec);
-        src.b = rec_rsrc(pb + rec0, 16 * (int64_t)nrec);
-        src.c = rec_rsrc(pc + rec0, 4 * (int64_t)nrec);
+        src.a = rec_rsrc(pa + base, 16 * (int64_t)RT);
+        src.b = rec_rsrc(pb + base, 16 * (int64_t)RT);
+        src.c = rec_rsrc(pc + base, 4 * (int64_t)RT);
     } else {
-        src.q = rec_rsrc(pq + rec0, 16 * (int64_t)nrec);
+        src.q = rec_rsrc(pq + base, 16 * (int64_t)RT);
     }
-    src.mm = MAT ? rec_rsrc(pmat + rec0, 8 * (int64_t)nrec) : src.t;
-
-    const int r0 = tileRec[tile] - rec0, r1 = tileRec[tile + 1] - rec0;
+    src.mm = MAT ? rec_rsrc(pmat + base, 8 * (int64_t)RT) : src.t;
     RecRegs<K, MAT> R;
-    load_records<K, MAT, COROT>(R, src, r0, pmax, tid, T);
-    // owner bases of the (at most two) 4096-record chunks the tile touches
-    const int chunkA = (rec0 + r0) >> CHUNK_SHIFT;
-    const int cbA = chunkBase[chunkA], cbB = chunkBase[(rec0 + max(r1 - 1, r0)) >> CHUNK_SHIFT];
+    load_records<K, MAT, COROT>(R, src, 0, RT - 1, tid, T);
     const int v0 = tileStart[tile], v1 = tileStart[tile + 1];
     const int nv = v1 - v0;
     // the tile's vertex data (clamped indices; lanes past nv load and ignore a copy)
-    const int vrow = row[min(v0 + tid, v1)];
+    const int vrow = row[min(v0 + tid, v1)], vrow0 = row[v0];
     const int vs = min(v0 + tid, v1 - 1);
     const f4 vx = pos[vs];
     const f4 vy = __builtin_nontemporal_load(&inert[vs]);
@@ -581,9 +576,7 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     f4 xs[K], ya[K], yb[K], yc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int p = min(r0 + tid + k * T, pmax);
-        const int cb = ((rec0 + p) >> CHUNK_SHIFT) == chunkA ? cbA : cbB;
-        xs[k] = gather_pos(prs, cb + owner_offset(R.t[k]));
+        xs[k] = gather_pos(prs, v0 + owner_offset(R.t[k]));  // owner: the tile-local vertex index
         ya[k] = gather_pos(prs, R.t[k].x & SLOT_MASK);
         yb[k] = gather_pos(prs, R.t[k].y & SLOT_MASK);
         yc[k] = gather_pos(prs, R.t[k].z & SLOT_MASK);
@@ -604,7 +597,7 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
         } else {
             A = compact_contrib(R.t[k].x < 0, R.q[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k], ya[k], yb[k], yc[k]);
         }
-        if (r0 + q < r1) {
+        {  // padding records contribute zeros past the last vertex's range
             red[0 * RT + q] = A.h00;
             red[1 * RT + q] = A.h01;
             red[2 * RT + q] = A.h02;
@@ -616,7 +609,7 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
             red[8 * RT + q] = A.r2;
         }
     }
-    if (tid <= nv) lrow[tid] = vrow - rec0 - r0;
+    if (tid <= nv) lrow[tid] = vrow - vrow0;
     if (tid < nv) {
         lx[tid] = vx;
         ly[tid] = vy;
@@ -666,10 +659,12 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
     const size_t lds = sizeof(float) * 9 * (size_t)K * block +
                        (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / TG);
     const int per = (ntiles + 7) / 8;
+    if (COROT && !m.tpa) return hipErrorInvalidValue;  // full records not padded (build_tiles full)
+    const size_t base = (size_t)m.colorTileBase[color] * m.tileRecords;
     hipLaunchKernelGGL((k_color_tiled<K, MAT, CONTACT, SAVE, COROT>), dim3(8 * per), dim3(block), lds, s,
-                       m.tileStart + m.colorTileBase[color], m.tileRec + m.colorTileBase[color], ntiles, m.nV, m.pos,
-                       m.inert, m.mass, m.row, m.chunkBase, m.ptet, m.pq, m.pa, m.pb, m.pc, m.pmat, save, mu, lambda,
-                       sdt, ct);
+                       m.tileStart + m.colorTileBase[color], ntiles, m.nV, m.pos, m.inert, m.mass, m.row,
+                       m.tpt + base, m.tpq + base, COROT ? m.tpa + base : nullptr, COROT ? m.tpb + base : nullptr,
+                       COROT ? m.tpc + base : nullptr, MAT ? m.tpm + base : nullptr, save, mu, lambda, sdt, ct);
     return hipGetLastError();
 }
 
@@ -696,7 +691,7 @@ bool tiled_config_supported(int per_thread, int block) {
 hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int block, float sdt, float mu,
                               float lambda, const ContactArgs& ct, f4* save, hipStream_t s, bool corotated) {
     if (!tiled_config_supported(per_thread, block) || m.tileRecords != per_thread * block ||
-        m.tileVerts != block / TG - 1 || !m.tileStart)
+        m.tileVerts != block / TG - 1 || !m.tileStart || !m.tpt)
         return hipErrorInvalidValue;
     if (corotated)
         return per_thread == 1 ? launch_tiled_k<1, true>(m, color, block, sdt, mu, lambda, ct, save, s)
@@ -708,8 +703,105 @@ hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int b
 // Greedy cut of every color's slot range into tiles of at most `records` records and
 // `max_verts` vertices (host; uses the host copy of the CSR row offsets).  Fails if a vertex
 // alone exceeds the record cap.
-hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err) {
-    if (m.tileRecords == records && m.tileVerts == max_verts && m.tileStart) return hipSuccess;
+// Padded tile records: slot j of tile entry i (an index into tileStart / tileRec, a color's
+// closing entry included) holds record tileRec[i] + j of the CSR order when j is below the
+// tile's record count, with the owner bits rewritten to the tile-local vertex index; other
+// slots are zero (a zero-volume record: no contribution, and no vertex sums it).
+__global__ __launch_bounds__(256) void k_pad_records(int nentries, int rt, const int* __restrict__ tileStart,
+                                                     const int* __restrict__ tileRec,
+                                                     const int* __restrict__ chunkBase, const i4* __restrict__ ptet,
+                                                     const f4* __restrict__ pq, const f2* __restrict__ pmat,
+                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                                     const float* __restrict__ pc, i4* __restrict__ tpt,
+                                                     f4* __restrict__ tpq, f2* __restrict__ tpm,
+                                                     f4* __restrict__ tpa, f4* __restrict__ tpb,
+                                                     float* __restrict__ tpc) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)nentries * rt) return;
+    const int i = (int)(g / rt), j = (int)(g % rt);
+    const int r0 = tileRec[i];
+    const int cnt = i + 1 < nentries ? tileRec[i + 1] - r0 : 0;
+    i4 t = i4{0, 0, 0, 0};
+    f4 q = f4{0.f, 0.f, 0.f, 0.f}, a = q, b = q;
+    f2 mm = f2{0.f, 0.f};
+    float c = 0.f;
+    if (j < cnt) {
+        const int p = r0 + j;
+        t = ptet[p];
+        const int local = chunkBase[p >> CHUNK_SHIFT] + owner_offset(t) - tileStart[i];
+        t.x = (t.x & ~(15 << 27)) | ((local & 15) << 27);
+        t.y = (t.y & ~(15 << 27)) | (((local >> 4) & 15) << 27);
+        t.z = (t.z & ~(15 << 27)) | (((local >> 8) & 15) << 27);
+        q = pq[p];
+        if (pmat) mm = pmat[p];
+        if (tpa) {
+            a = pa[p];
+            b = pb[p];
+            c = pc[p];
+        }
+    }
+    tpt[g] = t;
+    tpq[g] = q;
+    if (tpm) tpm[g] = mm;
+    if (tpa) {
+        tpa[g] = a;
+        tpb[g] = b;
+        tpc[g] = c;
+    }
+}
+
+void free_padded_records(DevMesh& m) {
+    void* ps[] = {m.tpt, m.tpq, m.tpm, m.tpa, m.tpb, m.tpc};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    m.bytes -= m.padBytes;
+    m.padBytes = 0;
+    m.tpt = nullptr;
+    m.tpq = nullptr;
+    m.tpm = nullptr;
+    m.tpa = nullptr;
+    m.tpb = nullptr;
+    m.tpc = nullptr;
+}
+
+// (Re)builds the padded record slots of the current tile cut; full adds the full records
+// (pa, pb, pc) the corotated material reads.
+hipError_t pad_records(DevMesh& m, bool full, hipStream_t s, std::string& err) {
+    const int nentries = m.tileEntries;
+    const size_t n = (size_t)std::max(nentries, 1) * (size_t)m.tileRecords;
+    (void)hipStreamSynchronize(s);
+    free_padded_records(m);
+    hipError_t e = hipMalloc(&m.tpt, sizeof(i4) * n);
+    if (e == hipSuccess) e = hipMalloc(&m.tpq, sizeof(f4) * n);
+    if (e == hipSuccess && m.pmat) e = hipMalloc(&m.tpm, sizeof(f2) * n);
+    if (e == hipSuccess && full) e = hipMalloc(&m.tpa, sizeof(f4) * n);
+    if (e == hipSuccess && full) e = hipMalloc(&m.tpb, sizeof(f4) * n);
+    if (e == hipSuccess && full) e = hipMalloc(&m.tpc, sizeof(float) * n);
+    if (e == hipSuccess) {
+        m.padBytes = (int64_t)n * (int64_t)(sizeof(i4) + sizeof(f4) + (m.tpm ? sizeof(f2) : 0) +
+                                            (full ? 2 * sizeof(f4) + sizeof(float) : 0));
+        m.bytes += m.padBytes;
+    }
+    if (e == hipSuccess && nentries > 0) {
+        const int64_t total = (int64_t)nentries * m.tileRecords;
+        hipLaunchKernelGGL(k_pad_records, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nentries,
+                           m.tileRecords, m.tileStart, m.tileRec, m.chunkBase, m.ptet, m.pq, m.pmat, m.pa, m.pb, m.pc,
+                           m.tpt, m.tpq, m.tpm, m.tpa, m.tpb, m.tpc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        err = std::string("padded tile records: ") + hipGetErrorString(e);
+        free_padded_records(m);
+    }
+    return e;
+}
+
+hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err, bool full) {
+    if (m.tileRecords == records && m.tileVerts == max_verts && m.tileStart) {
+        if (full && !m.tpa) return pad_records(m, true, s, err);
+        return hipSuccess;
+    }
     if (!m.tiledOk) {
         err = "tiled kernel unavailable: a 4096-record chunk spans more than 4096 vertices";
         return hipErrorInvalidValue;
@@ -751,6 +843,7 @@ hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, st
         m.tileStart = nullptr;
         m.tileRec = nullptr;
         m.tileRecords = 0;
+        free_padded_records(m);
     }
     const size_t n = starts.empty() ? 1 : starts.size();
     hipError_t e = hipMalloc(&m.tileStart, sizeof(int) * n);
@@ -772,7 +865,17 @@ hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, st
     m.colorTileCount = count;
     m.tileRecords = records;
     m.tileVerts = max_verts;
-    return hipSuccess;
+    m.tileEntries = (int)starts.size();
+    e = pad_records(m, full, s, err);
+    if (e != hipSuccess) {
+        (void)hipFree(m.tileStart);
+        (void)hipFree(m.tileRec);
+        m.tileStart = nullptr;
+        m.tileRec = nullptr;
+        m.tileRecords = 0;
+        m.tileVerts = 0;
+    }
+    return e;
 }
 
 template <int LPV, int U, bool MAT, bool CONTACT, bool SAVE>
@@ -2080,7 +2183,8 @@ fail:
 void free_device_mesh(DevMesh& m) {
     void* ptrs[] = {m.pos, m.inert, m.vel, m.xprev, m.save[0], m.save[1], m.mass, m.row, m.perm, m.inv,
                     m.ptet, m.pa, m.pb, m.pc, m.pmat, m.adjOff, m.adjEntry, m.colorList, m.tets, m.dminv,
-                    m.vol, m.tmu, m.tlam, m.chunkBase, m.tileStart, m.tileRec, m.pq};
+                    m.vol, m.tmu, m.tlam, m.chunkBase, m.tileStart, m.tileRec, m.pq,
+                    m.tpt, m.tpq, m.tpm, m.tpa, m.tpb, m.tpc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     m = DevMesh();
