@@ -124,7 +124,7 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
     nt = sum(m.num_tets for m in ms)
     b = DeviceMesh().init_batch(descs)
     b.set_params(params)
-    tune = b.autotune(3)
+    tune = b.autotune(8)
     b.set_positions(np.concatenate([d["positions"] for d in descs]))
     b.step(1)  # warm-up (captures the graph)
     b.synchronize()
@@ -182,7 +182,7 @@ def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30, slot_order:
                        mu=m.mu, lambda_=m.lam, contact=(2, -1.2, 1e4))
     d = upload(m, params=params, slot_order=slot_order)
     info = d.info()
-    tune = d.autotune(3)
+    tune = d.autotune(8)
     d.set_positions(m.x0)
     d.step(1)
     d.synchronize()
@@ -227,7 +227,7 @@ def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
     d.set_material_model(A.VBD_MATERIAL_COROTATED_GPU)
     d.set_positions(m.x0)
     if cfg is None:
-        tune = d.autotune(2)
+        tune = d.autotune(4)
     else:  # an explicit launch configuration (lanes, block, pairs per lane), as for the cube run
         d.set_launch_config(*cfg)
         tune = (cfg[0], cfg[2], cfg[1])
@@ -392,7 +392,7 @@ def main():
         d.set_launch_config(args.lanes, args.block, args.unroll)
     elif not args.no_autotune:
         d.set_positions(m.x0)
-        tune = d.autotune(3)
+        tune = d.autotune(8)
         d.set_positions(m.x0)
         log(f"[bench] autotune -> lanes={tune[0]} pairs/lane={tune[1]} block={tune[2]} ({tune[3]:.3f} ms/sweep)")
     info = d.info()
