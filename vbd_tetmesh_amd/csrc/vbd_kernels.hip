@@ -1431,7 +1431,10 @@ hipError_t launch_blend_vec(int n, f4* v, const f4* pp, float omega, hipStream_t
 // ---------------------------------------------------------------------------------------
 // Chebyshev blend over the free slots: x = omega (x - x_pp) + x_pp   (VBD.cpp:290-300)
 // ---------------------------------------------------------------------------------------
+// The Chebyshev step (VBD.cpp:290-300): omega (x - x_pp) + x_pp, rounded after the product and
+// after the sum as the reference's CPU code is (no contraction).
 __global__ __launch_bounds__(256) void k_blend(int n, f4* __restrict__ pos, const f4* __restrict__ pp, float omega) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const f4 x = pos[i];
