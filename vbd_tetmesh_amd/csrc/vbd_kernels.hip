@@ -617,7 +617,9 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     }
     lds_barrier();
 
-    // one group of TG lanes per tile vertex
+    // one group of TG lanes per tile vertex; waves past the tile's last group have nothing to
+    // sum (the test is uniform per wave)
+    if ((tid & ~63) >= nv * TG) return;
     const int lane = tid % TG;
     const int i = tid / TG;
     const bool ok = i < nv;
