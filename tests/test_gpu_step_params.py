@@ -92,3 +92,33 @@ def test_contact_plane_on_every_axis(axis):
     low = of[-1][:, axis].min()
     print(f"contact axis {axis}: max rel {rel:.3e}, lowest {low:.4f}", flush=True)
     assert low < -1.05 + 0.05 and rel <= TOL
+
+
+def test_step_pieces_equal_the_whole_step():
+    """vbd_forward_step + vbd_solve (x_t start) + vbd_update_velocity reproduce vbd_step bitwise,
+    and the forwardStep state is as TetMesh.h:319-348 defines it: x_prev = x; free vertices
+    v += h g, y = x + h v; pinned vertices y = x."""
+    m = meshgen.beam()
+    p = VBDParams(dt=H, numIterations=30, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95, mu=m.mu,
+                  lambda_=m.lam)
+    a = upload(m, params=p)
+    a.step(3)
+    b = upload(m, params=p)
+    hh = np.float32(H)
+    sdt = float(np.float32(1.0 / np.float64(hh * hh)))
+    g = np.asarray(m.gravity, np.float32)
+    free = m.fixed == 0
+    for _ in range(3):
+        x0, v0 = b.positions(), b.velocities()
+        b.forward_step()
+        assert np.array_equal(b.previous_positions(), x0)
+        v1 = v0.copy()
+        v1[free] = v0[free] + hh * g
+        y = x0.copy()
+        y[free] = x0[free] + hh * v1[free]
+        assert np.array_equal(b.velocities(), v1) and np.array_equal(b.inertia(), y)
+        b.solve(30, sdt, m.mu, m.lam, True, 0.95, start_from_inertia=False)
+        b.update_velocity()
+    assert np.array_equal(b.positions(), a.positions())
+    assert np.array_equal(b.velocities(), a.velocities())
+

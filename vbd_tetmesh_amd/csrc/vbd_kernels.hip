@@ -1458,6 +1458,7 @@ hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t
 __global__ __launch_bounds__(256) void k_predict(int n, int nFree, const f4* __restrict__ pos, f4* __restrict__ vel,
                                                  f4* __restrict__ xprev, f4* __restrict__ inert, float h, float gx,
                                                  float gy, float gz) {
+#pragma clang fp contract(off)  // product and sum rounded separately, as the reference's CPU code
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const f4 x = pos[i];
