@@ -5,6 +5,7 @@
 * Tet-less vertices (mass 0) and the corners of a zero-stiffness tet of tiny mass: the 3x3
   block has |det H| <= 1e-8, so the vertex is skipped (VBD.cpp:141-153) and never moves.
 * A color made only of tet-less vertices (a color with no records), placed last.
+* Batched initialisation with malformed per-mesh color offsets: rejected before they are used.
 
 Tolerance: positions within 1e-5 relative max-norm of the FP32 oracle; skipped vertices and
 the coloring-independence checks are exact.
@@ -121,3 +122,29 @@ def test_a_color_of_tetless_vertices_changes_nothing(lanes):
     print(f"[tet-less color] lanes={lanes} colors {len(off) - 1} -> {len(off2) - 1}", flush=True)
     assert np.array_equal(runs[0], runs[1])
     assert np.array_equal(runs[1][loose], m.x0[loose])
+
+
+def test_batch_init_rejects_malformed_color_lists():
+    """Each mesh's color offsets are validated before its color list is read."""
+    from vbd_tetmesh_amd import _capi as A
+    good, bad_end, bad_order = [], None, None
+    for k in range(2):
+        m = meshgen.beam(4, 2, 2)
+        t, off, verts, dminv, vol = prepare_host(m)
+        good.append(dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                         rest_volume=vol, color_off=off, color_verts=verts))
+    bad_end = [dict(good[0]), good[1]]
+    off = good[0]["color_off"].copy()
+    off[-1] += 5  # would read past the caller's color list
+    bad_end[0]["color_off"] = off
+    bad_order = [good[0], dict(good[1])]
+    off = good[1]["color_off"].copy()
+    off[1], off[2] = off[2], off[1]
+    bad_order[1]["color_off"] = off
+    for descs in (bad_end, bad_order):
+        d = DeviceMesh(0)
+        with pytest.raises(A.VBDError) as ei:
+            d.init_batch(descs)
+        assert ei.value.code == A.VBD_ERR_ARG
+    DeviceMesh(0).init_batch(good)
+

@@ -599,6 +599,12 @@ int vbd_init_batch(vbd_solver* h, int32_t num_meshes, const vbd_mesh_desc* meshe
         if ((d.tet_mu != nullptr) != mat || (d.tet_lambda != nullptr) != mat)
             return fail(h, VBD_ERR_ARG, "per-tet materials must be given for all meshes or none");
         if (!d.color_offsets) return fail(h, VBD_ERR_ARG, "color offsets null");
+        if (d.num_verts > 0 && !d.color_verts) return fail(h, VBD_ERR_ARG, "color vertex list null");
+        // each mesh's color offsets are read before the union is validated: check them first
+        if (d.color_offsets[0] != 0 || d.color_offsets[d.num_colors] != d.num_verts)
+            return fail(h, VBD_ERR_ARG, "color offsets must start at 0 and end at num_verts");
+        for (int c = 0; c < d.num_colors; ++c)
+            if (d.color_offsets[c + 1] < d.color_offsets[c]) return fail(h, VBD_ERR_ARG, "color offsets decrease");
         nV += d.num_verts;
         nT += d.num_tets;
         nC = std::max(nC, (int)d.num_colors);
