@@ -446,9 +446,11 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
     }
     if (d->color_offsets[0] != 0 || d->color_offsets[nC] != nV)
         return fail(h, VBD_ERR_ARG, "color offsets must start at 0 and end at num_verts");
+    // offsets first (monotonic and ending at nV, so no color reads past the caller's list)
+    for (int c = 0; c < nC; ++c)
+        if (d->color_offsets[c + 1] < d->color_offsets[c]) return fail(h, VBD_ERR_ARG, "color offsets decrease");
     std::vector<int32_t> colorOf(nV > 0 ? nV : 1, -1);
     for (int c = 0; c < nC; ++c) {
-        if (d->color_offsets[c + 1] < d->color_offsets[c]) return fail(h, VBD_ERR_ARG, "color offsets decrease");
         for (int i = d->color_offsets[c]; i < d->color_offsets[c + 1]; ++i) {
             const int v = d->color_verts[i];
             if (v < 0 || v >= nV || colorOf[v] != -1)
