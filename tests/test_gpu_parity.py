@@ -287,6 +287,46 @@ def test_cube128_color_launches_match_oracle_on_sampled_vertices():
     assert worst <= 1e-4
 
 
+def test_sphere96_contact_color_launches_match_oracle_on_sampled_vertices():
+    """Full-size 96^3 voxel sphere (485k verts, 2.78M tets, 11 colors, shuffled ids) with the
+    penalty plane cutting its bottom cap, so contact is active in the first sweep: one device
+    sweep, each color launch checked on its penetrating vertices plus a seeded sample (2000 in
+    all) against the oracle's local solves from the state that launch saw."""
+    m = meshgen.voxel_sphere(96, seed=0)
+    contact = (2, -0.95, 1e4)
+    print(f"[sphere96] nV={m.num_verts} nT={m.num_tets}", flush=True)
+    prep = O.prepare(m)
+    t, off, verts, dminv, vol = prepare_host(m)
+    y = m.x0.copy()
+    y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
+    d = DeviceMesh().init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, off, verts, inertia=y)
+    d.set_params(params_for(m, 1, contact=contact))
+    sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
+    d.solve_iteration(sdt, m.mu, m.lam)
+    got = d.positions()
+    rng = np.random.default_rng(1)
+    p = oracle_params(m, 1, accel=False, contact=contact)
+    worst, touching = 0.0, 0
+    for c in range(prep.n_colors):
+        vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
+        vs = vs[m.fixed[vs] == 0]
+        low = vs[m.x0[vs, 2] < contact[1]]
+        rest = np.setdiff1d(vs, low)
+        k = max(0, min(2000 - low.size, rest.size))
+        sample = np.sort(np.concatenate([low, rng.choice(rest, size=k, replace=False)])).astype(np.int32)
+        touching += low.size
+        xin = got.copy()
+        later = prep.color_verts[prep.color_off[c]:]
+        xin[later] = m.x0[later]
+        xo, _ = O.solve_subset(prep, p, xin, y, sample)
+        disp = np.abs(xo - m.x0[sample]).max()
+        err = np.abs(got[sample].astype(np.float64) - xo).max()
+        worst = max(worst, err / max(disp, 1e-30))
+        print(f"sphere96 color {c}: {sample.size} checked ({low.size} in contact), |dx| {disp:.3e} err {err:.3e}",
+              flush=True)
+    assert touching > 0 and worst <= 1e-4
+
+
 def test_sphere_with_contact_matches_oracle():
     """Voxel sphere (shuffled ids, irregular valence) dropped on a penalty plane, 24^3 grid."""
     m = meshgen.voxel_sphere(24, seed=0)
