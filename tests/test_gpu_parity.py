@@ -412,6 +412,35 @@ def test_batch_equals_per_mesh_runs_bitwise(cfg):
     assert rel <= TOL and rel64 <= TOL
 
 
+def test_benchmark_batch_of_256_beams_equals_per_mesh_runs_bitwise():
+    """The batched benchmark workload at full size (256 uneven beams, 394,633 verts, per-tet
+    materials, default launch configuration): 2 batched frames == the 256 meshes stepped alone."""
+    ms = meshgen.batch_beams(256, seed=0)
+    params = VBDParams(dt=H, numIterations=20, useAcceleration=True, accelerationRho=0.95, gravity=(0, -9.8, 0))
+    descs, singles = [], []
+    for m in ms:
+        t, off, verts, dminv, vol = prepare_host(m)
+        kw = dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                  rest_volume=vol, color_off=off, color_verts=verts, tet_mu=t.tetMu, tet_lambda=t.tetLambda)
+        descs.append(kw)
+    b = DeviceMesh().init_batch(descs)
+    b.set_params(params)
+    info = b.info()
+    print(f"[batch256] nV={info.num_verts} lanes={info.lanes_per_vertex} block={info.block_size} "
+          f"k={info.pairs_per_lane}", flush=True)
+    b.step(2)
+    got = b.positions()
+    for kw in descs:
+        d = DeviceMesh().init(**kw)
+        d.set_params(params)
+        d.set_launch_config(info.lanes_per_vertex, info.block_size, info.pairs_per_lane)
+        d.step(2)
+        singles.append(d.positions())
+        d.close()
+    assert info.num_verts == 394633
+    assert np.array_equal(got, np.concatenate(singles))
+
+
 def test_device_energy_matches_oracle():
     m = meshgen.noisy_cube(10, seed=1)
     prep = O.prepare(m)
