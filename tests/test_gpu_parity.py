@@ -252,25 +252,29 @@ def test_noisy_cube_32_matches_oracle():
     assert rel <= TOL and rel64 <= TOL
 
 
-def test_cube128_color_launches_match_oracle_on_sampled_vertices():
+@pytest.mark.parametrize("material", [0, 1])
+def test_cube128_color_launches_match_oracle_on_sampled_vertices(material):
     """Full-size 128^3 cube (2.15M verts, 12.6M tets): one device sweep, every color launch
     checked on a seeded sample of 2000 of its vertices against the oracle's local solves from
     the state that launch saw (earlier colors already updated by the device, later colors
-    and this color still at the start positions)."""
+    and this color still at the start positions).  Material 0: stable Neo-Hookean (compact
+    records); 1: the reference GPU kernel's corotated model (full records)."""
     m = meshgen.noisy_cube(128, seed=0)
-    print(f"[cube128] nV={m.num_verts} nT={m.num_tets}", flush=True)
+    print(f"[cube128] material {material} nV={m.num_verts} nT={m.num_tets}", flush=True)
     prep = O.prepare(m)
     t, off, verts, dminv, vol = prepare_host(m)
     y = m.x0.copy()
     y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
     d = DeviceMesh().init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, off, verts, inertia=y)
+    d.set_material_model(material)
     info = d.info()
-    assert info.num_verts == 129 ** 3 and info.num_tets == 6 * 128 ** 3
+    assert info.num_verts == 129 ** 3 and info.num_tets == 6 * 128 ** 3 and info.lanes_per_vertex == 0
     sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
     d.solve_iteration(sdt, m.mu, m.lam)
     got = d.positions()
     rng = np.random.default_rng(0)
     p = oracle_params(m, 1, accel=False)
+    p.material = material
     worst = 0.0
     for c in range(prep.n_colors):
         vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
@@ -283,7 +287,7 @@ def test_cube128_color_launches_match_oracle_on_sampled_vertices():
         disp = np.abs(xo - m.x0[sample]).max()
         err = np.abs(got[sample].astype(np.float64) - xo).max()
         worst = max(worst, err / max(disp, 1e-30))
-        print(f"cube128 color {c}: |dx| {disp:.3e} err {err:.3e}", flush=True)
+        print(f"cube128 material {material} color {c}: |dx| {disp:.3e} err {err:.3e}", flush=True)
     assert worst <= 1e-4
 
 
