@@ -57,7 +57,8 @@ struct DevMesh {
     float* vol = nullptr;   // nT
     float* tmu = nullptr;   // nT or null
     float* tlam = nullptr;  // nT or null
-    // tiled color kernel: owner bases of 4096-record chunks and the tile cut of each color
+    // tiled color kernel: owner bases of 4096-record chunks (read by the tile build) and the
+    // tile cut of each color
     int* chunkBase = nullptr;          // ceil(nPairs / 4096) + 1 chunk bases
     bool tiledOk = false;              // owner offsets fit the 12 record bits
     std::vector<int> hostRow;          // host copy of row (nFree+1)

@@ -66,7 +66,8 @@ __device__ __forceinline__ int xcd_block() {
 // ---------------------------------------------------------------------------------------
 // Record slot fields hold 27-bit slots.  Bit 31 of .x flags "vertex is corner 0"; bits 27-30
 // of .x, .y, .z together hold the record's owner as an offset (12 bits) from the owner of the
-// first record of its 4096-record chunk (chunkBase), which the tiled color kernel uses.
+// first record of its 4096-record chunk (chunkBase); the tile build reads it and rewrites it,
+// in the padded tile records, as the owner's index inside its tile.
 constexpr int SLOT_MASK = 0x07FFFFFF;
 constexpr int CHUNK_SHIFT = 12;
 
