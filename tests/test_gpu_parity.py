@@ -65,6 +65,24 @@ def test_device_built_adjacency_and_colors_match_host(gen):
     assert np.array_equal(d.positions(), m.x0)
 
 
+def test_device_matches_the_committed_oracle_fixtures():
+    """tests/golden/oracle_small.npz: the beam after one frame (1e-5 rel) and the device's color
+    lists of the beam and the 16^3 cube (exact)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_small.npz"))
+    m = meshgen.beam()
+    got, d = run_gpu_frames(m, params_for(m, 100), 1)
+    rel = relmax(got[0], z["beam_x1"].astype(np.float64))
+    print(f"beam frame 1 vs fixture {rel:.3e}", flush=True)
+    assert rel <= TOL
+    for mesh, key in ((m, "beam_color_of"), (meshgen.noisy_cube(16, seed=1), "cube16_color_of")):
+        coff, cverts = upload(mesh).color_lists()
+        color_of = np.empty(mesh.num_verts, np.int32)
+        for c in range(len(coff) - 1):
+            color_of[cverts[coff[c]:coff[c + 1]]] = c
+        assert np.array_equal(color_of, z[key]), key
+
+
 def test_one_sweep_matches_oracle_vertex_solves():
     """One VBD_GPU_Solver::solveIteration vs the oracle's per-vertex solves, color by color."""
     m = meshgen.beam()

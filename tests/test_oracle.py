@@ -435,3 +435,28 @@ def test_reference_over_colors_sweep_with_one_thread_is_the_sequential_sweep():
         xo, _ = O.solve_subset(prep, p, x, y, vs)
         x[vs] = xo
     assert np.array_equal(got, x)
+
+
+def test_oracle_reproduces_the_committed_fixtures():
+    """tests/golden/oracle_small.npz (written by tests/golden/make_oracle_golden.py): per-corner
+    blocks and gradients of one and two tets in FP64, the beam after one frame in FP32, and two
+    greedy colorings.  The FP32 trajectory and the colorings must match bit for bit; the FP64
+    vectors to 1e-12.  The fixture's own consistency: closed form == faithful form, and the
+    corner gradients of each tet sum to zero."""
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_oracle_golden as G
+    z = np.load(os.path.join(here, "oracle_small.npz"))
+    now = G.build()
+    for k in ("beam_x1", "beam_color_of", "cube16_color_of", "tet1_dminv", "tet2_dminv", "tet1_vol", "tet2_vol"):
+        assert np.array_equal(now[k], z[k]), k
+    for pre in ("tet1_", "tet2_"):
+        for k in ("block", "grad", "block_faithful", "grad_faithful"):
+            a, b = now[pre + k], z[pre + k]
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), pre + k
+        assert np.abs(z[pre + "block"] - z[pre + "block_faithful"]).max() <= 1e-9 * np.abs(z[pre + "block"]).max()
+        assert np.abs(z[pre + "grad"] - z[pre + "grad_faithful"]).max() <= 1e-9 * np.abs(z[pre + "grad"]).max()
+        assert np.abs(z[pre + "grad"].sum(axis=1)).max() <= 1e-9 * np.abs(z[pre + "grad"]).max()
+
