@@ -107,6 +107,66 @@ def test_cpp_tetgen_loader_rejects_bad_files(tmp_path):
     assert r.returncode == 1
 
 
+def test_cpp_host_solver_helpers_match_the_fp64_closed_form(tmp_path):
+    """include/vbd/VBD.h host helpers on a deformed 8x2x2 beam: initializeColoring == the
+    oracle's greedy coloring, forwardStep's inertia target == float32 y = x + h (h g), and
+    solveVertex's H = m/h^2 I + sum H_kk, f = m/h^2 (y - x) - sum g_k against the oracle's FP64
+    closed form on the same FP32 DmInv (1e-5 of the largest block entry, 1e-4 of the largest
+    force)."""
+    m = meshgen.beam(8, 2, 2)
+    prefix = str(tmp_path / "b")
+    meshgen.write_tetgen(prefix, m.rest, m.tets)
+    rng = np.random.default_rng(3)
+    cur = (m.rest + rng.normal(scale=0.004, size=m.rest.shape) * (m.fixed == 0)[:, None]).astype(np.float32)
+    curl = [f"{m.num_verts} 3 0 0"] + [f"{i + 1} {p[0]!r} {p[1]!r} {p[2]!r}" for i, p in enumerate(cur.tolist())]
+    (tmp_path / "cur.node").write_text("\n".join(curl) + "\n")
+    exe = str(tmp_path / "host_solver_check")
+    lib = os.path.join(ROOT, "vbd_tetmesh_amd")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "host_solver_check.cpp"), "-L", lib, "-lvbd_hip",
+                        f"-Wl,-rpath,{lib}", "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = str(tmp_path / "o.bin")
+    r = subprocess.run([exe, prefix, str(tmp_path / "cur.node"), str(m.mu), str(m.lam), out], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    b = open(out, "rb").read()
+    nv, nc = np.frombuffer(b, np.int32, 2)
+    o = 8
+    color_of = np.frombuffer(b, np.int32, nv, o); o += 4 * nv
+    y = np.frombuffer(b, np.float32, 3 * nv, o).reshape(-1, 3); o += 12 * nv
+    Hc = np.frombuffer(b, np.float32, 9 * nv, o).reshape(-1, 3, 3); o += 36 * nv
+    fc = np.frombuffer(b, np.float32, 3 * nv, o).reshape(-1, 3)
+    prep = O.prepare(m)
+    assert np.array_equal(color_of, prep.color_of)
+    h = np.float32(1.0 / 60.0)
+    g = np.asarray(m.gravity, np.float32)
+    free = m.fixed == 0
+    y_ref = cur.copy()
+    y_ref[free] = cur[free] + h * (h * g)
+    assert np.array_equal(y, y_ref)
+    dminv, vol = O.precompute(m.rest, m.tets)
+    mass, _, _, _ = O.mesh_init(m.rest, m.tets, m.density)
+    sdt = float(np.float32(1.0 / np.float64(h * h)))
+    H_ref = np.zeros((nv, 3, 3))
+    f_ref = np.zeros((nv, 3))
+    for v in np.nonzero(free)[0]:
+        H_ref[v] = np.eye(3) * float(mass[v]) * sdt
+        f_ref[v] = float(mass[v]) * (y[v].astype(np.float64) - cur[v]) * sdt
+    for t, tet in enumerate(m.tets):
+        x12 = cur[tet].astype(np.float64).reshape(12)
+        for k, v in enumerate(tet):
+            if not free[v]:
+                continue
+            Hk, gk = O.tet_block(x12, dminv[t].astype(np.float64), m.mu, m.lam, float(vol[t]), k, faithful=False)
+            H_ref[v] += Hk
+            f_ref[v] -= gk
+    eH = np.abs(Hc - H_ref).max() / np.abs(H_ref).max()
+    ef = np.abs(fc - f_ref).max() / np.abs(f_ref).max()
+    print(f"host solveVertex vs FP64 closed form: H {eH:.2e}, f {ef:.2e}")
+    assert eH <= 1e-5 and ef <= 1e-4
+
+
 def test_demo_binary_is_built():
     assert os.path.exists(DEMO), "vbd_dropin_demo not built (run __graft_entry__.build())"
 
