@@ -427,8 +427,9 @@ __device__ __forceinline__ f4 corot_vertex_solve(const f4 xs, const f4 ys, const
 //   1. thread i takes records i, i+T, ... (K of them) of its tile: the whole block streams one
 //      contiguous range, every lane busy.  The record's owner is tileStart[tile] + the
 //      tile-local vertex index held in the owner bits.  The 9 contribution values (6 Hessian
-//      entries, 3 force entries) go to LDS, one array per value; the tile's CSR row offsets,
-//      positions, inertia targets and masses are staged in LDS too.
+//      entries, 3 force entries) go to LDS, 9 consecutive words per record (an odd stride, so
+//      consecutive records fall in distinct banks); the tile's CSR row offsets, positions,
+//      inertia targets and masses are staged in LDS too.
 //   2. after a barrier, TG = 8 lanes per vertex sum its records (lane l: records l, l+TG, ...
 //      of the vertex, then a DPP tree), so the summation order depends only on the vertex's
 //      valence, not on the tile cut, the grid or the block size: results are bitwise
@@ -599,15 +600,16 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
             A = compact_contrib(R.t[k].x < 0, R.q[k], __int_as_float(R.t[k].w), tmu, tlam, xs[k], ya[k], yb[k], yc[k]);
         }
         {  // padding records contribute zeros past the last vertex's range
-            red[0 * RT + q] = A.h00;
-            red[1 * RT + q] = A.h01;
-            red[2 * RT + q] = A.h02;
-            red[3 * RT + q] = A.h11;
-            red[4 * RT + q] = A.h12;
-            red[5 * RT + q] = A.h22;
-            red[6 * RT + q] = A.r0;
-            red[7 * RT + q] = A.r1;
-            red[8 * RT + q] = A.r2;
+            float* rq = red + 9 * q;  // record-major: one address, immediate offsets
+            rq[0] = A.h00;
+            rq[1] = A.h01;
+            rq[2] = A.h02;
+            rq[3] = A.h11;
+            rq[4] = A.h12;
+            rq[5] = A.h22;
+            rq[6] = A.r0;
+            rq[7] = A.r1;
+            rq[8] = A.r2;
         }
     }
     if (tid <= nv) lrow[tid] = vrow - vrow0;
@@ -628,15 +630,16 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     if (ok) {
         const int q1 = lrow[i + 1];
         for (int q = lrow[i] + lane; q < q1; q += TG) {
-            s0 += red[0 * RT + q];
-            s1 += red[1 * RT + q];
-            s2 += red[2 * RT + q];
-            s3 += red[3 * RT + q];
-            s4 += red[4 * RT + q];
-            s5 += red[5 * RT + q];
-            s6 += red[6 * RT + q];
-            s7 += red[7 * RT + q];
-            s8 += red[8 * RT + q];
+            const float* rq = red + 9 * q;
+            s0 += rq[0];
+            s1 += rq[1];
+            s2 += rq[2];
+            s3 += rq[3];
+            s4 += rq[4];
+            s5 += rq[5];
+            s6 += rq[6];
+            s7 += rq[7];
+            s8 += rq[8];
         }
     }
     s0 = gsum<TG>(s0); s1 = gsum<TG>(s1); s2 = gsum<TG>(s2);
