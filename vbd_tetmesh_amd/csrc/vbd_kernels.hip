@@ -430,13 +430,14 @@ __device__ __forceinline__ f4 corot_vertex_solve(const f4 xs, const f4 ys, const
 //      entries, 3 force entries) go to LDS, 9 consecutive words per record (an odd stride, so
 //      consecutive records fall in distinct banks); the tile's CSR row offsets, positions,
 //      inertia targets and masses are staged in LDS too.
-//   2. after a barrier, TG = 8 lanes per vertex sum its records (lane l: records l, l+TG, ...
+//   2. after a barrier, TG = 4 lanes per vertex sum its records (lane l: records l, l+TG, ...
 //      of the vertex, then a DPP tree), so the summation order depends only on the vertex's
 //      valence, not on the tile cut, the grid or the block size: results are bitwise
 //      identical across (K, blockDim) choices.  The group's last lane solves and writes the
 //      vertex.
 // ---------------------------------------------------------------------------------------
-constexpr int TG = 8;
+constexpr int TG = 4;    // lanes per vertex in the reduction stage
+constexpr int VDIV = 8;  // a tile holds fewer than blockDim.x / VDIV vertices
 constexpr int BUF_NT = 2;  // nontemporal cache policy bit of the raw buffer loads
 
 template <int K, bool MAT>
@@ -536,12 +537,12 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
                                                      const float* __restrict__ pc, const f2* __restrict__ pmat,
                                                      f4* __restrict__ save, float mu, float lambda, float sdt,
                                                      ContactArgs ct) {
-    // LDS: 9 * RT contribution values, then VM = T/TG entries (a tile has at most VM - 1
+    // LDS: 9 * RT contribution values, then VM = T/VDIV entries (a tile has at most VM - 1
     // vertices): row offsets (nv + 1 of them), x, y, m
     extern __shared__ float red[];
     const int T = (int)blockDim.x;
     const int RT = K * T;
-    const int VM = T / TG;
+    const int VM = T / VDIV;
     int* lrow = reinterpret_cast<int*>(red + 9 * RT);
     f4* lx = reinterpret_cast<f4*>(red + 9 * RT + VM);
     f4* ly = lx + VM;
@@ -663,7 +664,7 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
     const int ntiles = m.colorTileCount[color];
     if (ntiles <= 0) return hipSuccess;
     const size_t lds = sizeof(float) * 9 * (size_t)K * block +
-                       (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / TG);
+                       (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / VDIV);
     const int per = (ntiles + 7) / 8;
     if (COROT && !m.tpa) return hipErrorInvalidValue;  // full records not padded (build_tiles full)
     const size_t base = (size_t)m.colorTileBase[color] * m.tileRecords;
@@ -697,7 +698,7 @@ bool tiled_config_supported(int per_thread, int block) {
 hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int block, float sdt, float mu,
                               float lambda, const ContactArgs& ct, f4* save, hipStream_t s, bool corotated) {
     if (!tiled_config_supported(per_thread, block) || m.tileRecords != per_thread * block ||
-        m.tileVerts != block / TG - 1 || !m.tileStart || !m.tpt)
+        m.tileVerts != block / VDIV - 1 || !m.tileStart || !m.tpt)
         return hipErrorInvalidValue;
     if (corotated)
         return per_thread == 1 ? launch_tiled_k<1, true>(m, color, block, sdt, mu, lambda, ct, save, s)
