@@ -188,7 +188,8 @@ int vbd_set_material_model(vbd_solver* h, int32_t model);
  * mean valence is 20 or more) when the maximum valence allows it. */
 int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane);
 /* Autotuner keyed on the valence histogram: on the first mesh with a given key it times each
- * candidate mapping over `iterations` sweeps, restores the positions and keeps the fastest;
+ * candidate mapping over `iterations` sweeps launched back to back, restores the positions and
+ * keeps the fastest;
  * later meshes with the same key reuse the choice without timing (*best_ms = -1). */
 int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_unroll, int32_t* best_block,
                  float* best_ms);

@@ -1001,6 +1001,32 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
     return VBD_OK;
 }
 
+namespace {
+// Milliseconds of `iterations` sweeps (with the Chebyshev blends when the parameters enable
+// them) launched back to back between two events, as a replayed step runs them.
+hipError_t time_sweeps(vbd_solver* h, int iterations, double* ms) {
+    const vbd_step_params p = h->params;
+    const bool accel = h->paramsSet && p.use_acceleration;
+    const float hstep = h->paramsSet ? p.dt / (float)p.substeps : 1.0f / 60.0f;
+    const float sdt = inv_h2_ref(hstep);
+    const float mu = h->paramsSet ? p.mu : 1e5f, lam = h->paramsSet ? p.lambda : 4e5f;
+    const ContactArgs ct = h->paramsSet ? contact_of(p) : ContactArgs{};
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, h->stream);
+    if (e == hipSuccess) e = enqueue_iterations(h, iterations, sdt, mu, lam, accel, p.acceleration_rho, ct);
+    if (e == hipSuccess) e = hipEventRecord(b, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    *ms = t;
+    return e;
+}
+}  // namespace
+
 int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_unroll, int32_t* best_block,
                  float* best_ms) {
     int rc = check_ready(h);
@@ -1056,11 +1082,11 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
             h->lpv = cd.l;
             h->unroll = cd.u;
             h->block = b;
-            double cms = 0, bms = 0;
-            int cn = 0, bn = 0;
-            // one untimed warm sweep, then the measurement
-            rc = vbd_time_iterations(h, 1, &cms, &cn, &bms, &bn);
-            if (rc == VBD_OK) rc = vbd_time_iterations(h, iterations, &cms, &cn, &bms, &bn);
+            // one untimed warm sweep, then the sweeps back to back (as a replayed step runs them)
+            double cms = 0;
+            hipError_t te = time_sweeps(h, 1, &cms);
+            if (te == hipSuccess) te = time_sweeps(h, iterations, &cms);
+            rc = te == hipSuccess ? VBD_OK : hip_fail(h, "autotune timing", te);
             if (rc != VBD_OK) {
                 h->lpv = ol;
                 h->unroll = ou;
