@@ -45,6 +45,7 @@ struct VBDParams {
     int contactAxis = 2;
     Real contactStiffness = 1e4f;
     Real contactPlane = 0.f;
+    int energyLogEvery = 10;  // step() records the diagnostic energy every n-th iteration (VBD.cpp:92-94); 0: off
 };
 
 class VBDsolver {
@@ -85,6 +86,13 @@ class VBDsolver {
         vbd_throw_if(vbd_upload_velocities(h_, mesh.mVelocity.data()), h_, "step");
         stepDevice(1);
         syncToHost(mesh);
+        if (params.energyLogEvery > 0) {  // VBD.cpp:92-94: energies of this step, in order
+            int32_t n = 0;
+            vbd_throw_if(vbd_energy_log(h_, nullptr, 0, &n), h_, "energy log");
+            std::vector<double> e((size_t)n);
+            vbd_throw_if(vbd_energy_log(h_, e.data(), n, &n), h_, "energy log");
+            for (double v : e) energyHistory.push_back((Real)v);
+        }
     }
 
     // Advance `frames` steps without host transfers (state stays on the device).
@@ -248,6 +256,7 @@ class VBDsolver {
         p.contact_k = params.contactStiffness;
         p.contact_plane = params.contactPlane;
         vbd_throw_if(vbd_set_params(h_, &p), h_, "setParams");
+        vbd_throw_if(vbd_set_energy_log(h_, params.energyLogEvery), h_, "setParams");
     }
 
     void ensureDevice(const TetMesh& mesh) {

@@ -152,6 +152,14 @@ int vbd_update_velocity(vbd_solver* h);
 /* num_frames calls of VBDsolver::step; the step is captured once in a hipGraph and
  * replayed.  Asynchronous: returns after enqueueing; transfers or vbd_synchronize wait. */
 int vbd_step(vbd_solver* h, int32_t num_frames);
+/* VBDsolver::step's energy record (VBD.cpp:92-94): with every > 0, each step also evaluates
+ * the diagnostic energy of vbd_total_energy (materials: the step parameters, or per-tet) after
+ * iterations 0, every, 2 every, ... of each substep into a device log (the reference logs at
+ * every = 10); every <= 0 turns it off (the default).  Part of the captured step. */
+int vbd_set_energy_log(vbd_solver* h, int32_t every);
+/* The energy log of the last step: *count = substeps * ceil(iterations / every) values, of which
+ * min(count, capacity) are written to out.  Waits for the handle's stream. */
+int vbd_energy_log(vbd_solver* h, double* out, int32_t capacity, int32_t* count);
 int vbd_synchronize(vbd_solver* h);
 /* Enqueues all further work of the handle on `stream` (a hipStream_t of the handle's device,
  * e.g. a framework's current stream); NULL returns to the handle's own non-blocking stream.

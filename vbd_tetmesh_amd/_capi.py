@@ -126,6 +126,8 @@ SIGNATURES = [
     ("vbd_total_energy", C.c_int, [H, C.c_float, C.c_float, f64p]),
     ("vbd_incremental_potential", C.c_int, [H, f64p, f64p]),
     ("vbd_time_energy", C.c_int, [H, C.c_int32, C.c_int32, f64p, f64p]),
+    ("vbd_set_energy_log", C.c_int, [H, C.c_int32]),
+    ("vbd_energy_log", C.c_int, [H, f64p, C.c_int32, i32p]),
     ("vbd_set_material_model", C.c_int, [H, C.c_int32]),
     ("vbd_set_stream", C.c_int, [H, C.c_void_p]),
     ("vbd_set_materials", C.c_int, [H, f32p, f32p]),

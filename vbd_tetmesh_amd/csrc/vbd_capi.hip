@@ -39,6 +39,9 @@ struct vbd_solver {
     bool graphValid = false;
     double* d_energy = nullptr;   // 2 doubles: energy results
     double* d_epart = nullptr;    // per-block energy partial sums
+    int energyEvery = 0;          // vbd_step's energy log: every n-th iteration (0: off)
+    double* d_elog = nullptr;     // the log of the last step: substeps * ceil(iterations / n)
+    int elogCap = 0;
     std::vector<int32_t> meshVertOffsets;  // batched uploads: per-mesh vertex offsets
     // differentiation scratch, allocated on first use (slot order, nV entries)
     f4* d_w = nullptr;
@@ -207,8 +210,10 @@ hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const
 // the color kernels save each vertex's start-of-sweep position into save[it & 1]; after the
 // sweep, omega > 1 blends against save[(it + 1) & 1], the start of the previous sweep, which
 // is the prevPrevPos of the reference (the two buffers alternate instead of being copied).
+// With `elog`, the diagnostic energy (VBD.cpp:92-94) after iterations 0, every, 2 every, ...
+// goes to elog[it / every] (mu, lambda: the energy's material, the step parameters).
 hipError_t enqueue_iterations(vbd_solver* h, int iters, float sdt, float mu, float lambda, bool accel, float rho,
-                              const ContactArgs& ct) {
+                              const ContactArgs& ct, double* elog = nullptr, int every = 0) {
     float omega = 1.0f;
     for (int it = 0; it < iters; ++it) {
         f4* save = accel ? h->m.save[it & 1] : nullptr;
@@ -221,8 +226,17 @@ hipError_t enqueue_iterations(vbd_solver* h, int iters, float sdt, float mu, flo
                 if (e != hipSuccess) return e;
             }
         }
+        if (elog && every > 0 && it % every == 0) {
+            e = launch_energy(h->m, 0, mu, lambda, h->d_epart, elog + it / every, h->stream);
+            if (e != hipSuccess) return e;
+        }
     }
     return hipSuccess;
+}
+
+int energy_log_slots(const vbd_solver* h) {
+    if (h->energyEvery <= 0 || !h->paramsSet) return 0;
+    return h->params.substeps * ((h->params.num_iterations + h->energyEvery - 1) / h->energyEvery);
 }
 
 // One VBDsolver::step (VBD.cpp:60-101): substeps of forwardStep, iterations, updateVelocity.
@@ -238,8 +252,9 @@ hipError_t enqueue_step(vbd_solver* h) {
             e = launch_copy_inertia(h->m, h->stream);
             if (e != hipSuccess) return e;
         }
+        const int per = h->energyEvery > 0 ? (p.num_iterations + h->energyEvery - 1) / h->energyEvery : 0;
         e = enqueue_iterations(h, p.num_iterations, sdt, p.mu, p.lambda, p.use_acceleration != 0, p.acceleration_rho,
-                               ct);
+                               ct, h->d_elog ? h->d_elog + (size_t)sub * per : nullptr, h->energyEvery);
         if (e != hipSuccess) return e;
         e = launch_velocity(h->m, hstep, p.velocity_damping, h->stream);
         if (e != hipSuccess) return e;
@@ -403,6 +418,7 @@ void vbd_destroy(vbd_solver* h) {
     free_scratch(h);
     free_device_mesh(h->m);
     if (h->d_energy) (void)hipFree(h->d_energy);
+    if (h->d_elog) (void)hipFree(h->d_elog);
     (void)hipStreamSynchronize(h->ownStream);
     (void)hipStreamDestroy(h->ownStream);
     delete h;
@@ -861,6 +877,18 @@ int vbd_step(vbd_solver* h, int32_t num_frames) {
     if (num_frames < 0) return fail(h, VBD_ERR_ARG, "negative frame count");
     if (h->m.nFree == 0) return VBD_OK;  // every update of a step touches free vertices only
     DeviceGuard g(h->device);
+    if (h->energyEvery > 0) {  // buffers of the energy log exist before a capture
+        HIP_CHECK(h, ensure_energy_scratch(h));
+        const int n = energy_log_slots(h);
+        if (h->elogCap < n) {
+            drop_graph(h);
+            if (h->d_elog) (void)hipFree(h->d_elog);
+            h->d_elog = nullptr;
+            h->elogCap = 0;
+            HIP_CHECK(h, hipMalloc(&h->d_elog, sizeof(double) * (size_t)n));
+            h->elogCap = n;
+        }
+    }
     if (!h->graphEnabled) {
         for (int f = 0; f < num_frames; ++f) HIP_CHECK(h, enqueue_step(h));
         return VBD_OK;
@@ -884,6 +912,27 @@ int vbd_step(vbd_solver* h, int32_t num_frames) {
         h->graphValid = true;
     }
     for (int f = 0; f < num_frames; ++f) HIP_CHECK(h, hipGraphLaunch(h->graphExec, h->stream));
+    return VBD_OK;
+}
+
+int vbd_set_energy_log(vbd_solver* h, int32_t every) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    DeviceGuard g(h->device);
+    drop_graph(h);
+    h->energyEvery = every > 0 ? every : 0;
+    return VBD_OK;
+}
+
+int vbd_energy_log(vbd_solver* h, double* out, int32_t capacity, int32_t* count) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    const int n = h->d_elog ? std::min(energy_log_slots(h), h->elogCap) : 0;
+    if (count) *count = n;
+    if (capacity < 0 || (!out && capacity > 0)) return fail(h, VBD_ERR_ARG, "bad output buffer");
+    const int k = std::min(n, (int)capacity);
+    DeviceGuard g(h->device);
+    if (k > 0) HIP_CHECK(h, hipMemcpyAsync(out, h->d_elog, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
     return VBD_OK;
 }
 

@@ -45,6 +45,7 @@ class VBDParams:
     startFromInertia: bool = False   # True reproduces VBD_GPU_Wrapper.cu:334 (x <- y)
     contact: Optional[tuple] = None  # (axis, plane, stiffness) penalty half-space x[axis] >= plane
     material: int = A.VBD_MATERIAL_STABLE_NEO_HOOKEAN  # or A.VBD_MATERIAL_COROTATED_GPU
+    energyLogEvery: int = 10  # VBDsolver.step records the diagnostic energy every n-th iteration; 0: off
 
     def to_c(self) -> A.StepParams:
         p = A.StepParams()
@@ -403,6 +404,18 @@ class DeviceMesh(_Handle):
         check(lib.vbd_incremental_potential(self.h, C.byref(g), C.byref(e)), self.h)
         return g.value, e.value
 
+    def set_energy_log(self, every: int):
+        """Record the diagnostic energy after every `every`-th iteration of each step (0: off)."""
+        check(lib.vbd_set_energy_log(self.h, int(every)), self.h)
+
+    def energy_log(self) -> np.ndarray:
+        """The energies recorded by the last step (VBD.cpp:92-94)."""
+        n = C.c_int32()
+        check(lib.vbd_energy_log(self.h, None, 0, C.byref(n)), self.h)
+        out = np.zeros(max(n.value, 1), np.float64)
+        check(lib.vbd_energy_log(self.h, ptr(out, C.c_double), n.value, C.byref(n)), self.h)
+        return out[:n.value]
+
     def time_energy(self, mode: int = 0, reps: int = 20) -> Tuple[float, float]:
         """(ms per evaluation, algorithmic bytes per evaluation) of the energy reduction:
         mode 0 the log-Neo-Hookean diagnostic, mode 1 the incremental potential."""
@@ -520,6 +533,7 @@ class VBDsolver:
         self.params = params
         if self._dev is not None:
             self._dev.set_params(self._device_params())
+            self._dev.set_energy_log(params.energyLogEvery)
 
     def initializeColoring(self, mesh: TetMesh):
         """VBD.cpp:23-34 (greedy coloring on the host, GraphColoring.cpp:28-66)."""
@@ -553,6 +567,7 @@ class VBDsolver:
                            self.colorOffsets, self.colorVerts, inertia=mesh.mInertia, velocity=mesh.mVelocity,
                            tet_mu=mu, tet_lambda=lam)
             self._dev.set_params(self._device_params())
+            self._dev.set_energy_log(self.params.energyLogEvery)
             self._uploaded_ids = ids
 
     def step(self, mesh: TetMesh, per_tet_material: bool = False):
@@ -565,6 +580,8 @@ class VBDsolver:
         self._dev.set_velocities(mesh.mVelocity)
         self._dev.step(1)
         self.syncToHost(mesh)
+        if self.params.energyLogEvery > 0:  # VBD.cpp:92-94
+            self.energyHistory.extend(float(e) for e in self._dev.energy_log())
 
     def syncToHost(self, mesh: TetMesh):
         mesh.mVertPos[:] = self._dev.positions()
