@@ -570,12 +570,20 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     load_records<K, MAT, COROT>(R, src, 0, RT - 1, tid, T);
     const int v0 = tileStart[tile], v1 = tileStart[tile + 1];
     const int nv = v1 - v0;
-    // the tile's vertex data (clamped indices; lanes past nv load and ignore a copy)
-    const int vrow = row[min(v0 + tid, v1)], vrow0 = row[v0];
-    const int vs = min(v0 + tid, v1 - 1);
-    const f4 vx = pos[vs];
-    const f4 vy = __builtin_nontemporal_load(&inert[vs]);
-    const float vm = __builtin_nontemporal_load(&mass[vs]);
+    // the tile's vertex data, loaded by the first wave only (a tile has fewer than 64 vertices);
+    // clamped indices, lanes past nv load and ignore a copy
+    const bool wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) == 0;
+    int vrow = 0;
+    f4 vx = {}, vy = {};
+    float vm = 0.f;
+    if (wave0) {
+        vrow = row[min(v0 + tid, v1)];
+        const int vs = min(v0 + tid, v1 - 1);
+        vx = pos[vs];
+        vy = __builtin_nontemporal_load(&inert[vs]);
+        vm = __builtin_nontemporal_load(&mass[vs]);
+    }
+    const int vrow0 = row[v0];
     f4 xs[K], ya[K], yb[K], yc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
