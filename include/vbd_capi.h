@@ -192,13 +192,13 @@ int vbd_set_material_model(vbd_solver* h, int32_t model);
  *   block_size / 8 vertices); every thread takes pairs_per_lane records.  Supported:
  *   block_size 64-512 (a multiple of 64), pairs_per_lane 1 or 2, block_size * pairs_per_lane
  *   >= max valence.  Results are bitwise identical across tiled configurations.
- * The default after vbd_init is the tiled kernel with block 256 and 1 pair per lane (2 when the
- * mean valence is 20 or more) when the maximum valence allows it. */
+ * The default after vbd_init is the tiled kernel with block 256 and 1 pair per lane (block 512
+ * when the mean valence is 20 or more) when the maximum valence allows it. */
 int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block_size, int32_t pairs_per_lane);
 /* Autotuner keyed on the valence histogram: on the first mesh with a given key it times each
  * candidate mapping over `iterations` sweeps launched back to back, restores the positions and
- * keeps the fastest;
- * later meshes with the same key reuse the choice without timing (*best_ms = -1). */
+ * keeps the fastest, or the configuration set before the call when no candidate beats it by
+ * 1% or more; later meshes with the same key reuse the choice without timing (*best_ms = -1). */
 int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t* best_unroll, int32_t* best_block,
                  float* best_ms);
 /* Runs `iterations` plain sweeps (+ blends if use_acceleration in params) with HIP events

@@ -255,6 +255,34 @@ def test_tiled_configurations_are_bitwise_identical():
     assert rel <= TOL
 
 
+def test_autotune_restores_the_state_and_caches_its_choice():
+    """vbd_autotune times candidate sweeps on the live mesh: afterwards the positions and
+    velocities are the ones before the call, the chosen mapping continues the trajectory (bitwise
+    for a tiled choice, whose bits do not depend on the configuration; within the parity
+    tolerance otherwise), and a second mesh with the same valence key reuses the choice without
+    timing."""
+    m = meshgen.beam(13, 3, 3)
+    p = params_for(m, 30)
+    a = upload(m, params=p)
+    a.step(2)
+    x, v = a.positions(), a.velocities()
+    lanes, unroll, block, ms = a.autotune(2)
+    print(f"autotune beam13: lanes={lanes} pairs/lane={unroll} block={block} {ms:.4f} ms/sweep", flush=True)
+    assert ms > 0
+    assert np.array_equal(a.positions(), x) and np.array_equal(a.velocities(), v)
+    ref = upload(m, params=p)
+    ref.step(2)
+    assert np.array_equal(ref.positions(), x)
+    a.step(3)
+    ref.step(3)
+    if lanes == 0:
+        assert np.array_equal(a.positions(), ref.positions())
+    else:
+        assert relmax(a.positions(), ref.positions()) <= TOL
+    c = upload(m, params=p)
+    assert c.autotune(2) == (lanes, unroll, block, -1.0)
+
+
 def test_noisy_cube_32_matches_oracle():
     """The 128^3 workload's setup (noise, pins, Chebyshev) at 32^3, 20 iterations, 3 frames."""
     m = meshgen.noisy_cube(32, seed=0)
