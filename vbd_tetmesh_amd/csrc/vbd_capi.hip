@@ -1115,6 +1115,9 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     int bl = h->lpv, bu = h->unroll, bb = h->block;
     double best = 1e30, current = 1e30;  // current: the configuration set when autotune was called
     const int ol = h->lpv, ou = h->unroll, ob = h->block;
+    // the current tile cut stays allocated (out of the mesh) while candidates build their own
+    TileCut kept;
+    detach_tiles(h->m, kept);
     for (const Cand& cd : cands) {
         for (int b : cand_b) {
             if (e != hipSuccess) break;
@@ -1141,10 +1144,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
                 h->lpv = ol;
                 h->unroll = ou;
                 h->block = ob;
-                if (ol == 0) {
-                    std::string terr;
-                    (void)tiles_for(h, ou * ob, ob / 8 - 1, terr);
-                }
+                attach_tiles(h->m, kept, h->stream);
                 (void)launch_copy_f4(h->m.nV, keep, h->m.pos, h->stream);
                 (void)hipStreamSynchronize(h->stream);
                 (void)hipFree(keep);
@@ -1170,10 +1170,22 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(keep);
-    if (e != hipSuccess) return hip_fail(h, "autotune", e);
-    if (bl == 0) {
-        std::string terr;
-        if (tiles_for(h, bu * bb, bb / 8 - 1, terr) != hipSuccess) return fail(h, VBD_ERR_STATE, terr);
+    if (e != hipSuccess) {
+        h->lpv = ol;
+        h->unroll = ou;
+        h->block = ob;
+        attach_tiles(h->m, kept, h->stream);
+        drop_graph(h);
+        return hip_fail(h, "autotune", e);
+    }
+    if (bl == ol && bu == ou && bb == ob && kept.tileStart) {
+        attach_tiles(h->m, kept, h->stream);  // the current configuration won: its own cut back
+    } else {
+        free_tile_cut(kept);
+        if (bl == 0) {
+            std::string terr;
+            if (tiles_for(h, bu * bb, bb / 8 - 1, terr) != hipSuccess) return fail(h, VBD_ERR_STATE, terr);
+        }
     }
     h->lpv = bl;
     h->unroll = bu;

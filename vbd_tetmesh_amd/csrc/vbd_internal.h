@@ -106,6 +106,26 @@ hipError_t launch_color_tiled(const DevMesh& m, int color, int per_thread, int b
 // tile cut for the tiled kernel with (per_thread, block): records = per_thread * block,
 // max_verts = block / 8 - 1
 hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, std::string& err, bool full = false);
+// A tile cut (tile tables and padded records) moved out of a DevMesh and back: the autotuner
+// keeps the current cut allocated while its candidates build their own, and puts it back when
+// the current configuration wins (a rebuilt cut lands elsewhere in device memory, and the
+// first one measured faster).
+struct TileCut {
+    int* tileStart = nullptr;
+    int* tileRec = nullptr;
+    int tileRecords = 0, tileVerts = 0, tileEntries = 0;
+    i4* tpt = nullptr;
+    f4* tpq = nullptr;
+    f2* tpm = nullptr;
+    f4* tpa = nullptr;
+    f4* tpb = nullptr;
+    float* tpc = nullptr;
+    int64_t padBytes = 0;
+    std::vector<int> colorTileBase, colorTileCount;
+};
+void detach_tiles(DevMesh& m, TileCut& out);                  // m is left without a cut
+void attach_tiles(DevMesh& m, TileCut& in, hipStream_t s);     // frees m's cut, moves `in` back
+void free_tile_cut(TileCut& t);
 hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
 hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s);
 hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t s);

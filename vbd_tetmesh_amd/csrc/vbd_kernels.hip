@@ -893,6 +893,64 @@ hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, st
     return e;
 }
 
+void detach_tiles(DevMesh& m, TileCut& out) {
+    out.tileStart = m.tileStart;
+    out.tileRec = m.tileRec;
+    out.tileRecords = m.tileRecords;
+    out.tileVerts = m.tileVerts;
+    out.tileEntries = m.tileEntries;
+    out.tpt = m.tpt;
+    out.tpq = m.tpq;
+    out.tpm = m.tpm;
+    out.tpa = m.tpa;
+    out.tpb = m.tpb;
+    out.tpc = m.tpc;
+    out.padBytes = m.padBytes;
+    out.colorTileBase.swap(m.colorTileBase);
+    out.colorTileCount.swap(m.colorTileCount);
+    m.bytes -= m.padBytes;
+    m.tileStart = m.tileRec = nullptr;
+    m.tileRecords = m.tileVerts = m.tileEntries = 0;
+    m.tpt = nullptr;
+    m.tpq = nullptr;
+    m.tpm = nullptr;
+    m.tpa = m.tpb = nullptr;
+    m.tpc = nullptr;
+    m.padBytes = 0;
+    m.colorTileBase.clear();
+    m.colorTileCount.clear();
+}
+
+void free_tile_cut(TileCut& t) {
+    void* ps[] = {t.tileStart, t.tileRec, t.tpt, t.tpq, t.tpm, t.tpa, t.tpb, t.tpc};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    t = TileCut();
+}
+
+void attach_tiles(DevMesh& m, TileCut& in, hipStream_t s) {
+    (void)hipStreamSynchronize(s);
+    TileCut cur;
+    detach_tiles(m, cur);
+    free_tile_cut(cur);
+    m.tileStart = in.tileStart;
+    m.tileRec = in.tileRec;
+    m.tileRecords = in.tileRecords;
+    m.tileVerts = in.tileVerts;
+    m.tileEntries = in.tileEntries;
+    m.tpt = in.tpt;
+    m.tpq = in.tpq;
+    m.tpm = in.tpm;
+    m.tpa = in.tpa;
+    m.tpb = in.tpb;
+    m.tpc = in.tpc;
+    m.padBytes = in.padBytes;
+    m.bytes += in.padBytes;
+    m.colorTileBase.swap(in.colorTileBase);
+    m.colorTileCount.swap(in.colorTileCount);
+    in = TileCut();
+}
+
 template <int LPV, int U, bool MAT, bool CONTACT, bool SAVE>
 static hipError_t launch_color_t(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
                                  const ContactArgs& ct, f4* save, hipStream_t s) {
