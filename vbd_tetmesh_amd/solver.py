@@ -287,8 +287,11 @@ class DeviceMesh(_Handle):
         return i
 
     # -- transfers
-    def _down(self, fn) -> np.ndarray:
-        out = np.zeros((self.num_verts, 3), np.float32)
+    def _down(self, fn, out=None) -> np.ndarray:
+        """Into `out` when it is a C-contiguous float32 (N, 3) array, else into a new one."""
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float32 and out.flags.c_contiguous
+                and out.shape == (self.num_verts, 3) and out.flags.writeable):
+            out = np.empty((self.num_verts, 3), np.float32)
         check(fn(self.h, ptr(out, C.c_float)), self.h)
         return out
 
@@ -298,17 +301,17 @@ class DeviceMesh(_Handle):
             raise ValueError("vertex count mismatch")
         check(fn(self.h, ptr(a, C.c_float)), self.h)
 
-    def positions(self) -> np.ndarray:
-        return self._down(lib.vbd_download_positions)
+    def positions(self, out=None) -> np.ndarray:
+        return self._down(lib.vbd_download_positions, out)
 
-    def velocities(self) -> np.ndarray:
-        return self._down(lib.vbd_download_velocities)
+    def velocities(self, out=None) -> np.ndarray:
+        return self._down(lib.vbd_download_velocities, out)
 
-    def inertia(self) -> np.ndarray:
-        return self._down(lib.vbd_download_inertia)
+    def inertia(self, out=None) -> np.ndarray:
+        return self._down(lib.vbd_download_inertia, out)
 
-    def previous_positions(self) -> np.ndarray:
-        return self._down(lib.vbd_download_previous_positions)
+    def previous_positions(self, out=None) -> np.ndarray:
+        return self._down(lib.vbd_download_previous_positions, out)
 
     def set_positions(self, x):
         self._up(lib.vbd_upload_positions, x)
@@ -500,7 +503,9 @@ class VBD_GPU_Solver:
 
     def copyPositionsToCPU(self, mesh: TetMesh):
         """VBD_GPU_Wrapper.cu:364-366."""
-        mesh.mVertPos[:] = self._dev.positions()
+        got = self._dev.positions(out=mesh.mVertPos)
+        if got is not mesh.mVertPos:
+            mesh.mVertPos[:] = got
 
     def updateInertiaFromCPU(self, mesh: TetMesh):
         """VBD_GPU_Wrapper.cu:369-371."""
@@ -584,10 +589,11 @@ class VBDsolver:
             self.energyHistory.extend(float(e) for e in self._dev.energy_log())
 
     def syncToHost(self, mesh: TetMesh):
-        mesh.mVertPos[:] = self._dev.positions()
-        mesh.mVelocity[:] = self._dev.velocities()
-        mesh.mInertia[:] = self._dev.inertia()
-        mesh.mVertPrevPos[:] = self._dev.previous_positions()
+        for dst, fn in ((mesh.mVertPos, self._dev.positions), (mesh.mVelocity, self._dev.velocities),
+                        (mesh.mInertia, self._dev.inertia), (mesh.mVertPrevPos, self._dev.previous_positions)):
+            got = fn(out=dst)  # straight into the mesh's array when it is float32 (N, 3)
+            if got is not dst:
+                dst[:] = got
 
     def step_device(self, frames: int = 1):
         self._dev.step(frames)
