@@ -148,3 +148,21 @@ def test_batch_init_rejects_malformed_color_lists():
         assert ei.value.code == A.VBD_ERR_ARG
     DeviceMesh(0).init_batch(good)
 
+
+
+def test_large_transfers_round_trip_exactly():
+    """Vertex arrays of 1 MB or more cross through pinned host staging in chunks (the last one
+    partial here): upload then download returns the same bits, for positions, inertia and
+    velocities, into a new array or into the caller's."""
+    m = meshgen.noisy_cube(48, seed=4)
+    d = upload(m)
+    assert d.info().num_verts * 12 >= 1 << 20
+    print(f"[large transfers] nV={m.num_verts} ({m.num_verts * 12 / 2**20:.2f} MiB)", flush=True)
+    rng = np.random.default_rng(0)
+    for setter, getter in ((d.set_positions, d.positions), (d.set_inertia, d.inertia),
+                           (d.set_velocities, d.velocities)):
+        a = rng.normal(size=(m.num_verts, 3)).astype(np.float32)
+        setter(a)
+        assert np.array_equal(getter(), a)
+        out = np.zeros_like(a)
+        assert getter(out=out) is out and np.array_equal(out, a)
