@@ -344,7 +344,11 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="cube cells per side")
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the cube's launch configuration first (default: the library's default, which "
+                         "is the autotuner's choice for this mesh; timing candidates in the same process leaves "
+                         "the cube about 0.5%% slower)")
+    ap.add_argument("--no-autotune", action="store_true", help="the default (kept for older command lines)")
     ap.add_argument("--lanes", type=int, default=None, help="color-kernel lanes per vertex (0: tiled kernel)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--unroll", type=int, default=1)
@@ -390,7 +394,7 @@ def main():
     tune = None
     if args.lanes is not None:
         d.set_launch_config(args.lanes, args.block, args.unroll)
-    elif not args.no_autotune:
+    elif args.autotune and not args.no_autotune:
         d.set_positions(m.x0)
         tune = d.autotune(8)
         d.set_positions(m.x0)
