@@ -6,6 +6,8 @@
   block has |det H| <= 1e-8, so the vertex is skipped (VBD.cpp:141-153) and never moves.
 * A color made only of tet-less vertices (a color with no records), placed last.
 * Batched initialisation with malformed per-mesh color offsets: rejected before they are used.
+* Sizes past the ABI's limits, null arrays and bad tet indices: rejected with VBD_ERR_ARG.
+* Vertex arrays of 1 MB or more (pinned, chunked transfers): exact round trips.
 
 Tolerance: positions within 1e-5 relative max-norm of the FP32 oracle; skipped vertices and
 the coloring-independence checks are exact.
@@ -166,3 +168,40 @@ def test_large_transfers_round_trip_exactly():
         assert np.array_equal(getter(), a)
         out = np.zeros_like(a)
         assert getter(out=out) is out and np.array_equal(out, a)
+
+
+def test_init_rejects_sizes_past_the_limits_and_bad_indices():
+    """vbd_init validates sizes before it reads any array (num_verts < 2^27, 4 num_tets < 2^31,
+    no negative counts, no null arrays) and the tet indices before it uploads (in range, four
+    distinct corners); each failure is VBD_ERR_ARG and leaves the handle usable."""
+    import ctypes as C
+    from vbd_tetmesh_amd import _capi as A
+    d = DeviceMesh(0)
+
+    def init(desc):
+        return A.lib.vbd_init(d.h, C.byref(desc))
+
+    off = np.array([0, 0], np.int32)
+    for nv, nt in ((1 << 27, 0), (0, 1 << 29), (-1, 0), (0, -1)):
+        desc = A.MeshDesc()
+        desc.num_verts, desc.num_tets, desc.num_colors = nv, nt, 1
+        desc.color_offsets = off.ctypes.data_as(C.POINTER(C.c_int32))
+        assert init(desc) == A.VBD_ERR_ARG, (nv, nt)
+    desc = A.MeshDesc()
+    desc.num_verts, desc.num_colors = 4, 1
+    assert init(desc) == A.VBD_ERR_ARG  # required arrays null
+    m = meshgen.beam(2, 1, 1)
+    t, coff, cverts, dminv, vol = prepare_host(m)
+    for bad in ("range", "repeat"):
+        tets = t.tets.copy()
+        if bad == "range":
+            tets[3, 2] = m.num_verts
+        else:
+            tets[5, 1] = tets[5, 0]
+        with pytest.raises(A.VBDError) as ei:
+            DeviceMesh(0).init(t.mVertPos, t.vertexMass, t.isFixed, tets, dminv, vol, coff, cverts)
+        assert ei.value.code == A.VBD_ERR_ARG
+    d.init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, coff, cverts)
+    d.set_params(VBDParams(dt=H, numIterations=5))
+    d.step(1)
+    assert np.all(np.isfinite(d.positions()))
