@@ -255,19 +255,24 @@ def test_tiled_configurations_are_bitwise_identical():
     assert rel <= TOL
 
 
-def test_autotune_restores_the_state_and_caches_its_choice():
+@pytest.mark.parametrize("gen", ["beam13", "cube20", "cube64"])
+def test_autotune_restores_the_state_and_caches_its_choice(gen):
     """vbd_autotune times candidate sweeps on the live mesh: afterwards the positions and
     velocities are the ones before the call, the chosen mapping continues the trajectory (bitwise
     for a tiled choice, whose bits do not depend on the configuration; within the parity
     tolerance otherwise), and a second mesh with the same valence key reuses the choice without
     timing."""
-    m = meshgen.beam(13, 3, 3)
+    # the small meshes pick a per-vertex mapping; the 64^3 cube picked another tiled
+    # configuration (a new tile cut) when this was written: both continue the trajectory
+    m = meshgen.beam(13, 3, 3) if gen == "beam13" else meshgen.noisy_cube(int(gen[4:]), seed=7)
     p = params_for(m, 30)
     a = upload(m, params=p)
     a.step(2)
     x, v = a.positions(), a.velocities()
+    before = (a.info().lanes_per_vertex, a.info().pairs_per_lane, a.info().block_size)
     lanes, unroll, block, ms = a.autotune(2)
-    print(f"autotune beam13: lanes={lanes} pairs/lane={unroll} block={block} {ms:.4f} ms/sweep", flush=True)
+    print(f"autotune {gen}: {before} -> lanes={lanes} pairs/lane={unroll} block={block} {ms:.4f} ms/sweep "
+          f"({'current cut kept' if (lanes, unroll, block) == before else 'new configuration'})", flush=True)
     assert ms > 0
     assert np.array_equal(a.positions(), x) and np.array_equal(a.velocities(), v)
     ref = upload(m, params=p)
