@@ -324,11 +324,15 @@ __device__ __forceinline__ void corot_accumulate(const bool corner0, const f4 a,
     // IEEE division, and every instantiation still rounds alike (no contraction by the compiler).
 #pragma clang fp contract(off)
     const Kin k = pair_kinematics(corner0, a, b, c, xs, ya, yb, yc);
-    // the first Higham step reuses cof(F)
+    // Higham steps R <- (R + R^-T) / 2 carried as U_k = 2^k R_k, so that a step is
+    // U <- U + cof(U) * (4^k / det U) without halving every entry: cof(U) = 4^k cof(R),
+    // det U = 8^k det R, and the singularity test scales by 8^k; all the scalings are powers of
+    // two (exact), and R = 2^-k U at the end.  The first step reuses cof(F).
     float R00 = k.F00, R01 = k.F01, R02 = k.F02, R10 = k.F10, R11 = k.F11, R12 = k.F12;
     float R20 = k.F20, R21 = k.F21, R22 = k.F22;
     float C00 = k.C00, C01 = k.C01, C02 = k.C02, C10 = k.C10, C11 = k.C11, C12 = k.C12;
     float C20 = k.C20, C21 = k.C21, C22 = k.C22;
+    float back = 1.0f;
 #pragma unroll
     for (int it = 0; it < 5; ++it) {
         if (it > 0) {
@@ -342,16 +346,20 @@ __device__ __forceinline__ void corot_accumulate(const bool corner0, const f4 a,
             C21 = __builtin_fmaf(R02, R10, -(R00 * R12));
             C22 = __builtin_fmaf(R00, R11, -(R01 * R10));
         }
-        // det R by its first row and cofactors (R^-T = cof R / det R)
+        // det U by its first row and cofactors (U^-T = cof U / det U)
         const float det = __builtin_fmaf(R00, C00, __builtin_fmaf(R01, C01, R02 * C02));
-        if (fabsf(det) < 1e-8f) break;
-        const float inv = __builtin_amdgcn_rcpf(det);
-        R00 = 0.5f * __builtin_fmaf(C00, inv, R00); R01 = 0.5f * __builtin_fmaf(C01, inv, R01);
-        R02 = 0.5f * __builtin_fmaf(C02, inv, R02); R10 = 0.5f * __builtin_fmaf(C10, inv, R10);
-        R11 = 0.5f * __builtin_fmaf(C11, inv, R11); R12 = 0.5f * __builtin_fmaf(C12, inv, R12);
-        R20 = 0.5f * __builtin_fmaf(C20, inv, R20); R21 = 0.5f * __builtin_fmaf(C21, inv, R21);
-        R22 = 0.5f * __builtin_fmaf(C22, inv, R22);
+        if (fabsf(det) < 1e-8f * (float)(1 << (3 * it))) break;
+        const float inv = (float)(1 << (2 * it)) * __builtin_amdgcn_rcpf(det);
+        R00 = __builtin_fmaf(C00, inv, R00); R01 = __builtin_fmaf(C01, inv, R01);
+        R02 = __builtin_fmaf(C02, inv, R02); R10 = __builtin_fmaf(C10, inv, R10);
+        R11 = __builtin_fmaf(C11, inv, R11); R12 = __builtin_fmaf(C12, inv, R12);
+        R20 = __builtin_fmaf(C20, inv, R20); R21 = __builtin_fmaf(C21, inv, R21);
+        R22 = __builtin_fmaf(C22, inv, R22);
+        back *= 0.5f;
     }
+    R00 *= back; R01 *= back; R02 *= back;
+    R10 *= back; R11 *= back; R12 *= back;
+    R20 *= back; R21 *= back; R22 *= back;
     const float tr = __builtin_fmaf(R00, k.F00, __builtin_fmaf(R01, k.F01, __builtin_fmaf(R02, k.F02,
                      __builtin_fmaf(R10, k.F10, __builtin_fmaf(R11, k.F11, __builtin_fmaf(R12, k.F12,
                      __builtin_fmaf(R20, k.F20, __builtin_fmaf(R21, k.F21, R22 * k.F22)))))))) - 3.0f;
