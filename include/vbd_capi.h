@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define VBD_CAPI_VERSION 1
+#define VBD_CAPI_VERSION 2
 
 enum vbd_status {
     VBD_OK = 0,
@@ -51,6 +51,12 @@ enum vbd_status {
 };
 
 typedef struct vbd_solver vbd_solver;
+
+/* Graphs of vbd_solve calls and of the converged-step / adjoint-solve iteration blocks are cached
+ * per handle, keyed by the scalars they bake in.  A key is captured on its second use (the first
+ * runs the same kernels directly), and at most this many are kept (least recently used evicted),
+ * so callers that vary dt or the material scalars on every call hold no growing set of graphs. */
+#define VBD_MAX_CACHED_GRAPHS 32
 
 /* Mesh as handed to vbd_init.  All arrays are borrowed for the duration of the call. */
 typedef struct vbd_mesh_desc {
@@ -105,6 +111,11 @@ typedef struct vbd_info {
     int32_t pairs_per_lane;
     int32_t block_size;
     int64_t device_bytes;       /* bytes of device memory the handle holds */
+    int32_t cached_graphs;      /* instantiated hipGraphs the handle keeps (the step's, plus at most
+                                   VBD_MAX_CACHED_GRAPHS of vbd_solve / converged-step / adjoint blocks) */
+    int32_t full_records;       /* 1 while the 52-byte full pair records are resident (built on first
+                                   use by the corotated material, the per-vertex kernels and the
+                                   differentiation entry points) */
 } vbd_info;
 
 /* ---- lifecycle -------------------------------------------------------------------- */

@@ -72,6 +72,28 @@ def test_all_pinned_mesh_does_not_move():
     assert np.array_equal(d.positions(), m.x0)
 
 
+def test_all_pinned_step_still_runs_forward_step_and_the_energy_record():
+    """VBDsolver::step with every vertex pinned: forwardStep still sets x_prev = x and y = x
+    (TetMesh.h:332-348) -- here after the caller moved the pinned vertices kinematically --
+    and the energy is recorded every 10 iterations (VBD.cpp:92-94)."""
+    m = meshgen.beam()
+    m.fixed[:] = 1
+    print(f"[all pinned, moved] nV={m.num_verts}", flush=True)
+    d = upload(m, params=params_for(m, 20))
+    d.set_energy_log(10)
+    moved = (m.x0 * np.float32(1.01) + np.float32(0.01)).astype(np.float32)  # stretched: nonzero energy
+    d.set_positions(moved)
+    d.step(1)
+    assert np.array_equal(d.positions(), moved)
+    assert np.array_equal(d.previous_positions(), moved)
+    assert np.array_equal(d.inertia(), moved)
+    assert np.array_equal(d.velocities(), np.zeros_like(moved))
+    log = d.energy_log()
+    assert log.shape == (2,)
+    e = d.total_energy(m.mu, m.lam)
+    assert np.all(log == e), (log, e)
+
+
 @pytest.mark.parametrize("lanes", [None, 8])
 def test_singular_blocks_are_skipped_like_the_oracle(lanes):
     m, loose = beam_with_loose_parts()
@@ -205,3 +227,12 @@ def test_init_rejects_sizes_past_the_limits_and_bad_indices():
     d.set_params(VBDParams(dt=H, numIterations=5))
     d.step(1)
     assert np.all(np.isfinite(d.positions()))
+    # a bad slot order is rejected before the handle's current mesh is released: it still steps
+    x1 = d.positions()
+    for so in (-1, 3):
+        with pytest.raises(A.VBDError) as ei:
+            d.init(t.mVertPos, t.vertexMass, t.isFixed, t.tets, dminv, vol, coff, cverts, slot_order=so)
+        assert ei.value.code == A.VBD_ERR_ARG
+    assert np.array_equal(d.positions(), x1)
+    d.step(1)
+    assert np.all(np.isfinite(d.positions())) and not np.array_equal(d.positions(), x1)

@@ -84,6 +84,8 @@ class Info(C.Structure):
         ("pairs_per_lane", C.c_int32),
         ("block_size", C.c_int32),
         ("device_bytes", C.c_int64),
+        ("cached_graphs", C.c_int32),
+        ("full_records", C.c_int32),
     ]
 
 

@@ -32,8 +32,15 @@ struct vbd_solver {
     int block = 256;
     int materialModel = VBD_MATERIAL_STABLE_NEO_HOOKEAN;
     bool graphEnabled = true;
-    // graphs of the converged-step and adjoint-solve iteration blocks, keyed by their content
-    std::map<std::string, hipGraphExec_t> blockGraphs;
+    // graphs of vbd_solve calls and of the converged-step / adjoint-solve iteration blocks, keyed
+    // by their content; at most VBD_MAX_CACHED_GRAPHS, least recently used evicted
+    struct BlockGraph {
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::map<std::string, BlockGraph> blockGraphs;
+    std::map<std::string, uint64_t> blockSeen;  // keys run once without a graph (captured on reuse)
+    uint64_t blockClock = 0;
     double* h_res = nullptr;  // pinned host copy of a block's residual
     hipGraphExec_t graphExec = nullptr;
     bool graphValid = false;
@@ -94,36 +101,68 @@ void drop_graph(vbd_solver* h) {
     if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
     h->graphExec = nullptr;
     h->graphValid = false;
-    for (auto& kv : h->blockGraphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : h->blockGraphs) (void)hipGraphExecDestroy(kv.second.exec);
     h->blockGraphs.clear();
+    h->blockSeen.clear();
 }
 
-// Runs the work `enqueue` puts on the handle's stream, replaying a graph captured on first use
-// under `key` (graphs enabled) or launching it directly, then waits (if `wait`); the same kernels run either
-// way, so results are identical.  The key must name everything the work bakes into arguments
-// that can change between calls (the buffers are fixed per handle; set_params, launch
-// configuration, material changes and re-initialisation drop the cache).
+// Keys remembered as run once: a key seen again within this many distinct keys is captured.
+constexpr size_t BLOCK_SEEN_CAP = 4 * VBD_MAX_CACHED_GRAPHS;
+
+template <class M>
+void evict_oldest(M& map) {
+    auto old = map.begin();
+    for (auto it = map.begin(); it != map.end(); ++it)
+        if (it->second < old->second) old = it;
+    map.erase(old);
+}
+
+// Runs the work `enqueue` puts on the handle's stream, then waits (if `wait`).  With graphs
+// enabled, a key's first use launches the work directly and a later use captures it once and
+// replays the graph; the same kernels run either way, so results are identical.  A caller that
+// changes the baked-in scalars on every call therefore never pays a capture, and the cache keeps
+// at most VBD_MAX_CACHED_GRAPHS graphs (least recently used evicted).  The key must name
+// everything the work bakes into arguments that can change between calls (the buffers are fixed
+// per handle; set_params, launch configuration, material changes and re-initialisation drop the
+// cache).
 template <class Fn>
 hipError_t run_block(vbd_solver* h, const std::string& key, Fn&& enqueue, bool wait = true) {
     hipError_t e = hipSuccess;
-    if (!h->graphEnabled) {
+    const uint64_t now = ++h->blockClock;
+    auto it = h->graphEnabled ? h->blockGraphs.find(key) : h->blockGraphs.end();
+    if (it != h->blockGraphs.end()) {
+        it->second.used = now;
+        e = hipGraphLaunch(it->second.exec, h->stream);
+    } else if (!h->graphEnabled || h->blockSeen.find(key) == h->blockSeen.end()) {
+        if (h->graphEnabled) {
+            if (h->blockSeen.size() >= BLOCK_SEEN_CAP) evict_oldest(h->blockSeen);
+            h->blockSeen.emplace(key, now);
+        }
         e = enqueue();
     } else {
-        auto it = h->blockGraphs.find(key);
-        if (it == h->blockGraphs.end()) {
-            hipGraph_t graph = nullptr;
-            e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        h->blockSeen.erase(key);
+        if (h->blockGraphs.size() >= (size_t)VBD_MAX_CACHED_GRAPHS) {
+            auto old = h->blockGraphs.begin();
+            for (auto jt = h->blockGraphs.begin(); jt != h->blockGraphs.end(); ++jt)
+                if (jt->second.used < old->second.used) old = jt;
+            // a replay of the evicted graph may still be in flight
+            e = hipStreamSynchronize(h->stream);
             if (e != hipSuccess) return e;
-            e = enqueue();
-            hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
-            if (e == hipSuccess) e = e2;
-            hipGraphExec_t exec = nullptr;
-            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (graph) (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return e;
-            it = h->blockGraphs.emplace(key, exec).first;
+            (void)hipGraphExecDestroy(old->second.exec);
+            h->blockGraphs.erase(old);
         }
-        e = hipGraphLaunch(it->second, h->stream);
+        hipGraph_t graph = nullptr;
+        e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        e = enqueue();
+        hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+        if (e == hipSuccess) e = e2;
+        hipGraphExec_t exec = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return e;
+        h->blockGraphs.emplace(key, vbd_solver::BlockGraph{exec, now});
+        e = hipGraphLaunch(exec, h->stream);
     }
     if (e == hipSuccess && wait) e = hipStreamSynchronize(h->stream);
     return e;
@@ -404,6 +443,20 @@ hipError_t tiles_for(vbd_solver* h, int records, int max_verts, std::string& err
     return build_tiles(h->m, records, max_verts, h->stream, err, h->materialModel == VBD_MATERIAL_COROTATED_GPU);
 }
 
+// The 52-byte full pair records are read by the per-vertex kernels (lanes > 0), the corotated
+// material and the differentiation passes; the default tiled stable Neo-Hookean kernel reads the
+// 32-byte compact records only.  They are built on first use and released when the handle's
+// configuration stops reading them.
+bool needs_full_records(const vbd_solver* h) {
+    return h->lpv != 0 || h->materialModel == VBD_MATERIAL_COROTATED_GPU || h->d_w != nullptr;
+}
+
+void release_full_records_if_unused(vbd_solver* h) {
+    if (needs_full_records(h) || (!h->m.pa && !h->m.tpa)) return;
+    drop_graph(h);  // captured launches may reference them
+    release_full_records(h->m, h->stream);
+}
+
 hipError_t ensure_energy_scratch(vbd_solver* h) {
     if (h->d_epart) return hipSuccess;
     return hipMalloc(&h->d_epart, sizeof(double) * ((size_t)energy_partials(h->m) + SUM_STAGE));
@@ -412,7 +465,7 @@ hipError_t ensure_energy_scratch(vbd_solver* h) {
 // Default launch configuration: the tiled kernel with the first tile in the preference order
 // below that holds the largest valence; else the per-vertex kernel with the histogram
 // heuristic.
-void choose_default_config(vbd_solver* h) {
+hipError_t choose_default_config(vbd_solver* h, std::string& err) {
     // block 512 with 1 record per thread when the mean valence is 20 or more (a 256-record tile
     // would hold fewer than 13 vertices): the autotuner's choice on the 128^3 cube and the 96^3
     // sphere (k = 23.4, 22.9), 3.4% faster than 256 x 2 on the cube; block 256 with 1 on the
@@ -432,12 +485,13 @@ void choose_default_config(vbd_solver* h) {
             h->lpv = 0;
             h->unroll = c[1];
             h->block = c[0];
-            return;
+            return hipSuccess;
         }
     }
     h->lpv = heuristic_lpv(h->m);
     h->unroll = 1;
     h->block = 256;
+    return ensure_full_records(h->m, h->stream, err);
 }
 
 }  // namespace
@@ -519,6 +573,7 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
         return fail(h, VBD_ERR_ARG, "required mesh array is null");
     if ((d->tet_mu == nullptr) != (d->tet_lambda == nullptr))
         return fail(h, VBD_ERR_ARG, "tet_mu and tet_lambda must both be set or both be null");
+    if (d->slot_order < 0 || d->slot_order > 2) return fail(h, VBD_ERR_ARG, "slot_order must be 0, 1 or 2");
     // host-side validation of the index arrays
     for (int64_t i = 0; i < (int64_t)4 * nT; ++i)
         if (d->tets[i] < 0 || d->tets[i] >= nV) return fail(h, VBD_ERR_ARG, "tet vertex index out of range");
@@ -591,7 +646,6 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
     // id or a Morton (Z-order) key of the positions, so that consecutive slots -- and the
     // neighbour gathers of consecutive vertices -- are close in space.  Results do not depend on
     // the order inside a color (same-color vertices share no tet).
-    if (d->slot_order < 0 || d->slot_order > 2) return fail(h, VBD_ERR_ARG, "slot_order must be 0, 1 or 2");
     std::vector<int32_t> perm, slotOff;
     if (d->slot_order != 1 && nV > 0) {
         std::vector<uint64_t> key(nV);
@@ -666,7 +720,11 @@ int vbd_init(vbd_solver* h, const vbd_mesh_desc* d) {
                                      perm.empty() ? nullptr : perm.data(), slotOff.empty() ? nullptr : slotOff.data(),
                                      h->stream, err);
     if (e != hipSuccess) return fail(h, e == hipErrorOutOfMemory ? VBD_ERR_ALLOC : VBD_ERR_HIP, "mesh build: " + err);
-    choose_default_config(h);
+    e = choose_default_config(h, err);
+    if (e != hipSuccess) {
+        free_device_mesh(h->m);
+        return fail(h, e == hipErrorOutOfMemory ? VBD_ERR_ALLOC : VBD_ERR_HIP, "launch configuration: " + err);
+    }
     h->meshVertOffsets.assign({0, nV});
     h->inited = true;
     return VBD_OK;
@@ -783,6 +841,8 @@ int vbd_get_info(const vbd_solver* h, vbd_info* info) {
     info->pairs_per_lane = h->unroll;
     info->block_size = h->block;
     info->device_bytes = h->m.bytes;
+    info->cached_graphs = (int32_t)h->blockGraphs.size() + (h->graphExec ? 1 : 0);
+    info->full_records = h->m.pa != nullptr;
     return VBD_OK;
 }
 
@@ -942,7 +1002,9 @@ int vbd_step(vbd_solver* h, int32_t num_frames) {
     if (rc) return rc;
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if (num_frames < 0) return fail(h, VBD_ERR_ARG, "negative frame count");
-    if (h->m.nFree == 0) return VBD_OK;  // every update of a step touches free vertices only
+    // an empty mesh has nothing to update; with every vertex pinned the step still runs
+    // forwardStep (x_prev = x, y = x for pinned vertices, VBD.cpp:103-112) and the energy record
+    if (h->m.nV == 0) return VBD_OK;
     DeviceGuard g(h->device);
     if (h->energyEvery > 0) {  // buffers of the energy log exist before a capture
         HIP_CHECK(h, ensure_energy_scratch(h));
@@ -1017,9 +1079,13 @@ int vbd_set_material_model(vbd_solver* h, int32_t model) {
     DeviceGuard g(h->device);
     drop_graph(h);
     h->materialModel = model;
-    if (h->inited && h->lpv == 0) {
+    if (h->inited) {
         std::string err;
-        if (tiles_for(h, h->unroll * h->block, h->block / 8 - 1, err) != hipSuccess) return fail(h, VBD_ERR_STATE, err);
+        if (h->lpv == 0 && tiles_for(h, h->unroll * h->block, h->block / 8 - 1, err) != hipSuccess)
+            return fail(h, VBD_ERR_STATE, err);
+        if (needs_full_records(h) && ensure_full_records(h->m, h->stream, err) != hipSuccess)
+            return fail(h, VBD_ERR_ALLOC, err);
+        release_full_records_if_unused(h);
     }
     return VBD_OK;
 }
@@ -1041,11 +1107,16 @@ int vbd_set_launch_config(vbd_solver* h, int32_t lanes_per_vertex, int32_t block
                     "unsupported launch configuration: lanes_per_vertex 4/8/16 with pairs_per_lane 1-3, or 32/64 with "
                     "pairs_per_lane 1; block_size 64/128/192/256; or lanes_per_vertex 0 (tiled)");
     }
+    DeviceGuard g(h->device);
+    if (lanes_per_vertex > 0 && h->inited) {  // the per-vertex kernels read the full records
+        std::string err;
+        if (ensure_full_records(h->m, h->stream, err) != hipSuccess) return fail(h, VBD_ERR_ALLOC, err);
+    }
     h->lpv = lanes_per_vertex;
     h->unroll = pairs_per_lane;
     h->block = block_size;
-    DeviceGuard g(h->device);
     drop_graph(h);
+    if (h->inited) release_full_records_if_unused(h);
     return VBD_OK;
 }
 
@@ -1155,17 +1226,24 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
         auto it = g_tune_cache.find(key);
         std::string terr;
         if (it != g_tune_cache.end() &&
-            (it->second.l != 0 || tiles_for(h, it->second.u * it->second.b, it->second.b / 8 - 1, terr) == hipSuccess)) {
+            (it->second.l != 0 || tiles_for(h, it->second.u * it->second.b, it->second.b / 8 - 1, terr) == hipSuccess) &&
+            (it->second.l == 0 || ensure_full_records(h->m, h->stream, terr) == hipSuccess)) {
             h->lpv = it->second.l;
             h->unroll = it->second.u;
             h->block = it->second.b;
             drop_graph(h);
+            release_full_records_if_unused(h);
             if (best_lanes) *best_lanes = h->lpv;
             if (best_unroll) *best_unroll = h->unroll;
             if (best_block) *best_block = h->block;
             if (best_ms) *best_ms = -1.f;
             return VBD_OK;
         }
+    }
+    // the per-vertex candidates read the full records (released below if the winner does not)
+    {
+        std::string ferr;
+        if (ensure_full_records(h->m, h->stream, ferr) != hipSuccess) return fail(h, VBD_ERR_ALLOC, ferr);
     }
     // keep the state: timing sweeps move the positions and the Chebyshev buffers
     f4* keep = nullptr;
@@ -1257,6 +1335,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     h->unroll = bu;
     h->block = bb;
     drop_graph(h);
+    release_full_records_if_unused(h);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = {bl, bu, bb};
@@ -1361,6 +1440,8 @@ hipError_t ensure_scratch(vbd_solver* h) {
     const size_t n = (size_t)(h->m.nV > 0 ? h->m.nV : 1);
     const size_t t = (size_t)(h->m.nT > 0 ? h->m.nT : 1);
     hipError_t e = hipSuccess;
+    std::string err;  // the differentiation passes read the full records
+    if ((e = ensure_full_records(h->m, h->stream, err)) != hipSuccess) return e;
     if (!h->d_w && (e = hipMalloc(&h->d_w, sizeof(f4) * n)) != hipSuccess) return e;
     if (!h->d_rhs && (e = hipMalloc(&h->d_rhs, sizeof(f4) * n)) != hipSuccess) return e;
     if (!h->d_out && (e = hipMalloc(&h->d_out, sizeof(f4) * n)) != hipSuccess) return e;
@@ -1434,6 +1515,8 @@ int vbd_set_materials(vbd_solver* h, const float* tet_mu, const float* tet_lambd
     HIP_CHECK(h, hipMemcpyAsync(h->m.tmu, tet_mu, sizeof(float) * h->m.nT, hipMemcpyHostToDevice, h->stream));
     HIP_CHECK(h, hipMemcpyAsync(h->m.tlam, tet_lambda, sizeof(float) * h->m.nT, hipMemcpyHostToDevice, h->stream));
     HIP_CHECK(h, launch_update_pmat(h->m, h->stream));
+    // the tiled kernel reads the padded copy of the records of its tile cut
+    HIP_CHECK(h, launch_pad_pmat(h->m, h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
     return VBD_OK;
 }

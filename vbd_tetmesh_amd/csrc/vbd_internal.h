@@ -40,7 +40,7 @@ struct DevMesh {
     int* inv = nullptr;     // nV caller vertex id -> slot
     i4* ptet = nullptr;     // nPairs  3 corner slots (27 bits; bit 31 of .x: the vertex is corner 0;
                             //         bits 27-30 of .x/.y/.z: owner offset from chunkBase), V0 bits in .w
-    f4* pa = nullptr;       // nPairs  R0.xyz, R1.x  (three DmInv rows, see k_build_pairs)
+    f4* pa = nullptr;       // nPairs  R0.xyz, R1.x  (three DmInv rows, see k_build_pairs; built on first use)
     f4* pb = nullptr;       // nPairs  R1.yz, R2.xy
     float* pc = nullptr;    // nPairs  R2.z
     f2* pmat = nullptr;     // nPairs  mu, lambda (per-tet materials only)
@@ -156,6 +156,8 @@ hipError_t launch_norm2_f4(int n, const f4* v, double* r2, hipStream_t s);
 hipError_t launch_gravity_load2(const DevMesh& m, const float g[3], double* r2, hipStream_t s);
 hipError_t launch_param_grad(const DevMesh& m, const f4* w, float* dmu, float* dlam, hipStream_t s);
 hipError_t launch_update_pmat(const DevMesh& m, hipStream_t s);
+// the current tile cut's padded (mu, lambda) slots from pmat (after launch_update_pmat)
+hipError_t launch_pad_pmat(const DevMesh& m, hipStream_t s);
 hipError_t launch_blend_vec(int n, f4* v, const f4* pp, float omega, hipStream_t s);
 
 // Device mesh build.  Host inputs are already on the device (d_* pointers, caller numbering).
@@ -167,5 +169,10 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
                              const float* d_mass, const int* h_perm, const int* h_slot_off, hipStream_t s,
                              std::string& err);
 void free_device_mesh(DevMesh& m);
+// The 52-byte full pair records (pa, pb, pc) read by the per-vertex kernels, the corotated
+// material and the differentiation passes: built on first use, released when no kernel of the
+// handle's configuration reads them (release also frees the tile cut's padded copies).
+hipError_t ensure_full_records(DevMesh& m, hipStream_t s, std::string& err);
+void release_full_records(DevMesh& m, hipStream_t s);
 
 }  // namespace vbd

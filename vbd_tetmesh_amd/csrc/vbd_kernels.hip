@@ -787,6 +787,29 @@ __global__ __launch_bounds__(256) void k_pad_records(int nentries, int rt, const
     }
 }
 
+// The padded (mu, lambda) slots of the current tile cut rewritten from pmat, laid out as
+// k_pad_records lays them out (slot j of entry i: pmat[tileRec[i] + j] below the entry's
+// record count, else zero).  Run after the per-tet materials change.
+__global__ __launch_bounds__(256) void k_pad_pmat(int nentries, int rt, const int* __restrict__ tileRec,
+                                                  const f2* __restrict__ pmat, f2* __restrict__ tpm) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)nentries * rt) return;
+    const int i = (int)(g / rt), j = (int)(g % rt);
+    const int r0 = tileRec[i];
+    const int cnt = i + 1 < nentries ? tileRec[i + 1] - r0 : 0;
+    f2 mm = f2{0.f, 0.f};
+    if (j < cnt) mm = pmat[r0 + j];
+    tpm[g] = mm;
+}
+
+hipError_t launch_pad_pmat(const DevMesh& m, hipStream_t s) {
+    if (!m.tpm || !m.pmat || m.tileEntries <= 0 || m.tileRecords <= 0) return hipSuccess;
+    const int64_t total = (int64_t)m.tileEntries * m.tileRecords;
+    hipLaunchKernelGGL(k_pad_pmat, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m.tileEntries,
+                       m.tileRecords, m.tileRec, m.pmat, m.tpm);
+    return hipGetLastError();
+}
+
 void free_padded_records(DevMesh& m) {
     void* ps[] = {m.tpt, m.tpq, m.tpm, m.tpa, m.tpb, m.tpc};
     for (void* p : ps)
@@ -804,6 +827,10 @@ void free_padded_records(DevMesh& m) {
 // (Re)builds the padded record slots of the current tile cut; full adds the full records
 // (pa, pb, pc) the corotated material reads.
 hipError_t pad_records(DevMesh& m, bool full, hipStream_t s, std::string& err) {
+    if (full) {
+        hipError_t ef = ensure_full_records(m, s, err);
+        if (ef != hipSuccess) return ef;
+    }
     const int nentries = m.tileEntries;
     const size_t n = (size_t)std::max(nentries, 1) * (size_t)m.tileRecords;
     (void)hipStreamSynchronize(s);
@@ -979,6 +1006,7 @@ static hipError_t launch_color_t(const DevMesh& m, int color, int block, float s
     const int slot0 = m.colorSlotOff[color];
     const int count = m.colorSlotOff[color + 1] - slot0;
     if (count <= 0) return hipSuccess;
+    if (!m.pa) return hipErrorInvalidValue;  // full records not built (ensure_full_records)
     const int gpb = block / LPV;
     int nb = (count + gpb - 1) / gpb;
     nb = (nb + 7) & ~7;
@@ -1071,6 +1099,7 @@ static hipError_t launch_corot_t(const DevMesh& m, int color, int block, float s
     const int slot0 = m.colorSlotOff[color];
     const int count = m.colorSlotOff[color + 1] - slot0;
     if (count <= 0) return hipSuccess;
+    if (!m.pa) return hipErrorInvalidValue;  // full records not built (ensure_full_records)
     const int gpb = block / LPV;
     int nb = (count + gpb - 1) / gpb;
     nb = (nb + 7) & ~7;
@@ -1317,6 +1346,7 @@ static hipError_t launch_pass_t(const DevMesh& m, int slot0, int count, int bloc
                                 double* r2, f4* save, float mu, float lambda, float sdt, const ContactArgs& ct,
                                 hipStream_t s) {
     if (count <= 0) return hipSuccess;
+    if (!m.pa) return hipErrorInvalidValue;  // full records not built (ensure_full_records)
     const int gpb = block / LPV;
     int nb = (count + gpb - 1) / gpb;
     nb = (nb + 7) & ~7;
@@ -1995,12 +2025,14 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
         r.z = o[2] | (((own >> 8) & 15) << 27);
         r.w = __float_as_int(vol[t]);
         ptet[p] = r;
-        f4 a, b;
-        a.x = Q[0]; a.y = Q[1]; a.z = Q[2]; a.w = Q[3];
-        b.x = Q[4]; b.y = Q[5]; b.z = Q[6]; b.w = Q[7];
-        pa[p] = a;
-        pb[p] = b;
-        pc[p] = Q[8];
+        if (pa) {  // full records (built here only when they are resident already)
+            f4 a, b;
+            a.x = Q[0]; a.y = Q[1]; a.z = Q[2]; a.w = Q[3];
+            b.x = Q[4]; b.y = Q[5]; b.z = Q[6]; b.w = Q[7];
+            pa[p] = a;
+            pb[p] = b;
+            pc[p] = Q[8];
+        }
         {
             // compact record (tiled kernel), in double from the FP32 DmInv rows R_n:
             // q = -(R0+R1+R2) (corner 0) or R0, alpha_n = R_n . q, beta = -alpha_0 (corner 0) or
@@ -2027,6 +2059,89 @@ __global__ __launch_bounds__(256) void k_build_pairs(int nFree, const int* __res
             mm.y = tlam[t];
             pmat[p] = mm;
         }
+    }
+}
+
+// The full records (three DmInv rows per record, as k_build_pairs orders them) of every free
+// slot, from the per-tet arrays kept on the device (corner slots in caller tet order, DmInv
+// row-major): built on first use by the kernels that read them.
+__global__ __launch_bounds__(256) void k_build_full(int nFree, const int* __restrict__ perm,
+                                                    const int* __restrict__ off, const int* __restrict__ entry,
+                                                    const float* __restrict__ dminv, const int* __restrict__ row,
+                                                    f4* __restrict__ pa, f4* __restrict__ pb, float* __restrict__ pc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nFree) return;
+    const int v = perm[s];
+    int p = row[s];
+    for (int e = off[v]; e < off[v + 1]; ++e, ++p) {
+        const int t = entry[e] >> 2;
+        const int k = entry[e] & 3;
+        const float* D = dminv + 9 * (size_t)t;
+        // rows: k == 0 -> 0, 1, 2; else k-1, then the other two in tet order
+        int rows[3];
+        if (k == 0) {
+            rows[0] = 0; rows[1] = 1; rows[2] = 2;
+        } else {
+            rows[0] = k - 1;
+            int n = 1;
+            for (int j = 1; j < 4; ++j)
+                if (j != k) rows[n++] = j - 1;
+        }
+        f4 a, b;
+        a.x = D[3 * rows[0]]; a.y = D[3 * rows[0] + 1]; a.z = D[3 * rows[0] + 2]; a.w = D[3 * rows[1]];
+        b.x = D[3 * rows[1] + 1]; b.y = D[3 * rows[1] + 2]; b.z = D[3 * rows[2]]; b.w = D[3 * rows[2] + 1];
+        pa[p] = a;
+        pb[p] = b;
+        pc[p] = D[3 * rows[2] + 2];
+    }
+}
+
+hipError_t ensure_full_records(DevMesh& m, hipStream_t s, std::string& err) {
+    if (m.pa) return hipSuccess;
+    const size_t n = (size_t)std::max<int64_t>(m.nPairs, 1);
+    hipError_t e = hipMalloc(&m.pa, sizeof(f4) * n);
+    if (e == hipSuccess) e = hipMalloc(&m.pb, sizeof(f4) * n);
+    if (e == hipSuccess) e = hipMalloc(&m.pc, sizeof(float) * n);
+    if (e == hipSuccess && m.nFree > 0) {
+        hipLaunchKernelGGL(k_build_full, dim3((unsigned)((m.nFree + 255) / 256)), dim3(256), 0, s, m.nFree, m.perm,
+                           m.adjOff, m.adjEntry, m.dminv, m.row, m.pa, m.pb, m.pc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        err = std::string("full pair records: ") + hipGetErrorString(e);
+        void* ps[] = {m.pa, m.pb, m.pc};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+        m.pa = m.pb = nullptr;
+        m.pc = nullptr;
+        return e;
+    }
+    m.bytes += (int64_t)n * (int64_t)(2 * sizeof(f4) + sizeof(float));
+    return hipSuccess;
+}
+
+void release_full_records(DevMesh& m, hipStream_t s) {
+    (void)hipStreamSynchronize(s);
+    if (m.pa) {
+        const size_t n = (size_t)std::max<int64_t>(m.nPairs, 1);
+        void* ps[] = {m.pa, m.pb, m.pc};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+        m.pa = m.pb = nullptr;
+        m.pc = nullptr;
+        m.bytes -= (int64_t)n * (int64_t)(2 * sizeof(f4) + sizeof(float));
+    }
+    if (m.tpa) {  // the tile cut's padded copies of them
+        const size_t n = (size_t)std::max(m.tileEntries, 1) * (size_t)m.tileRecords;
+        void* ps[] = {m.tpa, m.tpb, m.tpc};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+        m.tpa = m.tpb = nullptr;
+        m.tpc = nullptr;
+        const int64_t b = (int64_t)n * (int64_t)(2 * sizeof(f4) + sizeof(float));
+        m.padBytes -= b;
+        m.bytes -= b;
     }
 }
 
@@ -2205,9 +2320,7 @@ hipError_t build_device_mesh(DevMesh& m, int nV, int nT, int nColors, const i4* 
         VBD_TRY(hipStreamSynchronize(s));
     }
     VBD_TRY(dalloc(m, &m.ptet, (size_t)m.nPairs));
-    VBD_TRY(dalloc(m, &m.pa, (size_t)m.nPairs));
-    VBD_TRY(dalloc(m, &m.pb, (size_t)m.nPairs));
-    VBD_TRY(dalloc(m, &m.pc, (size_t)m.nPairs));
+    // the full records (pa, pb, pc) are built on first use (ensure_full_records)
     if (m.perTetMat) VBD_TRY(dalloc(m, &m.pmat, (size_t)m.nPairs));
     VBD_TRY(dalloc(m, &m.pq, (size_t)m.nPairs));
     if (m.nFree > 0) {
