@@ -240,6 +240,19 @@ float vbd_accelerator_omega(int32_t order, float rho, float prev_omega);
  * log-Neo-Hookean energy (|det F| for the singular value product) over the current positions,
  * in double with a fixed summation order (bit-reproducible). */
 int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy);
+/* No reference counterpart: for callers that cannot allocate (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The energy of vbd_total_energy, same kernels
+ * and summation order (bitwise the same value), computed without any device or host allocation
+ * and without waiting: the per-block partial sums live in one library-wide device array of
+ * VBD_NOALLOC_SCRATCH_DOUBLES doubles per GPU (1 MiB, held until the process ends), and the
+ * energy is written on the handle's stream to energy_out, a device-accessible double (device
+ * memory or pinned host memory) that the caller reads after vbd_synchronize or an event.  Every
+ * partial sum is written before it is read, so nothing is cleared between calls.  Meshes whose
+ * partial sums do not fit (more than 33,488,896 tets or free vertices) are rejected with
+ * VBD_ERR_ARG.  All calls on one GPU share the array: calls that overlap on one GPU (other
+ * handles, other streams) race. */
+#define VBD_NOALLOC_SCRATCH_DOUBLES 131072
+int vbd_total_energy_noalloc(vbd_solver* h, float mu, float lambda, double* energy_out);
 /* The incremental potential the step minimises (no reference counterpart; the consistent
  * objective for convergence monitoring): G = sum_free m/(2h^2) |x - y|^2 + sum_tets V0 Psi(F)
  * (+ the penalty contact energy), Psi the stable Neo-Hookean (J-1) energy of Energy.cpp:93-97

@@ -126,6 +126,7 @@ SIGNATURES = [
     ("vbd_precompute_tet_data", C.c_int, [f32p, C.c_int32, i32p, f32p, f32p]),
     ("vbd_accelerator_omega", C.c_float, [C.c_int32, C.c_float, C.c_float]),
     ("vbd_total_energy", C.c_int, [H, C.c_float, C.c_float, f64p]),
+    ("vbd_total_energy_noalloc", C.c_int, [H, C.c_float, C.c_float, C.c_void_p]),
     ("vbd_incremental_potential", C.c_int, [H, f64p, f64p]),
     ("vbd_time_energy", C.c_int, [H, C.c_int32, C.c_int32, f64p, f64p]),
     ("vbd_set_energy_log", C.c_int, [H, C.c_int32]),

@@ -63,9 +63,37 @@ struct vbd_solver {
     hipEvent_t xev[16] = {};   // one per chunk of a staged device-to-host copy
 };
 
+// Scratch of vbd_total_energy_noalloc: the per-block partial sums of the energy reduction (and
+// the second stage of its fixed-order sum), one copy per GPU, never allocated or freed.  Host
+// code uses the address looked up per device (scratch_ptr), never the symbol itself.
+static __device__ __align__(256) double scratch[VBD_NOALLOC_SCRATCH_DOUBLES];
+
 namespace {
 
 thread_local std::string g_err;
+
+// Device address of `scratch` on each GPU, looked up once per device by vbd_create.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// With `device` current: looks up its copy of `scratch` unless that was done already.
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    double* p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+double* scratch_ptr(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 int fail(vbd_solver* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -526,6 +554,13 @@ int vbd_create(vbd_solver** out, int32_t device) {
         (void)hipStreamDestroy(h->ownStream);
         delete h;
         return hip_fail(nullptr, "hipMalloc", e);
+    }
+    e = scratch_setup(device);  // this device's copy of the no-allocation scratch
+    if (e != hipSuccess) {
+        (void)hipFree(h->d_energy);
+        (void)hipStreamDestroy(h->ownStream);
+        delete h;
+        return hip_fail(nullptr, "hipGetSymbolAddress(scratch)", e);
     }
     *out = h;
     return VBD_OK;
@@ -1370,6 +1405,19 @@ int vbd_total_energy(vbd_solver* h, float mu, float lambda, double* energy) {
     HIP_CHECK(h, launch_energy(h->m, 0, mu, lambda, h->d_epart, h->d_energy, h->stream));
     HIP_CHECK(h, hipMemcpyAsync(energy, h->d_energy, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_total_energy_noalloc(vbd_solver* h, float mu, float lambda, double* energy_out) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!energy_out) return fail(h, VBD_ERR_ARG, "null output");
+    if (energy_partials(h->m) + SUM_STAGE > VBD_NOALLOC_SCRATCH_DOUBLES)
+        return fail(h, VBD_ERR_ARG, "mesh too large for the no-allocation scratch (at most 33488896 tets and free vertices)");
+    double* ptr = scratch_ptr(h->device);
+    if (!ptr) return fail(h, VBD_ERR_STATE, "no-allocation scratch not set up on this device");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, launch_energy(h->m, 0, mu, lambda, ptr, energy_out, h->stream));
     return VBD_OK;
 }
 
