@@ -1,0 +1,82 @@
+"""GPU: whole frames at full size against the CPU oracle (VBD.cpp:60-101).
+
+* The 128^3 noisy cube (2,146,689 verts, 12,582,912 tets), 20 iterations, Chebyshev 0.95,
+  h = 1/60, 2 whole frames.
+* The 96^3 voxel sphere (485,385 verts, 2,780,400 tets, 11 colors, shuffled ids), 30
+  iterations, Chebyshev 0.95, with the penalty plane cutting its bottom cap so that contact is
+  active from the first sweep, 2 whole frames.
+
+Both run the oracle's closed-form backend in FP32 and FP64 on the host's cores (the host's CPU
+share: OMP_NUM_THREADS when set).  Tolerance: per frame max|x_gpu - x_oracle| / max|x_oracle|
+<= 1e-5 against both; the FP32 oracle's own distance from FP64 is printed beside it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vbd_tetmesh_amd import VBDParams, meshgen
+from vbd_tetmesh_amd.solver import upload
+
+pytestmark = pytest.mark.gpu
+
+H = 1.0 / 60.0
+TOL = 1e-5
+
+
+def host_threads() -> int:
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(share))) if share and share.isdigit() else n
+
+
+def relmax(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def whole_frames(m, iterations, frames, contact=None):
+    threads = host_threads()
+    prep = O.prepare(m)
+    p = O.make_params(dt=H, iterations=iterations, rho=0.95, mu=m.mu, lam=m.lam, gravity=m.gravity, contact=contact,
+                      threads=threads)
+    _, _, of32, s32 = O.simulate(prep, p, m.x0, None, frames)
+    _, _, of64, s64 = O.simulate(prep, p, m.x0, None, frames, f64=True, rest=m.rest, density=m.density)
+    params = VBDParams(dt=H, numIterations=iterations, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95,
+                       mu=m.mu, lambda_=m.lam, contact=contact)
+    d = upload(m, params=params)
+    got = []
+    for _ in range(frames):
+        d.step(1)
+        got.append(d.positions())
+    print(f"  oracle on {threads} threads: FP32 {s32:.1f} s, FP64 {s64:.1f} s", flush=True)
+    return np.stack(got), of32, of64
+
+
+@pytest.mark.timeout(900)
+def test_cube128_whole_frames_match_oracle():
+    m = meshgen.noisy_cube(128, seed=0)
+    print(f"[cube128 whole frames] nV={m.num_verts} nT={m.num_tets}", flush=True)
+    got, of32, of64 = whole_frames(m, 20, 2)
+    for f in range(2):
+        r32, r64, o = relmax(got[f], of32[f]), relmax(got[f], of64[f]), relmax(of32[f], of64[f])
+        disp = np.abs(of64[f] - m.x0).max()
+        print(f"cube128 frame {f + 1}: gpu vs fp32 oracle {r32:.2e}, vs fp64 {r64:.2e}, fp32 oracle vs fp64 {o:.2e} "
+              f"(max displacement {disp:.3e} m)", flush=True)
+        assert r32 <= TOL and r64 <= TOL
+
+
+@pytest.mark.timeout(900)
+def test_sphere96_whole_frames_with_contact_match_oracle():
+    m = meshgen.voxel_sphere(96, seed=0)
+    contact = (2, -0.95, 1e4)  # cuts the bottom cap: contact from the first sweep
+    touching = int((m.x0[:, 2] < contact[1]).sum())
+    print(f"[sphere96 whole frames] nV={m.num_verts} nT={m.num_tets}, {touching} vertices below the plane", flush=True)
+    assert touching > 0
+    got, of32, of64 = whole_frames(m, 30, 2, contact=contact)
+    for f in range(2):
+        r32, r64, o = relmax(got[f], of32[f]), relmax(got[f], of64[f]), relmax(of32[f], of64[f])
+        below = int((of64[f][:, 2] < contact[1]).sum())
+        print(f"sphere96 frame {f + 1}: gpu vs fp32 oracle {r32:.2e}, vs fp64 {r64:.2e}, fp32 oracle vs fp64 {o:.2e} "
+              f"({below} vertices still below the plane)", flush=True)
+        assert r32 <= TOL and r64 <= TOL

@@ -460,3 +460,63 @@ def test_oracle_reproduces_the_committed_fixtures():
         assert np.abs(z[pre + "grad"] - z[pre + "grad_faithful"]).max() <= 1e-9 * np.abs(z[pre + "grad"]).max()
         assert np.abs(z[pre + "grad"].sum(axis=1)).max() <= 1e-9 * np.abs(z[pre + "grad"]).max()
 
+
+
+def _free_particle_mesh(seed=0):
+    """A 4x2x2 beam with nothing pinned, seeded initial velocities (|v| ~ 2 m/s) and a deformed
+    start, so that y - x_t is far from zero and differs per vertex."""
+    m = meshgen.beam(4, 2, 2)
+    m.fixed[:] = 0
+    rng = np.random.default_rng(seed)
+    m.v0 = rng.normal(scale=2.0, size=m.x0.shape).astype(np.float32)
+    m.x0 = (m.x0 + rng.normal(scale=0.01, size=m.x0.shape)).astype(np.float32)
+    return m
+
+
+@pytest.mark.parametrize("start_from_inertia", [False, True])
+@pytest.mark.parametrize("rho", [0.75, 0.95])
+@pytest.mark.parametrize("iterations", [1, 2, 3, 4, 5, 20])
+def test_chebyshev_bookkeeping_is_the_free_particle_closed_form(iterations, rho, start_from_inertia):
+    """mu = lambda = 0, nothing pinned: every sweep maps x to y, so the FP64 oracle's step must
+    follow x_n = omega_{n+1} (y - x_{n-2}) + x_{n-2} with x_{-1} = x_t (VBD.cpp:72-81) or y
+    (VBD_GPU_Wrapper.cu:334), omega from VBD.cpp:302-311 and x_pp <- the positions before the
+    sweep (VBD.cpp:75-90), over 3 frames (x_pp is a solver member kept across steps).
+    Tolerance 1e-12 of max|x|; each plausible bookkeeping slip differs by far more."""
+    from closed_forms import chebyshev_free_particles
+    m = _free_particle_mesh()
+    h, g = 1.0 / 60.0, (0.0, -9.8, 0.0)
+    prep = O.prepare(m)
+    p = O.make_params(dt=h, iterations=iterations, rho=rho, mu=0.0, lam=0.0, gravity=g, accel=True,
+                      init_from_inertia=start_from_inertia)
+    _, _, got, _ = O.simulate(prep, p, m.x0, m.v0, 3, f64=True, rest=m.rest, density=m.density)
+    # the step parameters cross the ABI as float32 (VBDParams holds Real = float, Types.h:18)
+    f = lambda a: np.float64(np.float32(a))
+    h32, g32, rho32 = f(h), [f(c) for c in g], f(rho)
+    ref = chebyshev_free_particles(m.x0, m.v0, g32, h32, iterations, rho32, start_from_inertia=start_from_inertia,
+                                   frames=3)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    assert err <= 1e-12, err
+    # e_n = x_n - y obeys e_n = (1 - omega_{n+1}) e_{n-2} with e_0 = 0: every even-index iterate
+    # is y, so after an odd iteration count (and with x <- y, always) all variants end at y and
+    # no slip is visible; after an even count the last iterate carries e_{-1} = x_t - y
+    if start_from_inertia or iterations % 2 == 1:
+        return
+    for slip in ("pp_after_blend", "omega_n", "blend_vs_prev"):
+        bad = chebyshev_free_particles(m.x0, m.v0, g32, h32, iterations, rho32, frames=3, slip=slip)
+        assert np.abs(bad - ref).max() / scale > 1e-9, slip
+
+
+def test_chebyshev_closed_form_holds_across_substeps_and_in_fp32():
+    """The same closed form with 2 substeps (x_pp survives the substep boundary unused), and the
+    FP32 oracle within FP32 rounding (2e-6 of max|x|)."""
+    from closed_forms import chebyshev_free_particles
+    m = _free_particle_mesh(1)
+    h, g = 1.0 / 60.0, (0.0, -9.8, 0.0)
+    prep = O.prepare(m)
+    for f64, tol in ((True, 1e-12), (False, 2e-6)):
+        p = O.make_params(dt=h, iterations=4, substeps=2, rho=0.95, mu=0.0, lam=0.0, gravity=g)
+        _, _, got, _ = O.simulate(prep, p, m.x0, m.v0, 3, f64=f64, rest=m.rest, density=m.density)
+        f = lambda a: np.float64(np.float32(a))
+        ref = chebyshev_free_particles(m.x0, m.v0, [f(c) for c in g], f(h), 4, f(0.95), frames=3, substeps=2)
+        assert np.abs(got - ref).max() / np.abs(ref).max() <= tol
