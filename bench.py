@@ -47,23 +47,47 @@ def read_traffic(name: str):
     return None
 
 
-def cpu_baseline(m, target_seconds: float = 10.0):
+def host_cpu():
+    """The host's CPU as this process sees it: model string, nproc, the affinity set, and the
+    threads to use -- the affinity set, capped by OMP_NUM_THREADS where the host sets it (a GPU
+    host may allot each GPU a share of its cores that way)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = max(1, min(aff, int(share))) if share.isdigit() and int(share) > 0 else aff
+    return {"model": model, "nproc": nproc, "affinity": aff, "omp_num_threads": share or None, "threads": threads}
+
+
+def cpu_baseline(m, iterations: int, target_seconds: float = 10.0):
     """Reference-cost CPU restatement (oracle mode 2: full 12x12 element Hessian per
     (vertex, tet) call and a per-call std::vector, as VBD.cpp:205-256 / Energy.cpp:90-158)
     timed on a bounded sample of the same mesh: whole sweeps (every free vertex of every
-    color, colors in order) with all host threads inside each color (race-free; the result is
-    independent of the thread count), repeated to about `target_seconds`; plus one thread on
-    a smaller sample."""
+    color, colors in order) with the host's threads inside each color (race-free; the result
+    is independent of the thread count), repeated to about `target_seconds`; plus one thread
+    on a smaller sample, one sweep in the reference's own OpenMP-over-colors scheme, and the
+    diagnostic energy the reference's step evaluates every 10 iterations (VBD.cpp:92-94; serial
+    in the reference, VBD.cpp:258-288, so timed on one thread).  Rates count every vertex
+    (pinned included) per iteration, like the device value."""
     from oracle import oracle as O
     prep = O.prepare(m)
     y = m.x0.copy()
     y[m.fixed == 0] += np.float32(H * H) * np.asarray(m.gravity, np.float32)
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    cores = max(1, min(cores, 16))
+    cpu = host_cpu()
+    cores = cpu["threads"]
     colors = []
     for c in range(prep.n_colors):
         vs = prep.color_verts[prep.color_off[c]:prep.color_off[c + 1]]
         colors.append(vs[m.fixed[vs] == 0].astype(np.int32))
+    nfree = int((m.fixed == 0).sum())
+    nv = m.num_verts
 
     def run(threads, n_per_color):
         p = O.make_params(dt=H, iterations=1, rho=0.95, mu=m.mu, lam=m.lam, gravity=m.gravity,
@@ -89,18 +113,29 @@ def cpu_baseline(m, target_seconds: float = 10.0):
         sw += s
         cw += c
     sweeps = reps + 1
+    sweep_s = sw / sweeps
     # the reference's own scheme, OpenMP over colors (racy; at most one thread per color)
     pr = O.make_params(dt=H, iterations=1, rho=0.95, mu=m.mu, lam=m.lam, gravity=m.gravity,
                        mode=O.MODE_FAITHFUL_COST, threads=cores)
     _, sr = O.sweep_over_colors(prep, pr, m.x0, y)
-    nfree = int((m.fixed == 0).sum())
-    return {"value": cw / sw / 1e6, "unit": "M vertex-iters/s", "cores": cores, "kind": "port",
-            "sample": f"{sweeps} whole sweeps of the same 128^3 cube ({cw} vertex solves in {sw:.1f} s), reference "
+    # the diagnostic energy (|det F| for the SVD's singular value product: cheaper than the
+    # reference's JacobiSVD per tet), one evaluation on one thread
+    t0 = time.perf_counter()
+    O.total_energy(prep, pr, m.x0)
+    energy_s = time.perf_counter() - t0
+    per_free = nv / nfree  # free-vertex solves -> vertex-iterations (pinned included)
+    return {"value": cw * per_free / sw / 1e6, "unit": "M vertex-iters/s", "cores": cores, "kind": "port",
+            "sample": f"{sweeps} whole sweeps of the same mesh, {m.name} ({cw} vertex solves in {sw:.1f} s), reference "
                       f"cost structure (full 12x12 element Hessian per (vertex, tet) call, per-call std::vector), "
                       f"{cores} threads inside each color; energy diagnostic excluded",
-            "single_thread_value": c1 / s1 / 1e6,
+            "host_cpu": cpu,
+            "with_energy_every_10_value": nv / (sweep_s + energy_s / 10.0) / 1e6,
+            "energy_eval_s": round(energy_s, 3),
+            "energy_sample": f"one log-Neo-Hookean diagnostic over the {m.num_tets} tets on 1 thread ({energy_s:.2f} s), "
+                             f"added once per 10 iterations as VBD.cpp:92-94 does",
+            "single_thread_value": c1 * per_free / s1 / 1e6,
             "single_thread_sample": f"first {n1} free vertices of each color, 1 thread ({c1} solves in {s1:.1f} s)",
-            "reference_scheme_value": nfree / sr / 1e6,
+            "reference_scheme_value": nv / sr / 1e6,
             "reference_scheme_sample": f"one whole sweep with OpenMP over the {prep.n_colors} colors as VBD.cpp:119-121 "
                                        f"({cores} threads requested, racy; {nfree} solves in {sr:.1f} s)"}
 
@@ -171,37 +206,101 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
-def bench_sphere(n: int = 96, frames: int = 5, iterations: int = 30, slot_order: int = 0):
-    """Irregular workload: sphere voxelised on a 96^3 grid, interior vertices jittered, ids
-    shuffled, mu = 5e4, lambda = 2e5, gravity -z, penalty plane z = -1.2 (k = 1e4 N/m),
-    30 iterations/step, Chebyshev 0.95, dropped from rest."""
+def launch_roofline(d, sweeps: int, traffic_name=None):
+    """The color kernel against the HBM roofline: the step's color launches of `sweeps` sweeps
+    replayed from one hipGraph between two HIP events (vbd_time_color_launches), the bytes the
+    layout must move per launch (vbd_layout_bytes: records as stored + vertex data, gathers and
+    tile padding excluded), the measured HBM traffic per launch where a rocprofv3 counter
+    summary is committed, and the fixed 60 B-per-record yardstick of the reference's data model
+    reported apart (the compact records hold 32 B)."""
+    info = d.info()
+    ms, n = d.time_color_launches(sweeps)
+    lbytes, _ = d.layout_bytes()
+    ybytes, _ = d.algorithmic_bytes()
+    per_sweep = max(1, n // sweeps)
+    sec = ms / 1e3
+    achieved = lbytes / per_sweep / sec / 1e9
+    kernel = "k_color_tiled" if info.lanes_per_vertex == 0 else "k_color_solve"
+    traffic = read_traffic(traffic_name) if traffic_name else None
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "traffic_gbs": round(traffic / sec / 1e9, 1) if traffic else None,
+           "traffic_frac": round(traffic / sec / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+           "kernel": kernel, "alg_bytes_per_launch": round(lbytes / per_sweep),
+           "alg_bytes": "layout minimum: records as stored (32 B compact, +8 per-tet mu/lambda) + 72 B per free "
+                        "vertex (row 4, x 16 read + 16 written, inertia 16, mass 4, saved x 16)",
+           "avg_launch_ms": round(ms, 4), "launches_timed": n,
+           "timing": "graph replay of the step's color launches, HIP events around the replay",
+           "yardstick_rate": {"bytes_per_launch": round(ybytes / per_sweep), "gbs": round(ybytes / per_sweep / sec / 1e9, 1),
+                              "note": "fixed yardstick of the reference data model (45 B per vertex, 60 B per "
+                                      "vertex-tet record); not bytes this kernel moves"}}
+    return out
+
+
+def bench_sphere(n: int = 96, frames: int = 30, iterations: int = 30, slot_order: int = 0, plane: float = -1.2,
+                 roofline: bool = True):
+    """Irregular workload as configured: the 96^3 voxel sphere (interior vertices jittered, ids
+    shuffled), mu = 5e4, lambda = 2e5, gravity -z, 30 iterations/step, Chebyshev 0.95, dropped
+    from rest onto the penalty plane z = `plane` (k = 1e4 N/m) for 30 frames.  `value` times the
+    frames in one replay; a second pass times each frame with HIP events (the handle on a torch
+    stream) and splits the frames at the first one that ends with a vertex below the plane.
+    With 30 iterations from x_t (VBD.cpp:72-81) this stiff mesh (k_elastic >> m/h^2 per vertex)
+    sinks at about 1.5 mm per frame instead of falling freely: Gauss-Seidel removes the uniform
+    translation, its lowest-frequency error, slowly.  Its bottom (z = -1) therefore never
+    reaches z = -1.2 within 30 frames; a plane at z = -1.02 is reached after about 12."""
+    import torch
     from vbd_tetmesh_amd import VBDParams, meshgen
     from vbd_tetmesh_amd.solver import upload
     m = meshgen.voxel_sphere(n, seed=0)
     params = VBDParams(dt=H, numIterations=iterations, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95,
-                       mu=m.mu, lambda_=m.lam, contact=(2, -1.2, 1e4))
+                       mu=m.mu, lambda_=m.lam, contact=(2, plane, 1e4))
     d = upload(m, params=params, slot_order=slot_order)
     info = d.info()
-    tune = d.autotune(8)
-    d.set_positions(m.x0)
-    d.step(1)
+    zero = np.zeros_like(m.x0)
+    d.step(1)  # captures the step's graph
     d.synchronize()
     d.set_positions(m.x0)
-    d.set_velocities(np.zeros_like(m.x0))
+    d.set_velocities(zero)
     d.synchronize()
     t0 = time.perf_counter()
     d.step(frames)
     d.synchronize()
     secs = time.perf_counter() - t0
-    cms, cn, bms, bn = d.time_iterations(iterations)
-    cbytes, bbytes = d.algorithmic_bytes()
+    # per-frame pass on a torch stream (torch's default stream is the null stream, which the
+    # handle cannot adopt)
+    d.set_positions(m.x0)
+    d.set_velocities(zero)
+    stream = torch.cuda.Stream()
+    d.set_stream(stream)
+    frame_ms, first_contact, zmin = [], None, []
+    for f in range(frames):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        d.step(1)
+        b.record(stream)
+        b.synchronize()
+        frame_ms.append(a.elapsed_time(b))
+        zmin.append(float(d.positions()[:, 2].min()))
+        if first_contact is None and zmin[-1] < plane:
+            first_contact = f
+    d.set_stream(None)
+    free_ms = frame_ms[:first_contact] if first_contact is not None else frame_ms
+    contact_ms = frame_ms[first_contact:] if first_contact is not None else []
+    roof = launch_roofline(d, iterations, "traffic_color_tiled_sphere96") if roofline else None
     vi = float(info.num_verts) * iterations * frames
+    per_frame_vi = float(info.num_verts) * iterations
     return {"workload": f"sphere {n}^3 voxels, {info.num_verts} verts, {info.num_tets} tets, {info.num_colors} colors, "
-                        f"max valence {info.max_valence}, shuffled ids, contact plane z=-1.2, {iterations} iters/step",
+                        f"max valence {info.max_valence}, shuffled ids, dropped from rest onto the contact plane "
+                        f"z={plane}, {iterations} iters/step, {frames} frames",
             "value": round(vi / secs / 1e6, 2), "unit": "M vertex-iters/s", "ms_per_step": round(secs * 1e3 / frames, 3),
-            "color_kernel_frac": round(cbytes / info.num_colors / (cms / cn / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "step_frac": round(vi / secs * (cbytes + bbytes) / info.num_verts / 1e9 / HBM_PEAK_GBS, 4),
-            "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
+            "frames": frames, "min_z_last_frame": round(zmin[-1], 4),
+            "first_frame_ending_in_contact": None if first_contact is None else first_contact + 1,
+            "frames_without_contact": len(free_ms), "frames_with_contact": len(contact_ms),
+            "value_without_contact": round(per_frame_vi / (np.mean(free_ms) / 1e3) / 1e6, 2) if free_ms else None,
+            "value_with_contact": round(per_frame_vi / (np.mean(contact_ms) / 1e3) / 1e6, 2) if contact_ms else None,
+            "roofline": roof,
+            "launch_config": {"lanes_per_vertex": info.lanes_per_vertex, "pairs_per_lane": info.pairs_per_lane,
+                              "block_size": info.block_size}}
 
 
 def bench_energy(d, reps: int = 20):
@@ -241,9 +340,9 @@ def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
     secs = time.perf_counter() - t0
     info = d.info()
     x = d.positions()
-    cms, cn, _, _ = d.time_iterations(iterations)
-    cbytes, _ = d.algorithmic_bytes()
-    per_launch_s = cms / cn / 1e3
+    roof = launch_roofline(d, iterations)
+    roof["kernel"] = "k_color_tiled (corotated)" if info.lanes_per_vertex == 0 else "k_color_corot"
+    roof["note"] = "VALU-bound (5 polar steps per record); HBM fraction on the 52 B full records"
     d.set_material_model(A.VBD_MATERIAL_STABLE_NEO_HOOKEAN)
     d.set_launch_config(info0.lanes_per_vertex, info0.block_size, info0.pairs_per_lane)
     if not np.all(np.isfinite(x)):
@@ -251,9 +350,7 @@ def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
     vi = float(info.num_verts) * iterations * steps
     return {"workload": "same cube, corotated material of the reference GPU kernel", "value": round(vi / secs / 1e6, 2),
             "unit": "M vertex-iters/s", "ms_per_step": round(secs * 1e3 / steps, 3), "steps": steps,
-            "kernel": "k_color_tiled (corotated)" if info.lanes_per_vertex == 0 else "k_color_corot",
-            "avg_launch_ms": round(per_launch_s * 1e3, 4),
-            "color_kernel_frac": round(cbytes / info.num_colors / per_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+            "roofline": roof, "device_bytes": info.device_bytes, "device_bytes_after": d.info().device_bytes,
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
@@ -356,6 +453,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only the copy probe")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
     ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
+    ap.add_argument("--sphere-frames", type=int, default=30, help="frames of the sphere workload (30 as configured)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-corotated", action="store_true", help="skip the corotated-material run of the cube")
     ap.add_argument("--no-hybrid", action="store_true", help="skip the host-buffer (PCIe-inclusive) frame loop")
@@ -378,7 +476,7 @@ def main():
     from vbd_tetmesh_amd.solver import upload
 
     if args.only:
-        fn = {"batched": bench_batch, "sphere": lambda: bench_sphere(slot_order=args.slot_order),
+        fn = {"batched": bench_batch, "sphere": lambda: bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order),
               "adjoint": bench_adjoint}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
@@ -418,16 +516,13 @@ def main():
     if not np.all(np.isfinite(x)):
         raise SystemExit("non-finite positions after the timed steps")
 
-    # dominant kernel (color solve): live HIP-event timing on the solver's stream
-    cms, cn, bms, bn = d.time_iterations(args.iterations)
-    cbytes, bbytes = d.algorithmic_bytes()
-    per_launch_bytes = cbytes / info.num_colors
-    per_launch_s = cms / cn / 1e3
-    achieved = per_launch_bytes / per_launch_s / 1e9
+    # dominant kernel (color solve): its launches replayed from a graph, HIP events on the
+    # solver's stream
     kernel = "k_color_tiled" if info.lanes_per_vertex == 0 else "k_color_solve"
-    traffic = read_traffic(f"traffic_{kernel[2:]}_cube{args.n}")
-    b_alg_per_vi = (cbytes + bbytes) / info.num_verts
-    overall = vi / secs * b_alg_per_vi / 1e9
+    roof = launch_roofline(d, args.iterations, f"traffic_{kernel[2:]}_cube{args.n}")
+    lbytes, lblend = d.layout_bytes()
+    b_vi = (lbytes + lblend * (args.iterations - 1) / args.iterations) / info.num_verts
+    overall = vi / secs * b_vi / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -445,17 +540,12 @@ def main():
                                f"{args.iterations} iters/step, Chebyshev 0.95, h=1/60, bottom face pinned",
                    "iterations_per_step": args.iterations, "colors": info.num_colors,
                    "lanes_per_vertex": info.lanes_per_vertex, "pairs_per_lane": info.pairs_per_lane,
-                   "block_size": info.block_size},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_gbs": round(traffic / per_launch_s / 1e9, 1) if traffic else None,
-                     "kernel": kernel, "alg_bytes_per_launch": round(per_launch_bytes),
-                     "avg_launch_ms": round(per_launch_s * 1e3, 4), "launches_timed": cn},
-        "roofline_step": {"alg_bytes_per_vertex_iter": round(b_alg_per_vi, 1), "achieved": round(overall, 1),
+                   "block_size": info.block_size, "device_bytes": info.device_bytes},
+        "roofline": roof,
+        "roofline_step": {"alg_bytes_per_vertex_iter": round(b_vi, 1), "achieved": round(overall, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(overall / HBM_PEAK_GBS, 4),
-                          "blend_avg_launch_ms": round(bms / max(bn, 1), 4),
-                          "note": "bytes of the reference data model (60 B per vertex-tet record); the color kernel "
-                                  "reads 32 B per record, so this rate can exceed the HBM peak"},
+                          "note": "whole timed step: layout bytes of the color launches and blends per "
+                                  "vertex-iteration x the measured rate"},
     }
     out["energy"] = bench_energy(d)
     log(f"[bench] energies: " + ", ".join(f"{k} {v['ms_per_eval']:.3f} ms ({v['frac']})" for k, v in out["energy"].items()))
@@ -463,7 +553,7 @@ def main():
         cr = bench_corotated(d, m, args.iterations,
                              cfg=None if args.lanes is None else (args.lanes, args.block, args.unroll))
         out["corotated"] = cr
-        log(f"[bench] corotated: {cr['value']:.1f} M vertex-iters/s, color kernel frac {cr['color_kernel_frac']}")
+        log(f"[bench] corotated: {cr['value']:.1f} M vertex-iters/s, color kernel frac {cr['roofline']['frac']}")
     if not args.no_hybrid:
         del d
         hy = bench_hybrid(m, args.iterations)
@@ -479,16 +569,24 @@ def main():
             f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
             f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
     if not args.no_sphere:
-        sp = bench_sphere(slot_order=args.slot_order)
+        sp = bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order)
+        # the same drop onto a plane the sinking sphere reaches within the frames
+        sc = bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order, plane=-1.02, roofline=False)
+        sp["contact_run"] = {k: sc[k] for k in ("workload", "value", "ms_per_step", "first_frame_ending_in_contact",
+                                                 "frames_without_contact", "frames_with_contact",
+                                                 "value_without_contact", "value_with_contact", "min_z_last_frame")}
         out["sphere"] = sp
-        log(f"[bench] sphere: {sp['value']:.1f} M vertex-iters/s, color kernel frac {sp['color_kernel_frac']}")
+        cr_ = sp["contact_run"]
+        log(f"[bench] sphere: {sp['value']:.1f} M vertex-iters/s, color kernel frac {sp['roofline']['frac']}; plane "
+            f"z=-1.02: {cr_['frames_without_contact']} frames without contact {cr_['value_without_contact']}, "
+            f"{cr_['frames_with_contact']} with {cr_['value_with_contact']}")
     if not args.no_adjoint:
         ad = bench_adjoint()
         out["adjoint"] = ad
         log(f"[bench] adjoint: forward {ad['forward_s']:.3f}s backward {ad['backward_s']:.3f}s "
             f"(x{ad['adjoint_over_forward']})")
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(m)
+        cb = cpu_baseline(m, args.iterations)
         out["cpu_baseline"] = cb
         log(f"[bench] cpu baseline {cb['value'] * 1e6:.0f} vertex-iters/s on {cb['cores']} threads")
     print(json.dumps(out), flush=True)

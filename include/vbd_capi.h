@@ -217,6 +217,21 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
  * number of color launches, total blend milliseconds, number of blend launches. */
 int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int32_t* color_launches,
                         double* blend_ms, int32_t* blend_launches);
+/* The color launches of `sweeps` sweeps (the step's own kernels: with Chebyshev in the
+ * parameters each launch also saves the start-of-sweep positions, as in vbd_step), captured in
+ * one hipGraph and replayed once untimed and once between two HIP events on the handle's stream
+ * (no reference counterpart).  Outputs: milliseconds per color launch (the replay's time over
+ * its launches, the gaps between graph nodes included) and the number of launches timed.  Moves
+ * the positions. */
+int vbd_time_color_launches(vbd_solver* h, int32_t sweeps, double* ms_per_launch, int32_t* launches);
+/* Bytes one sweep's color launches must move in the current configuration's layout, at least
+ * (no reference counterpart): per (free vertex, incident tet) record the record as stored (32 B
+ * compact for the tiled stable Neo-Hookean kernel, 52 B full for the per-vertex kernels and the
+ * corotated material; +8 B with per-tet mu, lambda), per free vertex 56 B (row offset 4, x read
+ * 16, inertia 16, mass 4, x write 16) plus 16 B of saved position when the parameters enable
+ * Chebyshev; the blend launches 48 B per free vertex (x and x_pp read, x written).  Tile padding
+ * and neighbour-position gathers (cache-resident) are not counted. */
+int vbd_layout_bytes(const vbd_solver* h, double* color_bytes_per_sweep, double* blend_bytes_per_sweep);
 /* Algorithmic bytes of one sweep, a fixed yardstick independent of the device layout:
  * color launches 45 B per free vertex (x read 12, x write 12, inertia 12, mass 4, CSR row 4,
  * fixed flag 1) + 60 B per (vertex, incident tet) (CSR entry 4, tet ids 16, DmInv 36, V0 4;

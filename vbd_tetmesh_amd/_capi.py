@@ -121,6 +121,8 @@ SIGNATURES = [
     ("vbd_autotune", C.c_int, [H, C.c_int32, i32p, i32p, i32p, f32p]),
     ("vbd_time_iterations", C.c_int, [H, C.c_int32, f64p, i32p, f64p, i32p]),
     ("vbd_algorithmic_bytes", C.c_int, [H, f64p, f64p]),
+    ("vbd_time_color_launches", C.c_int, [H, C.c_int32, f64p, i32p]),
+    ("vbd_layout_bytes", C.c_int, [H, f64p, f64p]),
     ("vbd_mesh_initialize", C.c_int, [C.c_int32, f32p, C.c_int32, i32p, C.c_float, f32p, f32p, i32p, i32p]),
     ("vbd_color_vertices", C.c_int, [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p, i32p]),
     ("vbd_precompute_tet_data", C.c_int, [f32p, C.c_int32, i32p, f32p, f32p]),

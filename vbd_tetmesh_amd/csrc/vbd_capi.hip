@@ -1382,6 +1382,64 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     return VBD_OK;
 }
 
+int vbd_time_color_launches(vbd_solver* h, int32_t sweeps, double* ms_per_launch, int32_t* launches) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (sweeps <= 0) return fail(h, VBD_ERR_ARG, "sweeps must be positive");
+    DeviceGuard g(h->device);
+    const vbd_step_params p = h->params;
+    const bool accel = h->paramsSet && p.use_acceleration;
+    const float hstep = h->paramsSet ? p.dt / (float)p.substeps : 1.0f / 60.0f;
+    const float sdt = inv_h2_ref(hstep);
+    const float mu = h->paramsSet ? p.mu : 1e5f, lam = h->paramsSet ? p.lambda : 4e5f;
+    const ContactArgs ct = h->paramsSet ? contact_of(p) : ContactArgs{};
+    int per_sweep = 0;
+    for (int c = 0; c < h->m.nColors; ++c)
+        if (h->m.colorSlotOff[c + 1] > h->m.colorSlotOff[c]) ++per_sweep;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t ee = hipSuccess;
+        for (int it = 0; it < sweeps && ee == hipSuccess; ++it)
+            ee = enqueue_sweep(h, sdt, mu, lam, ct, accel ? h->m.save[it & 1] : nullptr);
+        e = hipStreamEndCapture(h->stream, &graph);
+        if (ee != hipSuccess) e = ee;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, h->stream);  // untimed: warm caches and clocks
+    if (e == hipSuccess) e = hipEventRecord(a, h->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(b, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(h, "vbd_time_color_launches", e);
+    const int n = sweeps * per_sweep;
+    if (ms_per_launch) *ms_per_launch = n > 0 ? (double)ms / n : 0.0;
+    if (launches) *launches = n;
+    return VBD_OK;
+}
+
+int vbd_layout_bytes(const vbd_solver* h, double* color_bytes, double* blend_bytes) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (!h->inited) return fail(const_cast<vbd_solver*>(h), VBD_ERR_STATE, "not initialised");
+    const bool compact = h->lpv == 0 && h->materialModel == VBD_MATERIAL_STABLE_NEO_HOOKEAN;
+    const double per_rec = (compact ? 32.0 : 52.0) + (h->m.perTetMat ? 8.0 : 0.0);
+    const bool accel = h->paramsSet && h->params.use_acceleration;
+    const double per_vert = 56.0 + (accel ? 16.0 : 0.0);
+    if (color_bytes) *color_bytes = per_vert * h->m.nFree + per_rec * (double)h->m.nPairs;
+    if (blend_bytes) *blend_bytes = 48.0 * h->m.nFree;
+    return VBD_OK;
+}
+
 // Algorithmic bytes (fixed yardstick; neighbour-position gathers excluded):
 //   color launches, per solved vertex: x read 12 + x write 12 + y 12 + mass 4 + CSR row 4 +
 //   fixed flag 1 = 45; per (vertex, incident tet): CSR entry 4 + tet 16 + DmInv 36 + V0 4 = 60,

@@ -390,6 +390,19 @@ class DeviceMesh(_Handle):
         check(lib.vbd_time_iterations(self.h, iterations, C.byref(cms), C.byref(cn), C.byref(bms), C.byref(bn)), self.h)
         return cms.value, cn.value, bms.value, bn.value
 
+    def time_color_launches(self, sweeps: int):
+        """(ms per color launch, launches) of `sweeps` sweeps of the step's color kernels replayed
+        from one hipGraph between two HIP events on the handle's stream.  Moves the positions."""
+        ms, n = C.c_double(), C.c_int32()
+        check(lib.vbd_time_color_launches(self.h, int(sweeps), C.byref(ms), C.byref(n)), self.h)
+        return ms.value, n.value
+
+    def layout_bytes(self):
+        """(color-launch bytes per sweep, blend bytes per sweep) the current layout must move."""
+        c, b = C.c_double(), C.c_double()
+        check(lib.vbd_layout_bytes(self.h, C.byref(c), C.byref(b)), self.h)
+        return c.value, b.value
+
     def algorithmic_bytes(self):
         cb, bb = C.c_double(), C.c_double()
         check(lib.vbd_algorithmic_bytes(self.h, C.byref(cb), C.byref(bb)), self.h)
