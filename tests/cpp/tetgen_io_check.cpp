@@ -1,8 +1,9 @@
 // Loads a TetGen model with include/vbd/TetGenIO.h (the loader of reference main.cpp:14-168)
 // and writes what the loader produced to a binary file for tests/test_dropin.py:
 //   int32 nV, int32 nT, float rest[3 nV], float x[3 nV], int32 tets[4 nT], uint8 fixed[nV],
-//   float mass[nV], then the surface triangles: int32 nF, int32 faces[3 nF].
-// usage: tetgen_io_check <prefix> <current-node file or "-"> <pin axis> <pin fraction> <out>
+//   float mass[nV], then the surface triangles: int32 nF, int32 faces[3 nF]; with a sixth
+//   argument, also TetMesh::exportToObj to that path.
+// usage: tetgen_io_check <prefix> <current-node file or "-"> <pin axis> <pin fraction> <out> [obj]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -11,7 +12,7 @@
 #include "vbd/TetGenIO.h"
 
 int main(int argc, char** argv) {
-    if (argc != 6) {
+    if (argc != 6 && argc != 7) {
         std::fprintf(stderr, "usage: tetgen_io_check <prefix> <current|-> <pin axis> <pin fraction> <out>\n");
         return 2;
     }
@@ -43,5 +44,6 @@ int main(int argc, char** argv) {
         std::fwrite(v, sizeof(int32_t), 3, f);
     }
     std::fclose(f);
+    if (argc == 7 && !mesh.exportToObj(argv[6])) return 4;
     return 0;
 }

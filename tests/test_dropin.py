@@ -93,6 +93,76 @@ def test_cpp_tetgen_loader_parses_the_format_and_pins_like_main_cpp(tmp_path):
         assert np.dot(np.cross(b - a, d - a), (a + b + d) / 3 - c) > 0
 
 
+def reference_surface_order(tets):
+    """TetMesh::extractSurfaceFaces as the reference states it (TetMesh.h:80-141), brute force:
+    every tet's four faces in the order (0,2,1), (0,1,3), (0,3,2), (1,2,3), keyed by the sorted
+    vertex triple in a dict (the std::map), the first occurrence's winding kept; the faces that
+    occur exactly once, in ascending key order (the map's iteration order)."""
+    count, orient = {}, {}
+    for t in np.asarray(tets).tolist():
+        v0, v1, v2, v3 = t
+        for tri in ((v0, v2, v1), (v0, v1, v3), (v0, v3, v2), (v1, v2, v3)):
+            key = tuple(sorted(tri))
+            if key not in count:
+                count[key] = 1
+                orient[key] = tri
+            else:
+                count[key] += 1
+    return np.array([orient[k] for k in sorted(count) if count[k] == 1], np.int32).reshape(-1, 3)
+
+
+def reference_obj_text(x, tets, faces):
+    """The bytes TetMesh::exportToObj writes (TetMesh.h:147-181): float coordinates through an
+    ostream's default (printf %g) formatting, 1-based face ids."""
+    out = [f"# TetMesh OBJ Export\n# Vertices: {len(x)}\n# Tetrahedra: {len(tets)}\n\n"]
+    out += ["v %g %g %g\n" % (float(p[0]), float(p[1]), float(p[2])) for p in x]
+    out.append(f"\n# Surface Triangles: {len(faces)}\n")
+    out += [f"f {a + 1} {b + 1} {c + 1}\n" for a, b, c in faces.tolist()]
+    return "".join(out)
+
+
+def surface_cases():
+    rng = np.random.default_rng(7)
+    beam = meshgen.beam(4, 2, 2)
+    sph = meshgen.voxel_sphere(6, seed=1)  # shuffled vertex ids
+    # tets in a shuffled order with their corners rotated by an even permutation (same
+    # orientation, another first corner): another first-occurrence winding for interior faces
+    t = sph.tets[rng.permutation(sph.num_tets)][:, [1, 2, 0, 3]]
+    # a face shared by three tets and an isolated tet: counts 3 and 1
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -1], [0.3, 0.3, 0.8], [5, 5, 5], [6, 5, 5],
+                  [5, 6, 5], [5, 5, 6]], np.float32)
+    odd = np.array([[0, 1, 2, 3], [0, 2, 1, 4], [0, 1, 2, 5], [6, 7, 8, 9]], np.int32)
+    return [("beam", beam.rest, beam.tets), ("sphere", sph.rest, sph.tets), ("sphere_permuted", sph.rest, t),
+            ("three_tets_on_a_face", x, odd)]
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_surface_faces_and_obj_bytes_follow_the_reference_map_order(tmp_path, case):
+    """Face list (ids, winding and order) of the C++ drop-in and the Python mirror against the
+    brute-force restatement of the reference's std::map; the C++ exportToObj bytes against the
+    reference format; the Python exportToObj bytes equal to the C++ ones."""
+    from vbd_tetmesh_amd.solver import TetMesh
+    name, x, tets = surface_cases()[case]
+    prefix = str(tmp_path / name)
+    meshgen.write_tetgen(prefix, x, tets)
+    exe = _build_tetgen_check(tmp_path)
+    obj = str(tmp_path / "cpp.obj")
+    r = subprocess.run([exe, prefix, "-", "-1", "0.05", str(tmp_path / "o.bin"), obj], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, r_x, r_tets, _, _, faces = _read_check(str(tmp_path / "o.bin"))
+    ref = reference_surface_order(tets)
+    assert np.array_equal(faces, ref), name
+    t = TetMesh().initialize(x, x, 1000.0, tets, 1e5, 4e5)
+    assert np.array_equal(t.extractSurfaceFaces(), ref), name
+    expect = reference_obj_text(r_x, tets, ref)
+    assert open(obj).read() == expect
+    py = str(tmp_path / "py.obj")
+    assert t.exportToObj(py)
+    assert open(py, "rb").read() == open(obj, "rb").read()
+    print(f"[surface] {name}: {len(ref)} boundary faces in the reference order", flush=True)
+
+
 def test_cpp_tetgen_loader_rejects_bad_files(tmp_path):
     exe = _build_tetgen_check(tmp_path)
     (tmp_path / "bad.node").write_text("4 3 0 0\n1 0 0 0\n2 1 0 0\n3 0 1 0\n")  # one point short
@@ -196,3 +266,59 @@ def test_cpp_dropin_matches_oracle(tmp_path, mode):
     assert obj.count("\nv ") == m.num_verts
     # boundary of a 20x4x4 box: 2*(20*4 + 20*4 + 4*4) quads = 352 quads = 704 triangles
     assert obj.count("\nf ") == 704
+
+
+def steel_block_mm():
+    """A block in mm (8 x 10 x 8 one-mm cells, 891 verts, 3840 tets) and a warm-start current
+    shape: compressed 8% along y and sheared 2% in x over y, like the compressed current-shape
+    node file reference main.cpp:183 loads."""
+    rest = meshgen.grid_positions(8, 10, 8, 1.0)
+    tets = meshgen.grid_tets(8, 10, 8)
+    cur = rest.copy()
+    cur[:, 1] *= np.float32(0.92)
+    cur[:, 0] += np.float32(0.02) * rest[:, 1]
+    return rest, cur.astype(np.float32), tets
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("material", [0, 1])
+def test_cpp_reference_main_configuration_matches_oracle(tmp_path, material):
+    """Reference main.cpp:171-232 as written, through the C++ drop-in: steel in mm (E = 200e3,
+    nu = 0.3, density 7800e-12), warm start from the current-shape node file, the y <= min + 5%
+    pin on the current positions (main.cpp:146-164), dt = 1e-3, 2 substeps, 50 iterations,
+    Chebyshev rho 0.75, no gravity, host forwardStep / updateInertiaFromCPU / solveVBD (x <- y) /
+    copyPositionsToCPU / updateVelocity, 20 frames; material 1 is the reference GPU kernel's
+    corotated model, 0 the CPU solver's stable Neo-Hookean.  Here m/h^2 is about 1e-7 of the
+    elastic diagonal.  Against the FP32 and FP64 oracles: per frame max|x_gpu - x_oracle| /
+    max|x_oracle| <= 1e-5; the FP32 oracle's distance from FP64 is printed beside it."""
+    rest, cur, tets = steel_block_mm()
+    meshgen.write_tetgen(str(tmp_path / "block"), rest, tets)
+    meshgen.write_tetgen(str(tmp_path / "block_compressed"), cur, tets)
+    out = str(tmp_path / "frames.bin")
+    frames = 20
+    print(f"[reference main configuration] material {material}", flush=True)
+    r = subprocess.run([DEMO, "main", str(tmp_path / "block.node"), str(tmp_path / "block_compressed.node"),
+                        str(tmp_path / "block.ele"), str(frames), str(material), out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out, dtype=np.float32).reshape(frames, -1, 3)
+    E, nu = np.float32(200e3), np.float32(0.3)
+    mu = float(np.float32(E / (np.float32(2) * (np.float32(1) + nu))))
+    lam = float(np.float32(E * nu / ((np.float32(1) + nu) * (np.float32(1) - np.float32(2) * nu))))
+    lo, hi = cur[:, 1].min(), cur[:, 1].max()
+    fixed = (cur[:, 1] <= lo + (hi - lo) * np.float32(0.05)).astype(np.uint8)
+    m = meshgen.Mesh(name="steel_block_mm", rest=rest, x0=cur, tets=tets, fixed=fixed, mu=mu, lam=lam,
+                     density=float(np.float32(7800e-12)), gravity=(0.0, 0.0, 0.0))
+    prep = O.prepare(m)
+    p = O.make_params(dt=1e-3, iterations=50, substeps=2, gravity=(0.0, 0.0, 0.0), rho=0.75, mu=mu, lam=lam,
+                      init_from_inertia=True, material=material)
+    _, _, of32, _ = O.simulate(prep, p, m.x0, None, frames)
+    _, _, of64, _ = O.simulate(prep, p, m.x0, None, frames, f64=True, rest=m.rest, density=m.density)
+    r32 = max(float(np.abs(got[f].astype(np.float64) - of32[f]).max() / np.abs(of32[f]).max()) for f in range(frames))
+    r64 = max(float(np.abs(got[f].astype(np.float64) - of64[f]).max() / np.abs(of64[f]).max()) for f in range(frames))
+    o = max(float(np.abs(of32[f] - of64[f]).max() / np.abs(of64[f]).max()) for f in range(frames))
+    disp = float(np.abs(of64[-1] - cur).max())
+    print(f"reference main, material {material}: gpu vs fp32 oracle {r32:.2e}, vs fp64 {r64:.2e}, fp32 oracle vs "
+          f"fp64 {o:.2e}; {int(fixed.sum())} pinned, displacement after {frames} frames {disp:.3f} mm", flush=True)
+    assert disp > 0.1  # the warm start relaxes
+    assert r32 <= 1e-5 and r64 <= 1e-5

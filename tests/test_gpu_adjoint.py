@@ -212,6 +212,13 @@ def test_diffcube16_gradients_match_fp64_and_finite_differences():
         scale = np.abs(ref[kind]).max()
         worst_fd = max(worst_fd, abs(g - f) / scale)
         worst_adj = max(worst_adj, abs(g - r) / scale)
+        print(f"  {kind} {i}: gpu {g:+.6e} fd {f:+.6e} fp64 adjoint {r:+.6e}; |gpu - fd| / |fd| {abs(g - f) / abs(f):.2e}, "
+              f"fixture |fd - adjoint| / |fd| {abs(f - r) / abs(f):.2e}, |fd| / family max {abs(f) / scale:.2e}",
+              flush=True)
+    res = np.asarray(sim.stats.forward_residuals)
+    _, load = sim.dev.residual_norm()
+    print(f"  converged steps: |grad G| min {res.min():.3e} median {np.median(res):.3e} max {res.max():.3e} "
+          f"(|M g| = {load:.3e}; {sim.stats.forward_iterations[:3]}... iterations)", flush=True)
     print(f"diffcube16: loss {L:.6e} vs {float(z['loss']):.6e}; vector rel errors {vec}; "
           f"32 params: max |gpu - fd| {worst_fd:.2e}, |gpu - fp64 adjoint| {worst_adj:.2e} (of family max); "
           f"forward {fwd:.2f}s backward {bwd:.2f}s ratio {bwd / fwd:.2f}", flush=True)

@@ -8,6 +8,12 @@
 //
 // usage: vbd_dropin_demo <model prefix> <hybrid|step> <frames> <iterations> <rho>
 //                        <pin axis> <pin fraction> <mu> <lambda> <out.bin> [obj]
+//        vbd_dropin_demo main <rest.node> <current.node> <model.ele> <frames> <material 0|1> <out.bin>
+// The second form is reference main.cpp:171-232 as written: steel in mm (E = 200e3, nu = 0.3,
+// density 7800e-12), a warm start from the current-shape node file, the y <= min + 5% pin on the
+// current positions, dt = 1e-3 with 2 substeps, 50 iterations, Chebyshev rho 0.75, no gravity,
+// and the hybrid frame loop around solveVBD; material 1 is the reference GPU kernel's corotated
+// model, 0 the CPU solver's stable Neo-Hookean.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -20,7 +26,67 @@
 
 using namespace VBD;
 
+// main.cpp:171-232: returns the process status
+static int reference_main(int argc, char** argv) {
+    if (argc < 8) {
+        std::cerr << "usage: vbd_dropin_demo main <rest.node> <current.node> <model.ele> <frames> <material> <out.bin>\n";
+        return 2;
+    }
+    const int frames = std::atoi(argv[5]), material = std::atoi(argv[6]);
+    try {
+        Real youngs_modulus = 200.0e3f;
+        Real poisson_ratio = 0.3f;
+        Real mu = youngs_modulus / (2.0f * (1.0f + poisson_ratio));
+        Real lambda = youngs_modulus * poisson_ratio / ((1.0f + poisson_ratio) * (1.0f - 2.0f * poisson_ratio));
+        Real density = 7800.0e-12f;
+        TetMesh mesh;
+        if (!loadTetgenModel(argv[2], argv[3], argv[4], mesh, density, mu, lambda, 1, 0.05f)) {
+            std::cerr << "failed to load " << argv[2] << "\n";
+            return 1;
+        }
+        VBDParams params;
+        params.dt = 0.001f;
+        params.numIterations = 50;
+        params.substeps = 2;
+        params.gravity = Vec3(0.0f, 0.0f, 0.0f);
+        params.useAcceleration = true;
+        params.accelerationRho = 0.75f;
+        params.numThreads = 4;
+        params.mu = mu;
+        params.lambda = lambda;
+        VBDsolver cpuSolver;
+        cpuSolver.setParams(params);
+        cpuSolver.initializeColoring(mesh);
+        cpuSolver.precomputeTetData(mesh);
+        VBD_GPU_Solver gpuSolver;
+        gpuSolver.setMaterialModel(material);
+        gpuSolver.initFromCPU(mesh, cpuSolver.getVertexColors(), cpuSolver.getTetDmInv(), cpuSolver.getTetRestVol());
+        Real subDt = params.dt / params.substeps;
+        Real dtSqrReciprocal = 1.0f / (subDt * subDt);
+        std::cout << "mesh " << mesh.numVerts << " verts, " << mesh.numTets << " tets, "
+                  << cpuSolver.getVertexColors().size() << " colors, mu " << mu << " lambda " << lambda << std::endl;
+        std::ofstream f(argv[7], std::ios::binary);
+        for (int frame = 0; frame < frames; ++frame) {
+            for (int step = 0; step < params.substeps; ++step) {
+                cpuSolver.forwardStep(mesh);
+                gpuSolver.updateInertiaFromCPU(mesh);
+                gpuSolver.solveVBD(params.numIterations, dtSqrReciprocal, params.mu, params.lambda,
+                                   params.useAcceleration, params.accelerationRho);
+                gpuSolver.copyPositionsToCPU(mesh);
+                mesh.updateVelocity(subDt);
+            }
+            f.write((const char*)mesh.mVertPos.data(), sizeof(float) * 3 * mesh.numVerts);
+        }
+        std::cout << "done " << frames << " frames" << std::endl;
+    } catch (const std::exception& e) {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "main") return reference_main(argc, argv);
     if (argc < 11) {
         std::cerr << "usage: vbd_dropin_demo <prefix> <hybrid|step> <frames> <iterations> <rho> <pin axis> "
                      "<pin fraction> <mu> <lambda> <out.bin> [obj]\n";
