@@ -401,24 +401,31 @@ def bench_hybrid(m, iterations: int, frames: int = 5):
 def bench_adjoint(n: int = 16, steps: int = 30):
     """Differentiable workload: 16^3 cube, per-tet mu field, lambda = 4 mu, 30 converged steps;
     forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation +
-    colored block-GS adjoint + parameter gradients) wall time on the device."""
+    colored block-GS adjoint + parameter gradients) wall time on the device.  Precision 64 (the
+    default: every step converged to |grad G| < 1e-6) and precision 32 (FP32 state, steps end on
+    the stagnation test near 5e-3 |M g|)."""
     from vbd_tetmesh_amd import meshgen
     from vbd_tetmesh_amd.adjoint import DiffSim
     m, mu_t, lam_t, v0 = meshgen.diff_workload(n, seed=0)
-    sim = DiffSim(m, steps=steps)
     x_t = m.rest.copy()
-    sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)  # warm-up
-    L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
-    st = sim.stats
-    vi = float(m.num_verts) * sum(st.forward_iterations)
-    return {"workload": f"cube {n}^3 ({m.num_verts} verts, {m.num_tets} tets), per-tet mu/lambda, {steps} converged "
-                        f"steps, checkpoint every 5, gradients w.r.t. mu, lambda, v0",
+    out = {}
+    for prec in (64, 32):
+        sim = DiffSim(m, steps=steps, precision=prec)
+        sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)  # warm-up
+        sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+        st = sim.stats
+        vi = float(m.num_verts) * sum(st.forward_iterations)
+        out[f"fp{prec}"] = {
             "forward_s": round(st.forward_seconds, 4), "backward_s": round(st.backward_seconds, 4),
             "adjoint_over_forward": round(st.backward_seconds / st.forward_seconds, 2),
             "forward_iterations_mean": round(float(np.mean(st.forward_iterations)), 1),
+            "grad_G_max": float(max(st.forward_residuals)),
             "adjoint_sweeps_mean": round(float(np.mean(st.adjoint_sweeps)), 1),
-            "adjoint_chebyshev_rho": sim.adj_rho, "adjoint_fallbacks": st.adjoint_fallbacks,
+            "adjoint_rel_tol": sim.adj_rel_tol, "adjoint_chebyshev_rho": sim.adj_rho,
+            "adjoint_fallbacks": st.adjoint_fallbacks,
             "forward_value": round(vi / st.forward_seconds / 1e6, 2), "unit": "M vertex-iters/s"}
+    return {"workload": f"cube {n}^3 ({m.num_verts} verts, {m.num_tets} tets), per-tet mu/lambda, {steps} converged "
+                        f"steps, checkpoint every 5, gradients w.r.t. mu, lambda, v0", **out}
 
 
 def copy_probe(mb: int):
@@ -583,8 +590,9 @@ def main():
     if not args.no_adjoint:
         ad = bench_adjoint()
         out["adjoint"] = ad
-        log(f"[bench] adjoint: forward {ad['forward_s']:.3f}s backward {ad['backward_s']:.3f}s "
-            f"(x{ad['adjoint_over_forward']})")
+        log("[bench] adjoint: " + "; ".join(f"{k}: forward {v['forward_s']:.3f}s backward {v['backward_s']:.3f}s "
+                                             f"(x{v['adjoint_over_forward']}), |grad G| <= {v['grad_G_max']:.1e}"
+                                             for k, v in ad.items() if k.startswith("fp")))
     if not args.no_cpu_baseline:
         cb = cpu_baseline(m, args.iterations)
         out["cpu_baseline"] = cb

@@ -313,6 +313,29 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
 /* Per-tet -(d^2 E / dx dmu)^T w and -(d^2 E / dx dlambda)^T w at the current positions:
  * dmu_t = -V (F - C) : dF(w), dlambda_t = -V (J - 1) C : dF(w), in the caller's tet order. */
 int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dlambda);
+/* Precision of the differentiation entry points above (no reference counterpart).  32, the
+ * default: FP32 kernels on the FP32 state; a converged step's |grad G| stalls near H ulp(x) per
+ * vertex in absolute coordinates (about 5e-3 |M g| on a 16^3 unit cube).  64: the handle keeps an
+ * FP64 copy of the state (positions, velocities, inertia target, step-start positions), and
+ * vbd_step_converged, vbd_residual_norm, vbd_hessian_vector, vbd_adjoint_solve and
+ * vbd_parameter_gradients run FP64 per-vertex kernels on it (the pair records' FP32 DmInv rows
+ * promoted exactly), so a converged step reaches |grad G| below 1e-6; after each converged step
+ * the FP32 arrays hold its rounding.  Any entry point that writes the FP32 state (uploads, the
+ * FP32 steps and solves) makes the next FP64 call start from the FP32 state. */
+int vbd_set_differentiation_precision(vbd_solver* h, int32_t bits);
+/* The FP64 state's positions and velocities, 3 doubles per vertex in the caller's numbering
+ * (precision 64 only; for checkpoints that must not round). */
+int vbd_download_state_f64(vbd_solver* h, double* positions, double* velocities);
+/* Sets the FP64 state's positions and/or velocities (either may be NULL: kept) and their FP32
+ * rounding (precision 64 only). */
+int vbd_upload_state_f64(vbd_solver* h, const double* positions, const double* velocities);
+/* vbd_hessian_vector, vbd_adjoint_solve and vbd_parameter_gradients with double vectors and
+ * outputs (precision 64 only): a rollout's backward recursion (a <- M/h^2 w - (c/h) b) cancels
+ * most of its terms, so FP32 vectors there would cost the gradients about 1e-4 of their size. */
+int vbd_hessian_vector_f64(vbd_solver* h, const double* w, double* out);
+int vbd_adjoint_solve_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations, int32_t check_every,
+                          float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual);
+int vbd_parameter_gradients_f64(vbd_solver* h, const double* w, double* dmu, double* dlambda);
 
 #ifdef __cplusplus
 }

@@ -147,21 +147,27 @@ def test_checkpoint_recomputation_is_bitwise(small):
     assert np.array_equal(xs[-1], x)
 
 
-def test_small_rollout_gradients_match_fp64_adjoint(small):
-    """4^3 cube, 10 converged steps: loss and all gradients vs the FP64 adjoint (1e-3 rel)."""
+@pytest.mark.parametrize("precision", [32, 64])
+def test_small_rollout_gradients_match_fp64_adjoint(small, precision):
+    """4^3 cube, 10 converged steps: loss and all gradients vs the FP64 adjoint (relative 1e-3
+    with the FP32 state, 1e-5 with the FP64 state)."""
     m, mu_t, lam_t, v0, P, _, _ = small
     x_t = D.forward(P, m.rest.astype(np.float64), v0.astype(np.float64), mu_t.astype(np.float64),
                     lam_t.astype(np.float64), 10)[0][-1]
-    sim = DiffSim(m, steps=10)
+    sim = DiffSim(m, steps=10, precision=precision)
     L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
     L64, gmu64, glam64, gv064, _ = D.loss_and_grads(P, m.rest.astype(np.float64), v0.astype(np.float64),
                                                     m.tet_mu.astype(np.float64), m.tet_lambda.astype(np.float64), 10,
                                                     x_t)
     errs = {k: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for k, a, b in
             (("mu", gmu, gmu64), ("lambda", glam, glam64), ("v0", gv0, gv064))}
-    print(f"small rollout: loss {L:.6e} vs {L64:.6e}; gradient rel errors {errs}", flush=True)
-    assert abs(L - L64) <= 1e-3 * L64
-    assert all(e <= 1e-3 for e in errs.values())
+    tol = 1e-3 if precision == 32 else 1e-5
+    print(f"small rollout, precision {precision}: loss {L:.6e} vs {L64:.6e}; gradient rel errors {errs}; "
+          f"|grad G| per step max {max(sim.stats.forward_residuals):.2e}", flush=True)
+    assert abs(L - L64) <= tol * L64
+    assert all(e <= tol for e in errs.values())
+    if precision == 64:
+        assert max(sim.stats.forward_residuals) < 1e-6
 
 
 def test_torch_autograd_function(small):
@@ -186,16 +192,22 @@ def test_torch_autograd_function(small):
 
 
 @pytest.mark.skipif(not os.path.exists(GOLDEN), reason="FP64 golden fixture not generated")
-def test_diffcube16_gradients_match_fp64_and_finite_differences():
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("precision", [64, 32])
+def test_diffcube16_gradients_match_fp64_and_finite_differences(precision):
     """Differentiable workload: 16^3 cube (4913 verts, 24576 tets), per-tet mu field, lambda = 4 mu,
     30 converged steps, loss |x_30 - x_target|^2.  GPU adjoint vs the committed FP64 CPU adjoint
     and its central finite differences (step 1e-3 of the parameter, of max |v0| for v0) on 32 seeded
-    parameters.
-    Tolerance: per parameter |g_gpu - g_ref| <= 1e-3 * max|g_ref| of that parameter family, and
-    relative error of each whole gradient vector <= 1e-2."""
+    parameters.  The fixture's own FD-vs-adjoint spread, the FD noise floor, is at most 1.4e-6
+    relative on 31 of the 32 and 3.9e-5 on the one whose gradient is 4e-5 of its family's largest.
+    Tolerance, FP64 state: every step converged to |grad G| < 1e-6, and per parameter
+    |g_gpu - g_fd| / |g_fd| <= 1e-3 and |g_gpu - g_adjoint| / |g_adjoint| <= 1e-5.
+    FP32 state (steps end on the stagnation test near 5e-3 |M g|): per parameter
+    |g_gpu - g_ref| <= 1e-3 * max|g_ref| of that parameter family.
+    Both: loss within 1e-3 and each whole gradient vector within 1e-2, relative."""
     z = np.load(GOLDEN)
     m, mu_t, lam_t, v0 = meshgen.diff_workload(int(z["n"]), seed=0)
-    sim = DiffSim(m, steps=int(z["steps"]))
+    sim = DiffSim(m, steps=int(z["steps"]), precision=precision)
     L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, z["x_target"])
     fwd, bwd = sim.stats.forward_seconds, sim.stats.backward_seconds
     ref = {"mu": z["grad_mu"].astype(np.float64), "lambda": z["grad_lambda"].astype(np.float64),
@@ -205,13 +217,15 @@ def test_diffcube16_gradients_match_fp64_and_finite_differences():
     fd = z["fd"]
     picks = [("mu", int(i)) for i in z["idx_mu"]] + [("lambda", int(i)) for i in z["idx_lambda"]] + \
         [("v0", (int(i), int(c))) for i, c in z["idx_v0"]]
-    worst_fd, worst_adj = 0.0, 0.0
+    worst_fd, worst_adj, worst_rel_fd, worst_rel_adj = 0.0, 0.0, 0.0, 0.0
     for (kind, i), f in zip(picks, fd):
         g = got[kind][i]
         r = ref[kind][i]
         scale = np.abs(ref[kind]).max()
         worst_fd = max(worst_fd, abs(g - f) / scale)
         worst_adj = max(worst_adj, abs(g - r) / scale)
+        worst_rel_fd = max(worst_rel_fd, abs(g - f) / abs(f))
+        worst_rel_adj = max(worst_rel_adj, abs(g - r) / abs(r))
         print(f"  {kind} {i}: gpu {g:+.6e} fd {f:+.6e} fp64 adjoint {r:+.6e}; |gpu - fd| / |fd| {abs(g - f) / abs(f):.2e}, "
               f"fixture |fd - adjoint| / |fd| {abs(f - r) / abs(f):.2e}, |fd| / family max {abs(f) / scale:.2e}",
               flush=True)
@@ -219,9 +233,84 @@ def test_diffcube16_gradients_match_fp64_and_finite_differences():
     _, load = sim.dev.residual_norm()
     print(f"  converged steps: |grad G| min {res.min():.3e} median {np.median(res):.3e} max {res.max():.3e} "
           f"(|M g| = {load:.3e}; {sim.stats.forward_iterations[:3]}... iterations)", flush=True)
-    print(f"diffcube16: loss {L:.6e} vs {float(z['loss']):.6e}; vector rel errors {vec}; "
+    ar = np.asarray(sim.stats.adjoint_residuals)
+    print(f"  adjoint solves: relative residual max {ar.max():.2e} median {np.median(ar):.2e}, sweeps "
+          f"{sim.stats.adjoint_sweeps[:5]}..., fallbacks {sim.stats.adjoint_fallbacks}", flush=True)
+    print(f"diffcube16, precision {precision}: loss {L:.6e} vs {float(z['loss']):.6e}; vector rel errors {vec}; "
           f"32 params: max |gpu - fd| {worst_fd:.2e}, |gpu - fp64 adjoint| {worst_adj:.2e} (of family max); "
+          f"per parameter max |gpu - fd| / |fd| {worst_rel_fd:.2e}, |gpu - adjoint| / |adjoint| {worst_rel_adj:.2e}; "
           f"forward {fwd:.2f}s backward {bwd:.2f}s ratio {bwd / fwd:.2f}", flush=True)
     assert abs(L - float(z["loss"])) <= 1e-3 * float(z["loss"])
-    assert worst_fd <= 1e-3 and worst_adj <= 1e-3
     assert all(e <= 1e-2 for e in vec.values())
+    if precision == 64:
+        assert res.max() < 1e-6
+        assert worst_rel_fd <= 1e-3 and worst_rel_adj <= 1e-5
+    else:
+        assert worst_fd <= 1e-3 and worst_adj <= 1e-3
+
+
+def test_fp64_state_round_trips_and_rounds_into_the_fp32_state(small):
+    """vbd_upload_state_f64 / vbd_download_state_f64 are exact; the FP32 arrays hold the rounding
+    of the FP64 state after an upload and after a converged step; an FP32 upload replaces the
+    FP64 state by its promotion; precision 32 refuses the FP64 state calls."""
+    from vbd_tetmesh_amd import _capi as A
+    m, _, _, v0, _, _, _ = small
+    sim = DiffSim(m, steps=1, precision=64)
+    rng = np.random.default_rng(5)
+    x = m.rest.astype(np.float64) + rng.normal(scale=1e-3, size=m.rest.shape)
+    v = rng.normal(size=m.rest.shape)
+    sim.dev.set_state_f64(x, v)
+    gx, gv = sim.dev.state_f64()
+    assert np.array_equal(gx, x) and np.array_equal(gv, v)
+    assert np.array_equal(sim.dev.positions(), x.astype(np.float32))
+    sim.dev.set_state_f64(None, np.zeros_like(v))  # velocities only
+    assert np.array_equal(sim.dev.state_f64()[0], x)
+    sim._step()
+    gx, gv = sim.dev.state_f64()
+    assert np.array_equal(sim.dev.positions(), gx.astype(np.float32))
+    assert np.array_equal(sim.dev.velocities(), gv.astype(np.float32))
+    x32 = (m.rest + np.float32(1e-3)).astype(np.float32)
+    sim.dev.set_positions(x32)
+    assert np.array_equal(sim.dev.state_f64()[0], x32.astype(np.float64))
+    sim.dev.set_differentiation_precision(32)
+    with pytest.raises(A.VBDError) as ei:
+        sim.dev.state_f64()
+    assert ei.value.code == A.VBD_ERR_STATE
+
+
+def test_fp64_kernels_match_the_fp64_oracle(small):
+    """With precision 64 the HVP and the parameter gradients equal the FP64 oracle's to 1e-12
+    relative, and the adjoint solve a float64 direct solve to 1e-9 (double vectors), the oracle
+    given the device's inputs: the FP32 DmInv, rest volumes and masses of the host preparation
+    and the FP32 step h."""
+    from vbd_tetmesh_amd.solver import prepare_host
+    m, _, _, _, _, _, x = small
+    t, _, _, dminv32, vol32 = prepare_host(m)
+    P = D.make_problem(m, h=float(np.float32(1.0 / 60.0)))
+    P.dminv = dminv32.astype(np.float64).reshape(-1, 3, 3)
+    P.vol = vol32.astype(np.float64)
+    P.mass = t.vertexMass.astype(np.float64)
+    D._prepare(P)
+    sim = DiffSim(m, steps=1, precision=64)
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    xd = x.astype(np.float64)
+    sim.dev.set_state_f64(xd, None)
+    mu, lam = m.tet_mu.astype(np.float64), m.tet_lambda.astype(np.float64)
+    rng = np.random.default_rng(11)
+    w = rng.normal(size=x.shape)
+    w[m.fixed == 1] = 0
+    fr = m.fixed == 0
+    hv = sim.dev.hessian_vector_f64(w)
+    e_hv = relmax(hv[fr], D.hessian_vector(P, xd, w, mu, lam)[fr])
+    dm, dl = sim.dev.parameter_gradients_f64(w)
+    rm, rl = D.param_grads(P, xd, w, mu, lam)
+    rhs = rng.normal(size=x.shape)
+    rhs[m.fixed == 1] = 0
+    ws, sweeps, rr = sim.dev.adjoint_solve_f64(rhs, rel_tol=1e-12, rho=0.9)
+    A_ = D.system_matrix(P, xd, mu, lam)
+    wd = D.solve_linear(P, A_, rhs.ravel(), mu, lam).reshape(-1, 3)
+    e_w = relmax(ws[fr], wd[fr])
+    print(f"fp64 kernels: hvp {e_hv:.2e}, param grads {relmax(dm, rm):.2e} {relmax(dl, rl):.2e}, adjoint {sweeps} "
+          f"sweeps residual {rr:.1e} vs direct {e_w:.2e}", flush=True)
+    assert e_hv <= 1e-12 and relmax(dm, rm) <= 1e-12 and relmax(dl, rl) <= 1e-12
+    assert rr <= 1e-12 and e_w <= 1e-9

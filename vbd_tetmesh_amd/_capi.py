@@ -142,6 +142,13 @@ SIGNATURES = [
     ("vbd_adjoint_solve", C.c_int, [H, f32p, f32p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, i32p,
                                     f64p]),
     ("vbd_parameter_gradients", C.c_int, [H, f32p, f32p, f32p]),
+    ("vbd_set_differentiation_precision", C.c_int, [H, C.c_int32]),
+    ("vbd_download_state_f64", C.c_int, [H, f64p, f64p]),
+    ("vbd_upload_state_f64", C.c_int, [H, f64p, f64p]),
+    ("vbd_hessian_vector_f64", C.c_int, [H, f64p, f64p]),
+    ("vbd_adjoint_solve_f64", C.c_int, [H, f64p, f64p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, i32p,
+                                        f64p]),
+    ("vbd_parameter_gradients_f64", C.c_int, [H, f64p, f64p, f64p]),
 ]
 
 

@@ -1,11 +1,17 @@
 """Differentiable rollouts of converged VBD steps (implicit differentiation on the device).
 
 Forward: `steps` implicit-Euler steps, each iterated on the GPU until
-|grad G| <= rel_tol |M g| (vbd_step_converged), from x_0 = rest with initial velocity v_0 and
-per-tet materials (mu, lambda).  The reference has no differentiation; the step itself follows
-VBDsolver::step (VBD.cpp:60-101, TetMesh.h:305-348).
+|grad G| <= max(abs_tol, rel_tol |M g|) (vbd_step_converged; by default |grad G| < 1e-6), from
+x_0 = rest with initial velocity v_0 and per-tet materials (mu, lambda).  The reference has no
+differentiation; the step itself follows VBDsolver::step (VBD.cpp:60-101, TetMesh.h:305-348).
 
-Backward, per step n (x_{n+1} = argmin G, v_{n+1} = c (x_{n+1} - x_n)/h, c = 0.999):
+Precision: 64 (default) runs the differentiation entry points on the handle's FP64 state and
+checkpoints it in double, so every step converges below 1e-6; 32 runs the FP32 kernels on the
+FP32 state, whose converged steps stall near H ulp(x) per vertex (about 5e-3 |M g| on a 16^3
+unit cube) and end on the stagnation test.
+
+Backward, per step n (x_{n+1} = argmin G, v_{n+1} = c (x_{n+1} - x_n)/h, c = 0.999; with
+precision 64 the vectors cross the ABI in double, since the update of a cancels most of its terms):
     a_hat = a + (c/h) b
     A(x_{n+1}) w = a_hat           colored block Gauss-Seidel on the device (vbd_adjoint_solve)
     mu_bar  += -V (F - C) : dF(w),  lambda_bar += -V (J - 1) C : dF(w)   (vbd_parameter_gradients)
@@ -41,9 +47,10 @@ class DiffSim:
     """Device rollout + adjoint for one mesh with per-tet materials."""
 
     def __init__(self, mesh, steps: int = 30, h: float = 1.0 / 60.0, rho: float = 0.95, max_iterations: int = 4000,
-                 check_every: int = 10, rel_tol: float = 1e-6, stall_checks: int = 10, adj_max_iterations: int = 8000,
-                 adj_rel_tol: float = 1e-5, adj_rho: float = 0.95, adj_stall_checks: int = 5, checkpoint_every: int = 5,
-                 device: int = 0, damping: float = 0.999):
+                 check_every: int = 10, rel_tol: float = 0.0, stall_checks: int = 10, adj_max_iterations: int = 8000,
+                 adj_rel_tol: Optional[float] = None, adj_rho: float = 0.95, adj_stall_checks: int = 5,
+                 checkpoint_every: int = 5, device: int = 0, damping: float = 0.999, precision: int = 64,
+                 abs_tol: float = 1e-6):
         if mesh.tet_mu is None:
             raise ValueError("the differentiable rollout needs per-tet materials (mesh.tet_mu / tet_lambda)")
         t, off, verts, dminv, vol = prepare_host(mesh)
@@ -52,8 +59,14 @@ class DiffSim:
         self.params = VBDParams(dt=h, numIterations=max_iterations, substeps=1, gravity=mesh.gravity,
                                 useAcceleration=rho > 0, accelerationRho=rho, velocityDamping=damping)
         self.dev.set_params(self.params)
-        self.steps, self.h, self.c = steps, h, damping
-        self.max_iterations, self.check_every, self.rel_tol = max_iterations, check_every, rel_tol
+        self.precision = int(precision)
+        self.dev.set_differentiation_precision(self.precision)
+        if adj_rel_tol is None:  # the FP32 adjoint residual stalls near 1e-6
+            adj_rel_tol = 1e-10 if self.precision == 64 else 1e-5
+        # the step's h and damping as the device holds them (float32): the backward recursion
+        # (a <- M/h^2 w - (c/h) b) cancels most of its terms, so it must use the forward's values
+        self.steps, self.h, self.c = steps, float(np.float32(h)), float(np.float32(damping))
+        self.max_iterations, self.check_every, self.rel_tol, self.abs_tol = max_iterations, check_every, rel_tol, abs_tol
         self.stall_checks, self.adj_stall_checks = stall_checks, adj_stall_checks
         self.adj_max_iterations, self.adj_rel_tol, self.adj_rho = adj_max_iterations, adj_rel_tol, adj_rho
         self.k = max(1, checkpoint_every)
@@ -65,8 +78,23 @@ class DiffSim:
 
     # -- forward
     def _step(self):
-        it, res = self.dev.step_converged(self.max_iterations, self.check_every, 0.0, self.rel_tol, self.stall_checks)
+        it, res = self.dev.step_converged(self.max_iterations, self.check_every, self.abs_tol, self.rel_tol,
+                                          self.stall_checks)
         return it, res
+
+    def _state(self):
+        """(positions, velocities) of the device state, in double with precision 64."""
+        if self.precision == 64:
+            return self.dev.state_f64()
+        return self.dev.positions(), self.dev.velocities()
+
+    def _set_state(self, x, v=None):
+        if self.precision == 64:
+            self.dev.set_state_f64(x, v)
+        else:
+            self.dev.set_positions(x)
+            if v is not None:
+                self.dev.set_velocities(v)
 
     def rollout(self, mu, lam, v0) -> Tuple[np.ndarray, Dict[int, Tuple[np.ndarray, np.ndarray]]]:
         """Returns (x_N, checkpoints {n: (x_n, v_n)})."""
@@ -78,23 +106,22 @@ class DiffSim:
         self.stats = RolloutStats()
         for n in range(self.steps):
             if n % self.k == 0:
-                ck[n] = (self.dev.positions(), self.dev.velocities())
+                ck[n] = self._state()
             it, res = self._step()
             self.stats.forward_iterations.append(it)
             self.stats.forward_residuals.append(res)
-        x = self.dev.positions()
+        x = self._state()[0]
         self.stats.forward_seconds = time.perf_counter() - t0
         return x, ck
 
     def _segment_states(self, start: int, end: int, ck) -> List[np.ndarray]:
         """Positions x_start .. x_end recomputed from the checkpoint at `start`."""
         x, v = ck[start]
-        self.dev.set_positions(x)
-        self.dev.set_velocities(v)
+        self._set_state(x, v)
         xs = [x]
         for _ in range(start, end):
             self._step()
-            xs.append(self.dev.positions())
+            xs.append(self._state()[0])
         return xs
 
     # -- backward
@@ -118,11 +145,12 @@ class DiffSim:
             xs = self._segment_states(start, end, ck)
             for n in range(end - 1, start - 1, -1):
                 ahat = a + (c / h) * b
-                self.dev.set_positions(xs[n + 1 - start])
+                self._set_state(xs[n + 1 - start])
                 w, sweeps, rr = self._adjoint_solve(ahat, w)
                 self.stats.adjoint_sweeps.append(sweeps)
                 self.stats.adjoint_residuals.append(rr)
-                dm, dl = self.dev.parameter_gradients(w)
+                dm, dl = (self.dev.parameter_gradients_f64(w) if self.precision == 64 else
+                          self.dev.parameter_gradients(w))
                 gmu += dm
                 glam += dl
                 w64 = w.astype(np.float64)
@@ -138,14 +166,15 @@ class DiffSim:
         """Chebyshev-accelerated solve; plain Gauss-Seidel (rho = 0) if the accelerated one diverges."""
         kw = dict(max_iterations=self.adj_max_iterations, check_every=self.check_every, rel_tol=self.adj_rel_tol,
                   stall_checks=self.adj_stall_checks)
+        solve = self.dev.adjoint_solve_f64 if self.precision == 64 else self.dev.adjoint_solve
         if self.adj_rho > 0:
             try:
-                return self.dev.adjoint_solve(rhs, w0=w0, rho=self.adj_rho, **kw)
+                return solve(rhs, w0=w0, rho=self.adj_rho, **kw)
             except A.VBDError as e:
                 if e.code != A.VBD_ERR_NUMERIC:
                     raise
                 self.stats.adjoint_fallbacks += 1
-        return self.dev.adjoint_solve(rhs, w0=w0, rho=0.0, **kw)
+        return solve(rhs, w0=w0, rho=0.0, **kw)
 
     def loss_and_grads(self, mu, lam, v0, x_target):
         x, ck = self.rollout(mu, lam, v0)
@@ -165,7 +194,7 @@ def make_autograd_function():
             x, ck = sim.rollout(mu.detach().cpu().numpy(), lam.detach().cpu().numpy(), v0.detach().cpu().numpy())
             ctx.sim, ctx.ck = sim, ck
             ctx.dev = mu.device
-            return torch.from_numpy(x).to(device=mu.device, dtype=torch.float32)
+            return torch.from_numpy(np.ascontiguousarray(x)).to(device=mu.device, dtype=torch.float32)
 
         @staticmethod
         def backward(ctx, gx):

@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["csrc/vbd_kernels.hip", "csrc/vbd_capi.hip"]
+SOURCES = ["csrc/vbd_kernels.hip", "csrc/vbd_fp64.hip", "csrc/vbd_capi.hip"]
 OUT = os.path.join(HERE, "libvbd_hip.so")
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.

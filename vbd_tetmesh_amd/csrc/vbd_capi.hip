@@ -61,6 +61,14 @@ struct vbd_solver {
     float* d_x3 = nullptr;     // 3 * nV staging
     float* h_x3 = nullptr;     // 3 * nV pinned host staging of large transfers
     hipEvent_t xev[16] = {};   // one per chunk of a staged device-to-host copy
+    // FP64 differentiation state (vbd_set_differentiation_precision(h, 64)); fp64Fresh: it holds
+    // the current state (the FP32 arrays are its rounding); cleared by every entry point that
+    // changes the FP32 state, after which the next FP64 call promotes the FP32 state
+    int diffBits = 32;
+    Dev64 d64;
+    bool fp64Fresh = false;
+    double* d_x3d = nullptr;   // 3 * nV staging of FP64 state transfers
+    double* d_tgrad64 = nullptr;  // 2 * nT FP64 parameter gradients
 };
 
 // Scratch of vbd_total_energy_noalloc: the per-block partial sums of the energy reduction (and
@@ -226,6 +234,12 @@ void free_scratch(vbd_solver* h) {
     h->d_w = h->d_rhs = h->d_out = h->d_wsave[0] = h->d_wsave[1] = nullptr;
     h->d_r2 = h->d_sum = h->d_epart = nullptr;
     h->d_tgrad = h->d_x3 = nullptr;
+    free_dev64(h->d64);
+    if (h->d_x3d) (void)hipFree(h->d_x3d);
+    if (h->d_tgrad64) (void)hipFree(h->d_tgrad64);
+    h->d_x3d = nullptr;
+    h->d_tgrad64 = nullptr;
+    h->fp64Fresh = false;
 }
 
 // VBDsolver::getAcceleratorOmega (VBD.cpp:302-311): float arguments, double literals, so the
@@ -886,6 +900,7 @@ int vbd_upload_positions(vbd_solver* h, const float* x) {
     if (rc) return rc;
     if (!x && h->m.nV) return fail(h, VBD_ERR_ARG, "null positions");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     return upload_vertex_array(h, x, h->m.pos);
 }
 
@@ -894,6 +909,7 @@ int vbd_upload_inertia(vbd_solver* h, const float* y) {
     if (rc) return rc;
     if (!y && h->m.nV) return fail(h, VBD_ERR_ARG, "null inertia");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     return upload_vertex_array(h, y, h->m.inert);
 }
 
@@ -902,6 +918,7 @@ int vbd_upload_velocities(vbd_solver* h, const float* v) {
     if (rc) return rc;
     if (!v && h->m.nV) return fail(h, VBD_ERR_ARG, "null velocities");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     return upload_vertex_array(h, v, h->m.vel);
 }
 
@@ -966,6 +983,7 @@ int vbd_solve_iteration(vbd_solver* h, float dt_sqr_reciprocal, float mu, float 
     int rc = check_ready(h);
     if (rc) return rc;
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     ContactArgs ct{};
     if (h->paramsSet) ct = contact_of(h->params);
     HIP_CHECK(h, enqueue_sweep(h, dt_sqr_reciprocal, mu, lambda, ct, nullptr));
@@ -978,6 +996,7 @@ int vbd_solve(vbd_solver* h, int32_t num_iterations, float dt_sqr_reciprocal, fl
     if (rc) return rc;
     if (num_iterations < 0) return fail(h, VBD_ERR_ARG, "negative iteration count");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     ContactArgs ct{};
     if (h->paramsSet) ct = contact_of(h->params);
     if (num_iterations == 0 && !start_from_inertia) return VBD_OK;
@@ -1011,6 +1030,7 @@ int vbd_forward_step(vbd_solver* h) {
     if (rc) return rc;
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     HIP_CHECK(h, launch_predict(h->m, h->params.dt / (float)h->params.substeps, h->params.gravity, h->stream));
     return VBD_OK;
 }
@@ -1020,6 +1040,7 @@ int vbd_update_velocity(vbd_solver* h) {
     if (rc) return rc;
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     HIP_CHECK(h, launch_velocity(h->m, h->params.dt / (float)h->params.substeps, h->params.velocity_damping, h->stream));
     return VBD_OK;
 }
@@ -1041,6 +1062,7 @@ int vbd_step(vbd_solver* h, int32_t num_frames) {
     // forwardStep (x_prev = x, y = x for pinned vertices, VBD.cpp:103-112) and the energy record
     if (h->m.nV == 0) return VBD_OK;
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     if (h->energyEvery > 0) {  // buffers of the energy log exist before a capture
         HIP_CHECK(h, ensure_energy_scratch(h));
         const int n = energy_log_slots(h);
@@ -1161,6 +1183,7 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
     if (rc) return rc;
     if (iterations <= 0) return fail(h, VBD_ERR_ARG, "iterations must be positive");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     const vbd_step_params p = h->params;
     const bool accel = h->paramsSet && p.use_acceleration;
     const float hstep = h->paramsSet ? p.dt / (float)p.substeps : 1.0f / 60.0f;
@@ -1255,6 +1278,7 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
     if (rc) return rc;
     if (iterations <= 0) return fail(h, VBD_ERR_ARG, "iterations must be positive");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     const std::string key = tune_key(h->m, h->paramsSet && h->params.contact_on, h->materialModel);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1387,6 +1411,7 @@ int vbd_time_color_launches(vbd_solver* h, int32_t sweeps, double* ms_per_launch
     if (rc) return rc;
     if (sweeps <= 0) return fail(h, VBD_ERR_ARG, "sweeps must be positive");
     DeviceGuard g(h->device);
+    h->fp64Fresh = false;  // the FP32 state changes
     const vbd_step_params p = h->params;
     const bool accel = h->paramsSet && p.use_acceleration;
     const float hstep = h->paramsSet ? p.dt / (float)p.substeps : 1.0f / 60.0f;
@@ -1605,6 +1630,79 @@ hipError_t enqueue_sum_to_host(vbd_solver* h) {
     return e;
 }
 
+// ---- FP64 differentiation state ----
+bool fp64(const vbd_solver* h) { return h->diffBits == 64; }
+
+double step_sdt64(const vbd_solver* h) {
+    const double hs = (double)(h->params.dt / (float)h->params.substeps);
+    return 1.0 / (hs * hs);
+}
+
+// The FP64 arrays exist and hold the current state (promoted from the FP32 state when it changed).
+hipError_t ensure_fp64(vbd_solver* h) {
+    hipError_t e = ensure_scratch(h);
+    if (e == hipSuccess) e = alloc_dev64(h->d64, h->m.nV);
+    if (e == hipSuccess && !h->d_x3d) e = hipMalloc(&h->d_x3d, sizeof(double) * 3 * (size_t)(h->m.nV > 0 ? h->m.nV : 1));
+    if (e != hipSuccess || h->fp64Fresh) return e;
+    const int n = h->m.nV;
+    if ((e = launch_f4_to_d4(n, h->m.pos, h->d64.pos, h->stream)) != hipSuccess) return e;
+    if ((e = launch_f4_to_d4(n, h->m.inert, h->d64.inert, h->stream)) != hipSuccess) return e;
+    if ((e = launch_f4_to_d4(n, h->m.vel, h->d64.vel, h->stream)) != hipSuccess) return e;
+    if ((e = launch_f4_to_d4(n, h->m.xprev, h->d64.xprev, h->stream)) != hipSuccess) return e;
+    h->fp64Fresh = true;
+    return e;
+}
+
+// The FP32 state as the rounding of the FP64 state.
+hipError_t round_fp64(vbd_solver* h) {
+    const int n = h->m.nV;
+    hipError_t e = launch_d4_to_f4(n, h->d64.pos, h->m.pos, h->stream);
+    if (e == hipSuccess) e = launch_d4_to_f4(n, h->d64.inert, h->m.inert, h->stream);
+    if (e == hipSuccess) e = launch_d4_to_f4(n, h->d64.vel, h->m.vel, h->stream);
+    if (e == hipSuccess) e = launch_d4_to_f4(n, h->d64.xprev, h->m.xprev, h->stream);
+    return e;
+}
+
+// host 3N floats (caller order) -> slot double4 buffer, pinned slots zeroed
+hipError_t upload_vec64(vbd_solver* h, const float* src, d4* dst) {
+    hipError_t e = h2d_xyz(h, src);
+    if (e == hipSuccess) e = launch_scatter_fd(h->m.nV, h->d_x3, h->m.inv, dst, h->stream);
+    if (e == hipSuccess && h->m.nV > h->m.nFree)
+        e = hipMemsetAsync(dst + h->m.nFree, 0, sizeof(d4) * (size_t)(h->m.nV - h->m.nFree), h->stream);
+    return e;
+}
+
+hipError_t download_vec64(vbd_solver* h, const d4* src, float* dst) {
+    hipError_t e = launch_gather_df(h->m.nV, src, h->m.inv, h->d_x3, h->stream);
+    if (e == hipSuccess) e = d2h_xyz(h, dst);
+    return e;
+}
+
+// host 3N doubles (caller order) <-> slot double4 buffer (pinned slots zeroed on upload)
+hipError_t upload_vec64d(vbd_solver* h, const double* src, d4* dst) {
+    const size_t bytes = sizeof(double) * 3 * (size_t)h->m.nV;
+    hipError_t e = bytes ? hipMemcpyAsync(h->d_x3d, src, bytes, hipMemcpyHostToDevice, h->stream) : hipSuccess;
+    if (e == hipSuccess) e = launch_scatter_d(h->m.nV, h->d_x3d, h->m.inv, dst, h->stream);
+    if (e == hipSuccess && h->m.nV > h->m.nFree)
+        e = hipMemsetAsync(dst + h->m.nFree, 0, sizeof(d4) * (size_t)(h->m.nV - h->m.nFree), h->stream);
+    return e;
+}
+
+hipError_t download_vec64d(vbd_solver* h, const d4* src, double* dst) {
+    const size_t bytes = sizeof(double) * 3 * (size_t)h->m.nV;
+    hipError_t e = launch_gather_d(h->m.nV, src, h->m.inv, h->d_x3d, h->stream);
+    if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, h->d_x3d, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e;
+}
+
+hipError_t forward_residual64(vbd_solver* h, double* norm2) {
+    hipError_t e = launch_pass64(1, h->m, h->d64, 0, h->m.nFree, nullptr, nullptr, h->d64.out, h->d_r2, nullptr,
+                                 step_mu(h), step_lambda(h), step_sdt64(h), contact_of(h->params), h->stream);
+    if (e == hipSuccess) e = sum_r2(h, h->m.nFree, norm2);
+    return e;
+}
+
 
 }  // namespace
 
@@ -1634,9 +1732,9 @@ int vbd_residual_norm(vbd_solver* h, double* norm, double* gravity_load) {
         return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     DeviceGuard g(h->device);
-    HIP_CHECK(h, ensure_scratch(h));
+    HIP_CHECK(h, fp64(h) ? ensure_fp64(h) : ensure_scratch(h));
     double r2 = 0, l2 = 0;
-    HIP_CHECK(h, forward_residual(h, &r2));
+    HIP_CHECK(h, fp64(h) ? forward_residual64(h, &r2) : forward_residual(h, &r2));
     HIP_CHECK(h, launch_gravity_load2(h->m, h->params.gravity, h->d_r2, h->stream));
     HIP_CHECK(h, sum_r2(h, h->m.nFree, &l2));
     if (norm) *norm = std::sqrt(r2);
@@ -1654,17 +1752,25 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
     if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
     if (h->params.substeps != 1) return fail(h, VBD_ERR_ARG, "converged steps support substeps == 1");
     DeviceGuard g(h->device);
-    HIP_CHECK(h, ensure_scratch(h));
+    const bool f64 = fp64(h);
+    HIP_CHECK(h, f64 ? ensure_fp64(h) : ensure_scratch(h));
+    if (!f64) h->fp64Fresh = false;  // the FP32 state changes
     const vbd_step_params& p = h->params;
     const float hstep = p.dt;
     const float sdt = inv_h2_ref(hstep);
+    const double sdt64 = step_sdt64(h);
     const ContactArgs ct = contact_of(p);
     double l2 = 0;
     HIP_CHECK(h, launch_gravity_load2(h->m, p.gravity, h->d_r2, h->stream));
     HIP_CHECK(h, sum_r2(h, h->m.nFree, &l2));
     const double tol = std::max((double)abs_tol, (double)rel_tol * std::sqrt(l2));
-    HIP_CHECK(h, launch_predict(h->m, hstep, p.gravity, h->stream));
-    if (p.start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
+    if (f64) {
+        HIP_CHECK(h, launch_predict64(h->m, h->d64, (double)hstep, p.gravity, h->stream));
+        if (p.start_from_inertia) HIP_CHECK(h, launch_copy_d4(h->m.nFree, h->d64.inert, h->d64.pos, h->stream));
+    } else {
+        HIP_CHECK(h, launch_predict(h->m, hstep, p.gravity, h->stream));
+        if (p.start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
+    }
     float omega = 1.0f;
     int it = 0, stalled = 0;
     double r2 = 0, best = 1e300;
@@ -1676,15 +1782,29 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
         om.assign(n, 1.0f);
         for (int j = 0; j < n && accel; ++j) om[j] = omega = omega_ref(it + j + 1, p.acceleration_rho, omega);
         const int it0 = it;
-        HIP_CHECK(h, run_block(h, block_key('F', it0, om, 0.f), [&]() -> hipError_t {
+        HIP_CHECK(h, run_block(h, block_key(f64 ? 'D' : 'F', it0, om, 0.f), [&]() -> hipError_t {
             for (int j = 0; j < n; ++j) {
                 const int k = it0 + j;
-                hipError_t e = enqueue_sweep(h, sdt, p.mu, p.lambda, ct, accel ? h->m.save[k & 1] : nullptr);
-                if (e == hipSuccess && accel && om[j] > 1.0f) e = launch_blend(h->m, om[j], h->m.save[(k + 1) & 1], h->stream);
+                hipError_t e = hipSuccess;
+                if (f64) {
+                    for (int c = 0; c < h->m.nColors && e == hipSuccess; ++c) {
+                        const int s0 = h->m.colorSlotOff[c], cnt = h->m.colorSlotOff[c + 1] - s0;
+                        e = launch_pass64(0, h->m, h->d64, s0, cnt, nullptr, nullptr, nullptr, nullptr,
+                                          accel ? h->d64.save[k & 1] : nullptr, p.mu, p.lambda, sdt64, ct, h->stream);
+                    }
+                    if (e == hipSuccess && accel && om[j] > 1.0f)
+                        e = launch_blend64(h->m.nFree, h->d64.pos, h->d64.save[(k + 1) & 1], om[j], h->stream);
+                } else {
+                    e = enqueue_sweep(h, sdt, p.mu, p.lambda, ct, accel ? h->m.save[k & 1] : nullptr);
+                    if (e == hipSuccess && accel && om[j] > 1.0f)
+                        e = launch_blend(h->m, om[j], h->m.save[(k + 1) & 1], h->stream);
+                }
                 if (e != hipSuccess) return e;
             }
-            hipError_t e = launch_pass(3, h->m, 0, h->m.nFree, pass_lpv(h), 256, nullptr, nullptr, h->d_out, h->d_r2,
-                                       nullptr, p.mu, p.lambda, sdt, ct, h->stream);
+            hipError_t e = f64 ? launch_pass64(1, h->m, h->d64, 0, h->m.nFree, nullptr, nullptr, h->d64.out, h->d_r2,
+                                               nullptr, p.mu, p.lambda, sdt64, ct, h->stream)
+                               : launch_pass(3, h->m, 0, h->m.nFree, pass_lpv(h), 256, nullptr, nullptr, h->d_out,
+                                             h->d_r2, nullptr, p.mu, p.lambda, sdt, ct, h->stream);
             return e == hipSuccess ? enqueue_sum_to_host(h) : e;
         }));
         it += n;
@@ -1704,23 +1824,37 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
             break;
         }
     }
-    HIP_CHECK(h, launch_velocity(h->m, hstep, p.velocity_damping, h->stream));
+    if (f64) {
+        HIP_CHECK(h, launch_velocity64(h->m, h->d64, (double)hstep, (double)p.velocity_damping, h->stream));
+        HIP_CHECK(h, round_fp64(h));
+    } else {
+        HIP_CHECK(h, launch_velocity(h->m, hstep, p.velocity_damping, h->stream));
+    }
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
     if (iterations) *iterations = it;
     if (final_norm) *final_norm = std::sqrt(r2);
     return VBD_OK;
 }
 
-int vbd_hessian_vector(vbd_solver* h, const float* w, float* out) {
+static int hessian_vector_core(vbd_solver* h, const float* w, float* out, const double* w_d, double* out_d) {
     int rc = check_ready(h);
     if (rc) return rc;
     if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
         return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
-    if ((!w || !out) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
+    if (((!w && !w_d) || (!out && !out_d)) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
     DeviceGuard g(h->device);
+    if (fp64(h)) {
+        HIP_CHECK(h, ensure_fp64(h));
+        HIP_CHECK(h, w_d ? upload_vec64d(h, w_d, h->d64.w) : upload_vec64(h, w, h->d64.w));
+        HIP_CHECK(h, hipMemsetAsync(h->d64.out, 0, sizeof(d4) * (size_t)(h->m.nV > 0 ? h->m.nV : 1), h->stream));
+        HIP_CHECK(h, launch_pass64(4, h->m, h->d64, 0, h->m.nFree, h->d64.w, nullptr, h->d64.out, nullptr, nullptr,
+                                   step_mu(h), step_lambda(h), step_sdt64(h), contact_of(h->params), h->stream));
+        HIP_CHECK(h, out_d ? download_vec64d(h, h->d64.out, out_d) : download_vec64(h, h->d64.out, out));
+        return VBD_OK;
+    }
     HIP_CHECK(h, ensure_scratch(h));
-    HIP_CHECK(h, upload_vec(h, w, h->d_w, false));
+    HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
     HIP_CHECK(h, hipMemsetAsync(h->d_out, 0, sizeof(f4) * (size_t)(h->m.nV > 0 ? h->m.nV : 1), h->stream));
     HIP_CHECK(h, launch_pass(2, h->m, 0, h->m.nFree, pass_lpv(h), 256, h->d_w, nullptr, h->d_out, nullptr, nullptr,
                              step_mu(h), step_lambda(h), step_sdt(h), contact_of(h->params), h->stream));
@@ -1728,25 +1862,41 @@ int vbd_hessian_vector(vbd_solver* h, const float* w, float* out) {
     return VBD_OK;
 }
 
-int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_iterations, int32_t check_every,
-                      float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual) {
+int vbd_hessian_vector(vbd_solver* h, const float* w, float* out) { return hessian_vector_core(h, w, out, nullptr, nullptr); }
+
+int vbd_hessian_vector_f64(vbd_solver* h, const double* w, double* out) {
+    if (h && !fp64(h)) return fail(h, VBD_ERR_STATE, "the FP64 vectors need vbd_set_differentiation_precision(h, 64)");
+    return hessian_vector_core(h, nullptr, nullptr, w, out);
+}
+
+static int adjoint_solve_core(vbd_solver* h, const float* rhs, float* w, const double* rhs_d, double* w_d,
+                              int32_t max_iterations, int32_t check_every, float rel_tol, float rho,
+                              int32_t stall_checks, int32_t* iterations, double* rel_residual) {
     int rc = check_ready(h);
     if (rc) return rc;
     if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
         return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
-    if ((!w || !rhs) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
+    if (((!w && !w_d) || (!rhs && !rhs_d)) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
     if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
     if (!(rho >= 0.f && rho < 1.f)) return fail(h, VBD_ERR_ARG, "rho must be in [0, 1)");
     DeviceGuard g(h->device);
-    HIP_CHECK(h, ensure_scratch(h));
+    const bool f64 = fp64(h);
+    HIP_CHECK(h, f64 ? ensure_fp64(h) : ensure_scratch(h));
     const ContactArgs ct = contact_of(h->params);
     const float mu = step_mu(h), lam = step_lambda(h), sdt = step_sdt(h);
+    const double sdt64 = step_sdt64(h);
     const int lpv = pass_lpv(h);
-    HIP_CHECK(h, upload_vec(h, rhs, h->d_rhs, true));
-    HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
     double b2 = 0, r2 = 0;
-    HIP_CHECK(h, launch_norm2_f4(h->m.nFree, h->d_rhs, h->d_r2, h->stream));
+    if (f64) {
+        HIP_CHECK(h, rhs_d ? upload_vec64d(h, rhs_d, h->d64.rhs) : upload_vec64(h, rhs, h->d64.rhs));
+        HIP_CHECK(h, w_d ? upload_vec64d(h, w_d, h->d64.w) : upload_vec64(h, w, h->d64.w));
+        HIP_CHECK(h, launch_norm2_d4(h->m.nFree, h->d64.rhs, h->d_r2, h->stream));
+    } else {
+        HIP_CHECK(h, upload_vec(h, rhs, h->d_rhs, true));
+        HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
+        HIP_CHECK(h, launch_norm2_f4(h->m.nFree, h->d_rhs, h->d_r2, h->stream));
+    }
     HIP_CHECK(h, sum_r2(h, h->m.nFree, &b2));
     const double bn = std::sqrt(b2);
     float omega = 1.0f;
@@ -1760,23 +1910,28 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
         om.assign(n, 1.0f);
         for (int j = 0; j < n && accel; ++j) om[j] = omega = omega_ref(it + j + 1, rho, omega);
         const int it0 = it;
-        HIP_CHECK(h, run_block(h, block_key('A', it0, om, rho), [&]() -> hipError_t {
+        HIP_CHECK(h, run_block(h, block_key(f64 ? 'E' : 'A', it0, om, rho), [&]() -> hipError_t {
             for (int j = 0; j < n; ++j) {
                 const int k = it0 + j;
-                f4* save = accel ? h->d_wsave[k & 1] : nullptr;
                 for (int c = 0; c < h->m.nColors; ++c) {
                     const int s0 = h->m.colorSlotOff[c], cnt = h->m.colorSlotOff[c + 1] - s0;
-                    hipError_t e = launch_pass(0, h->m, s0, cnt, lpv, 256, h->d_w, h->d_rhs, nullptr, nullptr, save, mu,
-                                               lam, sdt, ct, h->stream);
+                    hipError_t e = f64 ? launch_pass64(2, h->m, h->d64, s0, cnt, h->d64.w, h->d64.rhs, nullptr, nullptr,
+                                                       accel ? h->d64.wsave[k & 1] : nullptr, mu, lam, sdt64, ct,
+                                                       h->stream)
+                                       : launch_pass(0, h->m, s0, cnt, lpv, 256, h->d_w, h->d_rhs, nullptr, nullptr,
+                                                     accel ? h->d_wsave[k & 1] : nullptr, mu, lam, sdt, ct, h->stream);
                     if (e != hipSuccess) return e;
                 }
                 if (accel && om[j] > 1.0f) {
-                    hipError_t e = launch_blend_vec(h->m.nFree, h->d_w, h->d_wsave[(k + 1) & 1], om[j], h->stream);
+                    hipError_t e = f64 ? launch_blend64(h->m.nFree, h->d64.w, h->d64.wsave[(k + 1) & 1], om[j], h->stream)
+                                       : launch_blend_vec(h->m.nFree, h->d_w, h->d_wsave[(k + 1) & 1], om[j], h->stream);
                     if (e != hipSuccess) return e;
                 }
             }
-            hipError_t e = launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2, nullptr,
-                                       mu, lam, sdt, ct, h->stream);
+            hipError_t e = f64 ? launch_pass64(3, h->m, h->d64, 0, h->m.nFree, h->d64.w, h->d64.rhs, h->d64.out, h->d_r2,
+                                               nullptr, mu, lam, sdt64, ct, h->stream)
+                               : launch_pass(1, h->m, 0, h->m.nFree, lpv, 256, h->d_w, h->d_rhs, h->d_out, h->d_r2,
+                                             nullptr, mu, lam, sdt, ct, h->stream);
             return e == hipSuccess ? enqueue_sum_to_host(h) : e;
         }));
         it += n;
@@ -1800,10 +1955,24 @@ int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_ite
             break;
         }
     }
-    HIP_CHECK(h, download_vec(h, h->d_w, w));
+    HIP_CHECK(h, w_d ? download_vec64d(h, h->d64.w, w_d)
+                     : (f64 ? download_vec64(h, h->d64.w, w) : download_vec(h, h->d_w, w)));
     if (iterations) *iterations = it;
     if (rel_residual) *rel_residual = bn > 0 ? std::sqrt(r2) / bn : 0.0;
     return VBD_OK;
+}
+
+int vbd_adjoint_solve(vbd_solver* h, const float* rhs, float* w, int32_t max_iterations, int32_t check_every,
+                      float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual) {
+    return adjoint_solve_core(h, rhs, w, nullptr, nullptr, max_iterations, check_every, rel_tol, rho, stall_checks,
+                              iterations, rel_residual);
+}
+
+int vbd_adjoint_solve_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations, int32_t check_every,
+                          float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual) {
+    if (h && !fp64(h)) return fail(h, VBD_ERR_STATE, "the FP64 vectors need vbd_set_differentiation_precision(h, 64)");
+    return adjoint_solve_core(h, nullptr, nullptr, rhs, w, max_iterations, check_every, rel_tol, rho, stall_checks,
+                              iterations, rel_residual);
 }
 
 int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dlambda) {
@@ -1813,14 +1982,96 @@ int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dl
         return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
     if ((!w || !dmu || !dlambda) && h->m.nT) return fail(h, VBD_ERR_ARG, "null argument");
     DeviceGuard g(h->device);
-    HIP_CHECK(h, ensure_scratch(h));
-    HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
-    HIP_CHECK(h, launch_param_grad(h->m, h->d_w, h->d_tgrad, h->d_tgrad + h->m.nT, h->stream));
+    if (fp64(h)) {
+        HIP_CHECK(h, ensure_fp64(h));
+        HIP_CHECK(h, upload_vec64(h, w, h->d64.w));
+        HIP_CHECK(h, launch_param_grad64(h->m, h->d64, h->d64.w, h->d_tgrad, h->d_tgrad + h->m.nT, h->stream));
+    } else {
+        HIP_CHECK(h, ensure_scratch(h));
+        HIP_CHECK(h, upload_vec(h, w, h->d_w, true));
+        HIP_CHECK(h, launch_param_grad(h->m, h->d_w, h->d_tgrad, h->d_tgrad + h->m.nT, h->stream));
+    }
     if (h->m.nT) {
         HIP_CHECK(h, hipMemcpyAsync(dmu, h->d_tgrad, sizeof(float) * h->m.nT, hipMemcpyDeviceToHost, h->stream));
         HIP_CHECK(h, hipMemcpyAsync(dlambda, h->d_tgrad + h->m.nT, sizeof(float) * h->m.nT, hipMemcpyDeviceToHost, h->stream));
     }
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_parameter_gradients_f64(vbd_solver* h, const double* w, double* dmu, double* dlambda) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!fp64(h)) return fail(h, VBD_ERR_STATE, "the FP64 vectors need vbd_set_differentiation_precision(h, 64)");
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
+    if ((!w || !dmu || !dlambda) && h->m.nT) return fail(h, VBD_ERR_ARG, "null argument");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_fp64(h));
+    if (!h->d_tgrad64)
+        HIP_CHECK(h, hipMalloc(&h->d_tgrad64, sizeof(double) * 2 * (size_t)(h->m.nT > 0 ? h->m.nT : 1)));
+    HIP_CHECK(h, upload_vec64d(h, w, h->d64.w));
+    HIP_CHECK(h, launch_param_grad64(h->m, h->d64, h->d64.w, h->d_tgrad64, h->d_tgrad64 + h->m.nT, h->stream));
+    if (h->m.nT) {
+        HIP_CHECK(h, hipMemcpyAsync(dmu, h->d_tgrad64, sizeof(double) * h->m.nT, hipMemcpyDeviceToHost, h->stream));
+        HIP_CHECK(h, hipMemcpyAsync(dlambda, h->d_tgrad64 + h->m.nT, sizeof(double) * h->m.nT, hipMemcpyDeviceToHost,
+                                    h->stream));
+    }
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return VBD_OK;
+}
+
+int vbd_set_differentiation_precision(vbd_solver* h, int32_t bits) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (bits != 32 && bits != 64) return fail(h, VBD_ERR_ARG, "precision must be 32 or 64 bits");
+    if (bits == h->diffBits) return VBD_OK;
+    DeviceGuard g(h->device);
+    drop_graph(h);
+    h->diffBits = bits;
+    h->fp64Fresh = false;  // entering FP64 promotes the FP32 state; leaving keeps its rounding
+    return VBD_OK;
+}
+
+int vbd_download_state_f64(vbd_solver* h, double* positions, double* velocities) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!fp64(h)) return fail(h, VBD_ERR_STATE, "the FP64 state needs vbd_set_differentiation_precision(h, 64)");
+    if ((!positions || !velocities) && h->m.nV) return fail(h, VBD_ERR_ARG, "null output");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_fp64(h));
+    const size_t bytes = sizeof(double) * 3 * (size_t)h->m.nV;
+    double* outs[2] = {positions, velocities};
+    const d4* srcs[2] = {h->d64.pos, h->d64.vel};
+    for (int k = 0; k < 2 && h->m.nV; ++k) {
+        HIP_CHECK(h, launch_gather_d(h->m.nV, srcs[k], h->m.inv, h->d_x3d, h->stream));
+        HIP_CHECK(h, hipMemcpyAsync(outs[k], h->d_x3d, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    }
+    return VBD_OK;
+}
+
+int vbd_upload_state_f64(vbd_solver* h, const double* positions, const double* velocities) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!fp64(h)) return fail(h, VBD_ERR_STATE, "the FP64 state needs vbd_set_differentiation_precision(h, 64)");
+    const size_t n3 = 3 * (size_t)h->m.nV;
+    for (const double* a : {positions, velocities})
+        if (a)
+            for (size_t i = 0; i < n3; ++i)
+                if (!std::isfinite(a[i])) return fail(h, VBD_ERR_NUMERIC, "non-finite value in the FP64 state");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_fp64(h));
+    const size_t bytes = sizeof(double) * n3;
+    const double* ins[2] = {positions, velocities};
+    d4* dsts[2] = {h->d64.pos, h->d64.vel};
+    f4* dst32[2] = {h->m.pos, h->m.vel};
+    for (int k = 0; k < 2 && h->m.nV; ++k) {
+        if (!ins[k]) continue;
+        HIP_CHECK(h, hipMemcpyAsync(h->d_x3d, ins[k], bytes, hipMemcpyHostToDevice, h->stream));
+        HIP_CHECK(h, launch_scatter_d(h->m.nV, h->d_x3d, h->m.inv, dsts[k], h->stream));
+        HIP_CHECK(h, launch_d4_to_f4(h->m.nV, dsts[k], dst32[k], h->stream));
+        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    }
     return VBD_OK;
 }
 

@@ -15,6 +15,7 @@ namespace vbd {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f2 __attribute__((ext_vector_type(2)));
 typedef int i4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 // Device image of one uploaded mesh (or batch union).  Vertex arrays are in "slot" order:
 // free vertices color-major (ascending caller id inside a color), then pinned vertices.
@@ -174,5 +175,42 @@ void free_device_mesh(DevMesh& m);
 // handle's configuration reads them (release also frees the tile cut's padded copies).
 hipError_t ensure_full_records(DevMesh& m, hipStream_t s, std::string& err);
 void release_full_records(DevMesh& m, hipStream_t s);
+
+// FP64 state of the differentiation entry points (slot order, nV double4 each): positions,
+// inertia target, velocity, x at the start of the step, the Chebyshev buffers, and the adjoint
+// vectors.
+struct Dev64 {
+    d4* pos = nullptr;
+    d4* inert = nullptr;
+    d4* vel = nullptr;
+    d4* xprev = nullptr;
+    d4* save[2] = {nullptr, nullptr};
+    d4* w = nullptr;
+    d4* rhs = nullptr;
+    d4* out = nullptr;
+    d4* wsave[2] = {nullptr, nullptr};
+};
+hipError_t alloc_dev64(Dev64& d, int nV);
+void free_dev64(Dev64& d);
+// mode: 0 Gauss-Seidel position update (one color), 1 forward residual grad G, 2 adjoint
+// block-GS update (one color), 3 adjoint residual rhs - A w, 4 A w; on free slots
+// [slot0, slot0 + count), reading the full pair records
+hipError_t launch_pass64(int mode, const DevMesh& m, const Dev64& d, int slot0, int count, d4* w, const d4* rhs, d4* out,
+                         double* r2, d4* save, double mu, double lambda, double sdt, const ContactArgs& ct,
+                         hipStream_t s);
+hipError_t launch_predict64(const DevMesh& m, const Dev64& d, double h, const float g[3], hipStream_t s);
+hipError_t launch_velocity64(const DevMesh& m, const Dev64& d, double h, double damping, hipStream_t s);
+hipError_t launch_blend64(int n, d4* v, const d4* pp, double omega, hipStream_t s);
+hipError_t launch_copy_d4(int n, const d4* src, d4* dst, hipStream_t s);
+hipError_t launch_f4_to_d4(int n, const f4* src, d4* dst, hipStream_t s);
+hipError_t launch_d4_to_f4(int n, const d4* src, f4* dst, hipStream_t s);
+hipError_t launch_norm2_d4(int n, const d4* v, double* r2, hipStream_t s);
+hipError_t launch_scatter_d(int n, const double* src3, const int* inv, d4* dst, hipStream_t s);
+hipError_t launch_scatter_fd(int n, const float* src3, const int* inv, d4* dst, hipStream_t s);
+hipError_t launch_gather_d(int n, const d4* src, const int* inv, double* dst3, hipStream_t s);
+hipError_t launch_gather_df(int n, const d4* src, const int* inv, float* dst3, hipStream_t s);
+hipError_t launch_param_grad64(const DevMesh& m, const Dev64& d, const d4* w, float* dmu, float* dlam, hipStream_t s);
+hipError_t launch_param_grad64(const DevMesh& m, const Dev64& d, const d4* w, double* dmu, double* dlam,
+                               hipStream_t s);
 
 }  // namespace vbd

@@ -440,6 +440,23 @@ class DeviceMesh(_Handle):
         return ms.value, nb.value
 
     # -- differentiation
+    def set_differentiation_precision(self, bits: int):
+        """32 (FP32 state) or 64 (an FP64 copy of the state for the differentiation entry points)."""
+        check(lib.vbd_set_differentiation_precision(self.h, int(bits)), self.h)
+
+    def state_f64(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(positions, velocities) of the FP64 state, (N, 3) float64 each (precision 64)."""
+        x = np.zeros((self.num_verts, 3), np.float64)
+        v = np.zeros((self.num_verts, 3), np.float64)
+        check(lib.vbd_download_state_f64(self.h, ptr(x, C.c_double), ptr(v, C.c_double)), self.h)
+        return x, v
+
+    def set_state_f64(self, x=None, v=None):
+        """Sets the FP64 state's positions and / or velocities (None keeps them)."""
+        xa = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        va = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        check(lib.vbd_upload_state_f64(self.h, ptr(xa, C.c_double), ptr(va, C.c_double)), self.h)
+
     def set_materials(self, tet_mu, tet_lambda):
         mu = np.ascontiguousarray(tet_mu, np.float32)
         lam = np.ascontiguousarray(tet_lambda, np.float32)
@@ -474,6 +491,33 @@ class DeviceMesh(_Handle):
         check(lib.vbd_adjoint_solve(self.h, ptr(rhs, C.c_float), ptr(w, C.c_float), max_iterations, check_every,
                                     rel_tol, rho, stall_checks, C.byref(it), C.byref(rr)), self.h)
         return w, it.value, rr.value
+
+    def hessian_vector_f64(self, w) -> np.ndarray:
+        """A w with double vectors (precision 64)."""
+        w = np.ascontiguousarray(w, np.float64).reshape(-1, 3)
+        out = np.zeros_like(w)
+        check(lib.vbd_hessian_vector_f64(self.h, ptr(w, C.c_double), ptr(out, C.c_double)), self.h)
+        return out
+
+    def adjoint_solve_f64(self, rhs, w0=None, max_iterations: int = 5000, check_every: int = 10,
+                          rel_tol: float = 1e-10, rho: float = 0.0, stall_checks: int = 0):
+        """adjoint_solve with double vectors (precision 64); returns (w, sweeps, relative residual)."""
+        rhs = np.ascontiguousarray(rhs, np.float64).reshape(-1, 3)
+        w = np.zeros_like(rhs) if w0 is None else np.ascontiguousarray(w0, np.float64).reshape(-1, 3).copy()
+        it, rr = C.c_int32(), C.c_double()
+        check(lib.vbd_adjoint_solve_f64(self.h, ptr(rhs, C.c_double), ptr(w, C.c_double), max_iterations, check_every,
+                                        rel_tol, rho, stall_checks, C.byref(it), C.byref(rr)), self.h)
+        return w, it.value, rr.value
+
+    def parameter_gradients_f64(self, w):
+        """parameter_gradients with double w and outputs (precision 64)."""
+        w = np.ascontiguousarray(w, np.float64).reshape(-1, 3)
+        nt = self.info().num_tets
+        dmu = np.zeros(max(nt, 1), np.float64)
+        dlam = np.zeros(max(nt, 1), np.float64)
+        check(lib.vbd_parameter_gradients_f64(self.h, ptr(w, C.c_double), ptr(dmu, C.c_double),
+                                              ptr(dlam, C.c_double)), self.h)
+        return dmu[:nt], dlam[:nt]
 
     def parameter_gradients(self, w):
         w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
