@@ -179,6 +179,15 @@ int vbd_synchronize(vbd_solver* h);
 int vbd_set_stream(vbd_solver* h, void* stream);
 /* Enables (1, default) or disables (0) hipGraph capture of vbd_step. */
 int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
+/* Sweep launches (no reference counterpart).  0 (default): one launch per color and one per
+ * Chebyshev blend.  1: one persistent launch per sweep -- every color through the tiled kernel's
+ * tile code with a device-wide barrier between colors, then the blend -- with the grid sized to
+ * the resident blocks; bitwise the same positions.  8k (k >= 1): the same with at most 8k blocks,
+ * each looping over more tiles.  Applies to the tiled configuration of the
+ * stable Neo-Hookean material (others launch per color).  A barrier whose blocks are not all
+ * resident gives up after a bounded wait; the next vbd_synchronize or download then fails with
+ * VBD_ERR_HIP. */
+int vbd_set_persistent(vbd_solver* h, int32_t mode);
 
 /* ---- material model ------------------------------------------------------------------ */
 /* VBD_MATERIAL_STABLE_NEO_HOOKEAN (default): the CPU solver's material, VBDsolver / Energy.cpp.

@@ -117,6 +117,7 @@ SIGNATURES = [
     ("vbd_step", C.c_int, [H, C.c_int32]),
     ("vbd_synchronize", C.c_int, [H]),
     ("vbd_set_graph_enabled", C.c_int, [H, C.c_int32]),
+    ("vbd_set_persistent", C.c_int, [H, C.c_int32]),
     ("vbd_set_launch_config", C.c_int, [H, C.c_int32, C.c_int32, C.c_int32]),
     ("vbd_autotune", C.c_int, [H, C.c_int32, i32p, i32p, i32p, f32p]),
     ("vbd_time_iterations", C.c_int, [H, C.c_int32, f64p, i32p, f64p, i32p]),

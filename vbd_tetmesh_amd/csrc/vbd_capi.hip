@@ -64,6 +64,8 @@ struct vbd_solver {
     // FP64 differentiation state (vbd_set_differentiation_precision(h, 64)); fp64Fresh: it holds
     // the current state (the FP32 arrays are its rounding); cleared by every entry point that
     // changes the FP32 state, after which the next FP64 call promotes the FP32 state
+    int persistent = 0;          // 1 or a grid cap (8k): one persistent launch per sweep (tiled stable Neo-Hookean)
+    unsigned* d_bar = nullptr;   // its grid barrier: count, generation, timeout flag
     int diffBits = 32;
     Dev64 d64;
     bool fp64Fresh = false;
@@ -295,6 +297,30 @@ hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const
     return hipSuccess;
 }
 
+// Polls of a persistent sweep's grid barrier before it gives up (each poll sleeps about 50 ns,
+// and an atomic read crosses the device): a few milliseconds at the least, far above any
+// legitimate wait (a color's slowest block), and a bound on a launch whose blocks are not all
+// resident.
+constexpr unsigned PERSIST_SPIN = 200000;
+
+bool persistent_on(const vbd_solver* h) {
+    return h->persistent && h->lpv == 0 && h->materialModel == VBD_MATERIAL_STABLE_NEO_HOOKEAN && h->d_bar;
+}
+
+// Grid of the persistent sweep: the resident blocks (occupancy x CUs), at most the largest
+// color's tile count and the caller's cap, a multiple of 8; 0 when not even 8 blocks fit.
+int persistent_grid(vbd_solver* h, const ContactArgs& ct, bool save_on) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) return 0;
+    const int occ = persistent_occupancy(h->m, h->unroll, h->block, ct, save_on);
+    int tiles = 0;
+    for (int c : h->m.colorTileCount) tiles = std::max(tiles, c);
+    int g = std::min(occ * cus, (tiles + 7) & ~7);
+    if (h->persistent >= 8) g = std::min(g, h->persistent);
+    g &= ~7;
+    return g >= 8 ? g : 0;
+}
+
 // Iteration loop of VBDsolver::step (VBD.cpp:75-96) / VBD_GPU_Solver::solveVBD (.cu:338-361):
 // the color kernels save each vertex's start-of-sweep position into save[it & 1]; after the
 // sweep, omega > 1 blends against save[(it + 1) & 1], the start of the previous sweep, which
@@ -304,8 +330,19 @@ hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const
 hipError_t enqueue_iterations(vbd_solver* h, int iters, float sdt, float mu, float lambda, bool accel, float rho,
                               const ContactArgs& ct, double* elog = nullptr, int every = 0) {
     float omega = 1.0f;
+    const int pgrid = persistent_on(h) ? persistent_grid(h, ct, accel) : 0;
     for (int it = 0; it < iters; ++it) {
         f4* save = accel ? h->m.save[it & 1] : nullptr;
+        if (pgrid > 0) {  // one launch: every color, then the blend
+            if (accel) omega = omega_ref(it + 1, rho, omega);
+            hipError_t e = launch_sweep_persistent(h->m, h->unroll, h->block, pgrid, sdt, mu, lambda, ct, save,
+                                                   h->m.save[(it + 1) & 1], accel ? omega : 1.0f, h->d_bar,
+                                                   PERSIST_SPIN, h->stream);
+            if (e == hipSuccess && elog && every > 0 && it % every == 0)
+                e = launch_energy(h->m, 0, mu, lambda, h->d_epart, elog + it / every, h->stream);
+            if (e != hipSuccess) return e;
+            continue;
+        }
         hipError_t e = enqueue_sweep(h, sdt, mu, lambda, ct, save);
         if (e != hipSuccess) return e;
         if (accel) {
@@ -433,13 +470,27 @@ int upload_vertex_array(vbd_solver* h, const float* src, f4* dst) {
     return VBD_OK;
 }
 
+// A persistent sweep whose grid barrier timed out leaves its flag raised: reported (and the
+// barrier reset) at the next synchronising call, since the positions it left are not a sweep's.
+int check_persistent(vbd_solver* h) {
+    if (!h->d_bar) return VBD_OK;
+    unsigned flag = 0;
+    HIP_CHECK(h, hipMemcpyAsync(&flag, h->d_bar + 2, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    if (!flag) return VBD_OK;
+    HIP_CHECK(h, hipMemsetAsync(h->d_bar, 0, 3 * sizeof(unsigned), h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return fail(h, VBD_ERR_HIP, "persistent sweep: a grid barrier timed out (blocks not co-resident); the positions "
+                                "are not valid");
+}
+
 int download_vertex_array(vbd_solver* h, const f4* src, float* dst) {
     const size_t n = (size_t)h->m.nV;
     HIP_CHECK(h, ensure_stage3(h));
     hipError_t e = launch_gather_pos((int)n, src, h->m.inv, h->d_x3, h->stream);
     if (e == hipSuccess) e = d2h_xyz(h, dst);
     if (e != hipSuccess) return hip_fail(h, "download", e);
-    return VBD_OK;
+    return check_persistent(h);
 }
 
 // Valence-histogram key of the autotuner cache: free-vertex count bucket (log2), mean
@@ -589,6 +640,7 @@ void vbd_destroy(vbd_solver* h) {
     free_device_mesh(h->m);
     if (h->d_energy) (void)hipFree(h->d_energy);
     if (h->d_elog) (void)hipFree(h->d_elog);
+    if (h->d_bar) (void)hipFree(h->d_bar);
     (void)hipStreamSynchronize(h->ownStream);
     (void)hipStreamDestroy(h->ownStream);
     delete h;
@@ -1126,6 +1178,20 @@ int vbd_synchronize(vbd_solver* h) {
     if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
     DeviceGuard g(h->device);
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return check_persistent(h);
+}
+
+int vbd_set_persistent(vbd_solver* h, int32_t mode) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (mode < 0 || (mode > 1 && (mode & 7))) return fail(h, VBD_ERR_ARG, "persistent mode must be 0, 1 or a multiple of 8");
+    DeviceGuard g(h->device);
+    drop_graph(h);
+    if (mode && !h->d_bar) {
+        HIP_CHECK(h, hipMalloc(&h->d_bar, 3 * sizeof(unsigned)));
+        HIP_CHECK(h, hipMemsetAsync(h->d_bar, 0, 3 * sizeof(unsigned), h->stream));
+        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    }
+    h->persistent = mode;
     return VBD_OK;
 }
 

@@ -79,6 +79,8 @@ struct DevMesh {
     int64_t padBytes = 0;              // bytes of the padded records
     std::vector<int> colorTileBase;    // index of each color's first entry in tileStart
     std::vector<int> colorTileCount;   // tiles per color
+    int* dColorTileBase = nullptr;     // device copies of the two (one allocation, base then count)
+    int* dColorTileCount = nullptr;
     int64_t bytes = 0;
 };
 
@@ -123,11 +125,21 @@ struct TileCut {
     float* tpc = nullptr;
     int64_t padBytes = 0;
     std::vector<int> colorTileBase, colorTileCount;
+    int* dColorTileBase = nullptr;
+    int* dColorTileCount = nullptr;
 };
 void detach_tiles(DevMesh& m, TileCut& out);                  // m is left without a cut
 void attach_tiles(DevMesh& m, TileCut& in, hipStream_t s);     // frees m's cut, moves `in` back
 void free_tile_cut(TileCut& t);
 hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
+// One launch per sweep: every color through the tiled kernel's tile_solve with a grid barrier
+// between colors, then the Chebyshev blend when omega > 1 (pp: x at the start of the previous
+// sweep).  grid: a multiple of 8 not above the resident blocks (persistent_occupancy per CU x
+// CUs); bar: 3 zeroed words; spin: polls before a barrier gives up and raises bar[2].
+int persistent_occupancy(const DevMesh& m, int per_thread, int block, const ContactArgs& ct, bool save_on);
+hipError_t launch_sweep_persistent(const DevMesh& m, int per_thread, int block, int grid, float sdt, float mu,
+                                   float lambda, const ContactArgs& ct, f4* save, const f4* pp, float omega,
+                                   unsigned* bar, unsigned spin, hipStream_t s);
 hipError_t launch_predict(const DevMesh& m, float h, const float g[3], hipStream_t s);
 hipError_t launch_velocity(const DevMesh& m, float h, float damping, hipStream_t s);
 hipError_t launch_copy_inertia(const DevMesh& m, hipStream_t s);

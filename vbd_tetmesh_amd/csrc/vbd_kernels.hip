@@ -548,20 +548,22 @@ __device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, 
     return A;
 }
 
-// COROT: the corotated material of the reference GPU kernel on the full records (pa, pb, pc)
-// instead of the compact stable Neo-Hookean records (pq); same tiles, reduction and order.
+// One tile of one color: records, gathers, contributions, the LDS reduction and the vertex
+// solves (see the block comment above).  `tileStart` points at the color's tile table and the
+// record pointers at the color's first padded record slot.  Uses every thread of the block; the
+// waves past the tile's last vertex group skip the reduction (no early return, so a persistent
+// caller can loop over tiles).
 template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
-__global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, int ntiles, int nslots,
-                                                     f4* __restrict__ pos, const f4* __restrict__ inert,
-                                                     const float* __restrict__ mass, const int* __restrict__ row,
-                                                     const i4* __restrict__ ptet, const f4* __restrict__ pq,
-                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
-                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
-                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
-                                                     ContactArgs ct) {
+__device__ __forceinline__ void tile_solve(const int tile, const int* __restrict__ tileStart, const int nslots,
+                                           f4* __restrict__ pos, const f4* __restrict__ inert,
+                                           const float* __restrict__ mass, const int* __restrict__ row,
+                                           const i4* __restrict__ ptet, const f4* __restrict__ pq,
+                                           const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                           const float* __restrict__ pc, const f2* __restrict__ pmat,
+                                           f4* __restrict__ save, const float mu, const float lambda, const float sdt,
+                                           const ContactArgs& ct, float* red) {
     // LDS: 9 * RT contribution values, then VM = T/VDIV entries (a tile has at most VM - 1
     // vertices): row offsets (nv + 1 of them), x, y, m
-    extern __shared__ float red[];
     const int T = (int)blockDim.x;
     const int RT = K * T;
     const int VM = T / VDIV;
@@ -570,12 +572,8 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
     f4* ly = lx + VM;
     float* lm = reinterpret_cast<float*>(ly + VM);
     const int tid = (int)threadIdx.x;
-
-    const int xcd = (int)(blockIdx.x & 7);
-    const int tile = (int)(((int64_t)ntiles * xcd) >> 3) + (int)(blockIdx.x >> 3);
-    if (tile >= (int)(((int64_t)ntiles * (xcd + 1)) >> 3)) return;  // whole block
     const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, nslots);
-    // the tile's RT padded record slots: their address depends on the block index alone, so the
+    // the tile's RT padded record slots: their address depends on the tile index alone, so the
     // record loads issue first, with nothing to wait for
     const size_t base = (size_t)tile * RT;
     RecSrc src;
@@ -653,39 +651,59 @@ __global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ til
 
     // one group of TG lanes per tile vertex; waves past the tile's last group have nothing to
     // sum (the test is uniform per wave)
-    if ((tid & ~63) >= nv * TG) return;
-    const int lane = tid % TG;
-    const int i = tid / TG;
-    const bool ok = i < nv;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
-    if (ok) {
-        const int q1 = lrow[i + 1];
-        for (int q = lrow[i] + lane; q < q1; q += TG) {
-            const float* rq = red + 9 * q;
-            s0 += rq[0];
-            s1 += rq[1];
-            s2 += rq[2];
-            s3 += rq[3];
-            s4 += rq[4];
-            s5 += rq[5];
-            s6 += rq[6];
-            s7 += rq[7];
-            s8 += rq[8];
+    if ((tid & ~63) < nv * TG) {
+        const int lane = tid % TG;
+        const int i = tid / TG;
+        const bool ok = i < nv;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
+        if (ok) {
+            const int q1 = lrow[i + 1];
+            for (int q = lrow[i] + lane; q < q1; q += TG) {
+                const float* rq = red + 9 * q;
+                s0 += rq[0];
+                s1 += rq[1];
+                s2 += rq[2];
+                s3 += rq[3];
+                s4 += rq[4];
+                s5 += rq[5];
+                s6 += rq[6];
+                s7 += rq[7];
+                s8 += rq[8];
+            }
+        }
+        s0 = gsum<TG>(s0); s1 = gsum<TG>(s1); s2 = gsum<TG>(s2);
+        s3 = gsum<TG>(s3); s4 = gsum<TG>(s4); s5 = gsum<TG>(s5);
+        s6 = gsum<TG>(s6); s7 = gsum<TG>(s7); s8 = gsum<TG>(s8);
+        if (ok && lane == TG - 1) {
+            const f4 x = lx[i];
+            f4 xn;
+            if constexpr (COROT)
+                xn = corot_vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
+            else
+                xn = vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
+            if constexpr (SAVE) save[v0 + i] = x;
+            pos[v0 + i] = xn;
         }
     }
-    s0 = gsum<TG>(s0); s1 = gsum<TG>(s1); s2 = gsum<TG>(s2);
-    s3 = gsum<TG>(s3); s4 = gsum<TG>(s4); s5 = gsum<TG>(s5);
-    s6 = gsum<TG>(s6); s7 = gsum<TG>(s7); s8 = gsum<TG>(s8);
-    if (ok && lane == TG - 1) {
-        const f4 x = lx[i];
-        f4 xn;
-        if constexpr (COROT)
-            xn = corot_vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
-        else
-            xn = vertex_solve<CONTACT>(x, ly[i], lm[i], s0, s1, s2, s3, s4, s5, s6, s7, s8, sdt, ct);
-        if constexpr (SAVE) save[v0 + i] = x;
-        pos[v0 + i] = xn;
-    }
+}
+
+// COROT: the corotated material of the reference GPU kernel on the full records (pa, pb, pc)
+// instead of the compact stable Neo-Hookean records (pq); same tiles, reduction and order.
+template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
+__global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, int ntiles, int nslots,
+                                                     f4* __restrict__ pos, const f4* __restrict__ inert,
+                                                     const float* __restrict__ mass, const int* __restrict__ row,
+                                                     const i4* __restrict__ ptet, const f4* __restrict__ pq,
+                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
+                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
+                                                     f4* __restrict__ save, float mu, float lambda, float sdt,
+                                                     ContactArgs ct) {
+    extern __shared__ float red[];
+    const int xcd = (int)(blockIdx.x & 7);
+    const int tile = (int)(((int64_t)ntiles * xcd) >> 3) + (int)(blockIdx.x >> 3);
+    if (tile >= (int)(((int64_t)ntiles * (xcd + 1)) >> 3)) return;  // whole block
+    tile_solve<K, MAT, CONTACT, SAVE, COROT>(tile, tileStart, nslots, pos, inert, mass, row, ptet, pq, pa, pb, pc, pmat,
+                                             save, mu, lambda, sdt, ct, red);
 }
 
 template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
@@ -704,6 +722,164 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
                        COROT ? m.tpc + base : nullptr, MAT ? m.tpm + base : nullptr, save, mu, lambda, sdt, ct);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------------------
+// Persistent sweep: one launch runs every color of one Gauss-Seidel sweep (and the Chebyshev
+// blend after it) with a grid barrier between the phases, instead of one launch per color.
+// Block b takes the tiles L, L + G, ... of each color (L: the XCD-aware logical index, G the
+// grid), through the same tile_solve as k_color_tiled, so the results are bitwise those of the
+// per-color launches.
+//
+// Grid barrier: bar[0] counts arrivals, bar[1] is the generation, bar[2] the timeout flag.  Every
+// wave first releases its stores to the device (agent scope: its stores complete and the L2 of
+// its XCD is written back; a block barrier alone does not wait for outstanding stores), then the
+// block's first thread reads the generation and arrives; the last arrival resets the count and
+// bumps the generation; the others spin on the generation with s_sleep, at most `spin` polls.
+// After the block barrier every wave acquires (its CU's L1 and its XCD's L2 are invalidated).
+// The count is back to zero after every barrier, so graph replays need no reset.  A spin that
+// runs out (blocks that never became resident) raises bar[2] and the block returns, so every
+// barrier after it times out too and the launch ends; the host checks bar[2].
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, unsigned spin) {
+    __shared__ unsigned ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's positions, device-wide
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned good = 1;
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned n = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++n > spin || __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok = good;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other blocks' positions, in every wave
+    return ok != 0;
+}
+
+template <int K, bool MAT, bool CONTACT, bool SAVE>
+__global__ __launch_bounds__(512) void k_sweep_persistent(int nColors, const int* __restrict__ colorBase,
+                                                          const int* __restrict__ colorTiles,
+                                                          const int* __restrict__ tileStart, int nslots, int nFree,
+                                                          f4* __restrict__ pos, const f4* __restrict__ inert,
+                                                          const float* __restrict__ mass, const int* __restrict__ row,
+                                                          const i4* __restrict__ ptet, const f4* __restrict__ pq,
+                                                          const f2* __restrict__ pmat, f4* __restrict__ save,
+                                                          const f4* __restrict__ pp, float omega, float mu,
+                                                          float lambda, float sdt, ContactArgs ct,
+                                                          unsigned* __restrict__ bar, unsigned spin) {
+    extern __shared__ __attribute__((aligned(16))) float red[];  // after the barrier's word: f4 arrays inside
+    const unsigned G = gridDim.x;
+    const int per = (int)(G >> 3);
+    const int xcd = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
+    const int RT = K * (int)blockDim.x;
+    for (int c = 0; c < nColors; ++c) {
+        const int n = colorTiles[c];
+        const int cb = colorBase[c];
+        // XCD x takes the x-th eighth of the color's tiles, its blocks strided through them
+        const int lo = (int)(((int64_t)n * xcd) >> 3), hi = (int)(((int64_t)n * (xcd + 1)) >> 3);
+        bool first = true;
+        for (int t = lo + j; t < hi; t += per) {
+            if (!first) lds_barrier();  // the previous tile's LDS reads are done
+            first = false;
+            const size_t rb = (size_t)cb * RT;
+            tile_solve<K, MAT, CONTACT, SAVE, false>(t, tileStart + cb, nslots, pos, inert, mass, row, ptet + rb,
+                                                     pq + rb, nullptr, nullptr, nullptr, MAT ? pmat + rb : nullptr,
+                                                     save, mu, lambda, sdt, ct, red);
+        }
+        if (c + 1 < nColors || omega > 1.0f)
+            if (!grid_barrier(bar, G, spin)) return;
+    }
+    if (omega > 1.0f) {  // the Chebyshev blend (VBD.cpp:290-300), rounded as k_blend rounds it
+#pragma clang fp contract(off)
+        for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < nFree; i += (int)(G * blockDim.x)) {
+            const f4 x = pos[i];
+            const f4 q = pp[i];
+            f4 r = x;
+            r.x = omega * (x.x - q.x) + q.x;
+            r.y = omega * (x.y - q.y) + q.y;
+            r.z = omega * (x.z - q.z) + q.z;
+            pos[i] = r;
+        }
+    }
+}
+
+template <int K, bool MAT, bool CONTACT, bool SAVE>
+static hipError_t persistent_t(const DevMesh& m, int block, int grid, float sdt, float mu, float lambda,
+                               const ContactArgs& ct, f4* save, const f4* pp, float omega, unsigned* bar,
+                               unsigned spin, hipStream_t s) {
+    const size_t lds = sizeof(float) * 9 * (size_t)K * block +
+                       (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / VDIV);
+    hipLaunchKernelGGL((k_sweep_persistent<K, MAT, CONTACT, SAVE>), dim3(grid), dim3(block), lds, s, m.nColors,
+                       m.dColorTileBase, m.dColorTileCount, m.tileStart, m.nV, m.nFree, m.pos, m.inert, m.mass, m.row,
+                       m.tpt, m.tpq, m.tpm, save, pp, omega, mu, lambda, sdt, ct, bar, spin);
+    return hipGetLastError();
+}
+
+// Resident blocks of the persistent sweep per CU (occupancy) for (K, block), 0 if unsupported.
+template <int K, bool MAT, bool CONTACT, bool SAVE>
+static int persistent_occupancy_t(int block) {
+    const size_t lds = sizeof(float) * 9 * (size_t)K * block +
+                       (sizeof(int) + 2 * sizeof(f4) + sizeof(float)) * (size_t)(block / VDIV);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&k_sweep_persistent<K, MAT, CONTACT, SAVE>),
+                                                     block, lds) != hipSuccess)
+        return 0;
+    return n;
+}
+
+#define VBD_PERSIST_DISPATCH(FN, ...)                                                             \
+    do {                                                                                         \
+        const bool mat_ = m.perTetMat, con_ = ct.on != 0, sv_ = save != nullptr;                 \
+        if (per_thread == 1) {                                                                   \
+            if (!mat_ && !con_ && !sv_) return FN<1, false, false, false>(__VA_ARGS__);          \
+            if (!mat_ && !con_ && sv_) return FN<1, false, false, true>(__VA_ARGS__);            \
+            if (!mat_ && con_ && !sv_) return FN<1, false, true, false>(__VA_ARGS__);            \
+            if (!mat_ && con_ && sv_) return FN<1, false, true, true>(__VA_ARGS__);              \
+            if (mat_ && !con_ && !sv_) return FN<1, true, false, false>(__VA_ARGS__);            \
+            if (mat_ && !con_ && sv_) return FN<1, true, false, true>(__VA_ARGS__);              \
+            if (mat_ && con_ && !sv_) return FN<1, true, true, false>(__VA_ARGS__);              \
+            return FN<1, true, true, true>(__VA_ARGS__);                                         \
+        }                                                                                        \
+        if (!mat_ && !con_ && !sv_) return FN<2, false, false, false>(__VA_ARGS__);              \
+        if (!mat_ && !con_ && sv_) return FN<2, false, false, true>(__VA_ARGS__);                \
+        if (!mat_ && con_ && !sv_) return FN<2, false, true, false>(__VA_ARGS__);                \
+        if (!mat_ && con_ && sv_) return FN<2, false, true, true>(__VA_ARGS__);                  \
+        if (mat_ && !con_ && !sv_) return FN<2, true, false, false>(__VA_ARGS__);                \
+        if (mat_ && !con_ && sv_) return FN<2, true, false, true>(__VA_ARGS__);                  \
+        if (mat_ && con_ && !sv_) return FN<2, true, true, false>(__VA_ARGS__);                  \
+        return FN<2, true, true, true>(__VA_ARGS__);                                             \
+    } while (0)
+
+int persistent_occupancy(const DevMesh& m, int per_thread, int block, const ContactArgs& ct, bool save_on) {
+    if (!tiled_config_supported(per_thread, block)) return 0;
+    const f4* save = save_on ? reinterpret_cast<const f4*>(1) : nullptr;
+    VBD_PERSIST_DISPATCH(persistent_occupancy_t, block);
+}
+
+hipError_t launch_sweep_persistent(const DevMesh& m, int per_thread, int block, int grid, float sdt, float mu,
+                                   float lambda, const ContactArgs& ct, f4* save, const f4* pp, float omega,
+                                   unsigned* bar, unsigned spin, hipStream_t s) {
+    if (!tiled_config_supported(per_thread, block) || m.tileRecords != per_thread * block ||
+        m.tileVerts != block / VDIV - 1 || !m.tileStart || !m.tpt || !m.dColorTileBase || grid < 8 || (grid & 7) ||
+        !bar || (omega > 1.0f && !pp))
+        return hipErrorInvalidValue;
+    VBD_PERSIST_DISPATCH(persistent_t, m, block, grid, sdt, mu, lambda, ct, save, pp, omega, bar, spin, s);
+}
+#undef VBD_PERSIST_DISPATCH
 
 template <int K, bool COROT>
 static hipError_t launch_tiled_k(const DevMesh& m, int color, int block, float sdt, float mu, float lambda,
@@ -927,6 +1103,20 @@ hipError_t build_tiles(DevMesh& m, int records, int max_verts, hipStream_t s, st
     }
     m.colorTileBase = base;
     m.colorTileCount = count;
+    if (m.dColorTileBase) (void)hipFree(m.dColorTileBase);
+    m.dColorTileBase = nullptr;
+    if (m.nColors > 0) {
+        e = hipMalloc(&m.dColorTileBase, sizeof(int) * 2 * (size_t)m.nColors);
+        if (e == hipSuccess) e = hipMemcpyAsync(m.dColorTileBase, base.data(), sizeof(int) * m.nColors, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(m.dColorTileBase + m.nColors, count.data(), sizeof(int) * m.nColors, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            err = std::string("tile tables: ") + hipGetErrorString(e);
+            return e;
+        }
+        m.dColorTileCount = m.dColorTileBase + m.nColors;
+    }
     m.tileRecords = records;
     m.tileVerts = max_verts;
     m.tileEntries = (int)starts.size();
@@ -957,6 +1147,9 @@ void detach_tiles(DevMesh& m, TileCut& out) {
     out.padBytes = m.padBytes;
     out.colorTileBase.swap(m.colorTileBase);
     out.colorTileCount.swap(m.colorTileCount);
+    out.dColorTileBase = m.dColorTileBase;
+    out.dColorTileCount = m.dColorTileCount;
+    m.dColorTileBase = m.dColorTileCount = nullptr;
     m.bytes -= m.padBytes;
     m.tileStart = m.tileRec = nullptr;
     m.tileRecords = m.tileVerts = m.tileEntries = 0;
@@ -971,7 +1164,7 @@ void detach_tiles(DevMesh& m, TileCut& out) {
 }
 
 void free_tile_cut(TileCut& t) {
-    void* ps[] = {t.tileStart, t.tileRec, t.tpt, t.tpq, t.tpm, t.tpa, t.tpb, t.tpc};
+    void* ps[] = {t.tileStart, t.tileRec, t.tpt, t.tpq, t.tpm, t.tpa, t.tpb, t.tpc, t.dColorTileBase};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     t = TileCut();
@@ -997,6 +1190,8 @@ void attach_tiles(DevMesh& m, TileCut& in, hipStream_t s) {
     m.bytes += in.padBytes;
     m.colorTileBase.swap(in.colorTileBase);
     m.colorTileCount.swap(in.colorTileCount);
+    m.dColorTileBase = in.dColorTileBase;
+    m.dColorTileCount = in.dColorTileCount;
     in = TileCut();
 }
 
@@ -2396,7 +2591,7 @@ void free_device_mesh(DevMesh& m) {
     void* ptrs[] = {m.pos, m.inert, m.vel, m.xprev, m.save[0], m.save[1], m.mass, m.row, m.perm, m.inv,
                     m.ptet, m.pa, m.pb, m.pc, m.pmat, m.adjOff, m.adjEntry, m.colorList, m.tets, m.dminv,
                     m.vol, m.tmu, m.tlam, m.chunkBase, m.tileStart, m.tileRec, m.pq,
-                    m.tpt, m.tpq, m.tpm, m.tpa, m.tpb, m.tpc};
+                    m.tpt, m.tpq, m.tpm, m.tpa, m.tpb, m.tpc, m.dColorTileBase};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     m = DevMesh();

@@ -372,6 +372,11 @@ class DeviceMesh(_Handle):
         raw = getattr(stream, "cuda_stream", stream)
         check(lib.vbd_set_stream(self.h, C.c_void_p(raw) if raw else None), self.h)
 
+    def set_persistent(self, mode: int):
+        """1: one persistent launch per sweep (colors + blend, grid barriers) on the resident grid;
+        8k: the same on at most 8k blocks; 0: one launch per color."""
+        check(lib.vbd_set_persistent(self.h, int(mode)), self.h)
+
     def set_graph(self, enabled: bool):
         check(lib.vbd_set_graph_enabled(self.h, int(bool(enabled))), self.h)
 
