@@ -193,6 +193,7 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             d.synchronize()
         return (time.perf_counter() - t0) / baseline_frames
 
+    pab = persistent_ab(b, np.concatenate([d["positions"] for d in descs]), frames)
     tp = per_mesh(False)
     tg = per_mesh(True)
     vi = float(nv) * iterations
@@ -202,8 +203,81 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             "frames": frames,
             "per_mesh_plain_ms_per_step": round(tp * 1e3, 3), "per_mesh_graph_ms_per_step": round(tg * 1e3, 3),
             "speedup_vs_per_mesh_plain": round(tp / tb, 2), "speedup_vs_per_mesh_graph": round(tg / tb, 2),
-            "baseline_frames": baseline_frames,
+            "baseline_frames": baseline_frames, "persistent_ab": pab,
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
+
+
+def persistent_ab(d, x0, frames: int, modes=(1,)):
+    """The same frames from x0 with one persistent launch per sweep (vbd_set_persistent: colors
+    and blend in one kernel, a device-wide barrier between colors; mode 1 the library's grid, 8k
+    at most 8k blocks) against the default per-color launches, all replayed from the step's
+    hipGraph: ms per frame each and whether the final positions are bitwise equal.  The handle is
+    left in the default mode."""
+    zero = np.zeros_like(x0)
+
+    def run(mode):
+        try:
+            d.set_persistent(mode)
+            d.set_positions(x0)
+            d.set_velocities(zero)
+            d.step(1)  # captures the step's graph
+            d.synchronize()
+            d.set_positions(x0)
+            d.set_velocities(zero)
+            d.synchronize()
+            t0 = time.perf_counter()
+            d.step(frames)
+            d.synchronize()
+            return (time.perf_counter() - t0) / frames, d.positions(), d.info().persistent_grid
+        except Exception as e:  # a barrier that timed out is reported, not fatal to the bench
+            return None, None, str(e)
+
+    t0_, x0_, _ = run(0)
+    res = []
+    for mode in modes:
+        t1_, x1_, g1 = run(mode)
+        res.append({"mode": mode, "persistent_grid": g1,
+                    "persistent_ms_per_step": round(t1_ * 1e3, 4) if t1_ else None,
+                    "persistent_over_per_color": round(t1_ / t0_, 3) if (t0_ and t1_) else None,
+                    "bitwise_equal": bool(x1_ is not None and x0_ is not None and np.array_equal(x0_, x1_))})
+    d.set_persistent(0)
+    out = {"frames": frames, "per_color_ms_per_step": round(t0_ * 1e3, 4) if t0_ else None,
+           "kept": "per-color launches (default)"}
+    if len(res) == 1:
+        out.update({k: v for k, v in res[0].items() if k != "mode"})
+    else:
+        out["modes"] = res
+    return out
+
+
+def bench_persistent(caps=(256, 512, 768, 1)):
+    """Persistent-sweep grid sweep on the three workloads at their default launch configuration:
+    the cube128 step (20 iterations), the sphere96 step (30 iterations, contact plane z = -1.2)
+    and the 256-beam batch (20 iterations), each against its per-color launches."""
+    from vbd_tetmesh_amd import VBDParams, meshgen
+    from vbd_tetmesh_amd.solver import DeviceMesh, prepare_host, upload
+    out = {}
+    m = meshgen.noisy_cube(128, seed=0)
+    d = upload(m, params=VBDParams(dt=H, numIterations=20, gravity=m.gravity, useAcceleration=True,
+                                   accelerationRho=0.95, mu=m.mu, lambda_=m.lam))
+    out["cube128"] = persistent_ab(d, m.x0, 5, caps)
+    del d
+    m = meshgen.voxel_sphere(96, seed=0)
+    d = upload(m, params=VBDParams(dt=H, numIterations=30, gravity=m.gravity, useAcceleration=True,
+                                   accelerationRho=0.95, mu=m.mu, lambda_=m.lam, contact=(2, -1.2, 1e4)))
+    out["sphere96"] = persistent_ab(d, m.x0, 10, caps)
+    del d
+    descs = []
+    for mm in meshgen.batch_beams(256, seed=0):
+        t, off, verts, dminv, vol = prepare_host(mm)
+        descs.append(dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                          rest_volume=vol, color_off=off, color_verts=verts, tet_mu=t.tetMu, tet_lambda=t.tetLambda))
+    b = DeviceMesh().init_batch(descs)
+    b.set_params(VBDParams(dt=H, numIterations=20, gravity=(0.0, -9.8, 0.0), useAcceleration=True,
+                           accelerationRho=0.95))
+    b.set_launch_config(0, 256, 1)
+    out["batch256"] = persistent_ab(b, np.concatenate([x["positions"] for x in descs]), 20, caps)
+    return out
 
 
 def launch_roofline(d, sweeps: int, traffic_name=None):
@@ -287,6 +361,7 @@ def bench_sphere(n: int = 96, frames: int = 30, iterations: int = 30, slot_order
     free_ms = frame_ms[:first_contact] if first_contact is not None else frame_ms
     contact_ms = frame_ms[first_contact:] if first_contact is not None else []
     roof = launch_roofline(d, iterations, "traffic_color_tiled_sphere96") if roofline else None
+    pab = persistent_ab(d, m.x0, min(frames, 10)) if roofline else None
     vi = float(info.num_verts) * iterations * frames
     per_frame_vi = float(info.num_verts) * iterations
     return {"workload": f"sphere {n}^3 voxels, {info.num_verts} verts, {info.num_tets} tets, {info.num_colors} colors, "
@@ -298,7 +373,7 @@ def bench_sphere(n: int = 96, frames: int = 30, iterations: int = 30, slot_order
             "frames_without_contact": len(free_ms), "frames_with_contact": len(contact_ms),
             "value_without_contact": round(per_frame_vi / (np.mean(free_ms) / 1e3) / 1e6, 2) if free_ms else None,
             "value_with_contact": round(per_frame_vi / (np.mean(contact_ms) / 1e3) / 1e6, 2) if contact_ms else None,
-            "roofline": roof,
+            "roofline": roof, "persistent_ab": pab,
             "launch_config": {"lanes_per_vertex": info.lanes_per_vertex, "pairs_per_lane": info.pairs_per_lane,
                               "block_size": info.block_size}}
 
@@ -464,7 +539,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-corotated", action="store_true", help="skip the corotated-material run of the cube")
     ap.add_argument("--no-hybrid", action="store_true", help="skip the host-buffer (PCIe-inclusive) frame loop")
-    ap.add_argument("--only", choices=["batched", "sphere", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["batched", "sphere", "adjoint", "persistent"], default=None,
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.copy_probe:
@@ -484,7 +559,7 @@ def main():
 
     if args.only:
         fn = {"batched": bench_batch, "sphere": lambda: bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order),
-              "adjoint": bench_adjoint}[args.only]
+              "adjoint": bench_adjoint, "persistent": bench_persistent}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
 
@@ -554,6 +629,10 @@ def main():
                           "note": "whole timed step: layout bytes of the color launches and blends per "
                                   "vertex-iteration x the measured rate"},
     }
+    out["persistent_ab"] = persistent_ab(d, m.x0, args.steps)
+    pa = out["persistent_ab"]
+    log(f"[bench] persistent sweep: {pa['persistent_ms_per_step']} ms/step on {pa['persistent_grid']} blocks vs "
+        f"per-color {pa['per_color_ms_per_step']} ms/step, bitwise {pa['bitwise_equal']}")
     out["energy"] = bench_energy(d)
     log(f"[bench] energies: " + ", ".join(f"{k} {v['ms_per_eval']:.3f} ms ({v['frac']})" for k, v in out["energy"].items()))
     if not args.no_corotated:

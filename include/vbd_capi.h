@@ -116,6 +116,8 @@ typedef struct vbd_info {
     int32_t full_records;       /* 1 while the 52-byte full pair records are resident (built on first
                                    use by the corotated material, the per-vertex kernels and the
                                    differentiation entry points) */
+    int32_t persistent_grid;    /* blocks of one persistent sweep launch for the step's parameters
+                                   (0: one launch per color; see vbd_set_persistent) */
 } vbd_info;
 
 /* ---- lifecycle -------------------------------------------------------------------- */
@@ -180,12 +182,14 @@ int vbd_set_stream(vbd_solver* h, void* stream);
 /* Enables (1, default) or disables (0) hipGraph capture of vbd_step. */
 int vbd_set_graph_enabled(vbd_solver* h, int32_t enabled);
 /* Sweep launches (no reference counterpart).  0 (default): one launch per color and one per
- * Chebyshev blend.  1: one persistent launch per sweep -- every color through the tiled kernel's
- * tile code with a device-wide barrier between colors, then the blend -- with the grid sized to
- * the resident blocks; bitwise the same positions.  8k (k >= 1): the same with at most 8k blocks,
- * each looping over more tiles.  Applies to the tiled configuration of the
- * stable Neo-Hookean material (others launch per color).  A barrier whose blocks are not all
- * resident gives up after a bounded wait; the next vbd_synchronize or download then fails with
+ * Chebyshev blend, replayed from the step's hipGraph.  1: one persistent launch per sweep --
+ * every color through the tiled kernel's tile code with a device-wide barrier between colors,
+ * then the blend -- on 2 blocks per CU; bitwise the same positions, and slower on MI355X (a
+ * cross-XCD barrier costs more than the kernel boundary it replaces: 1.7x to 2.6x the per-color
+ * graph's time on the measured workloads).  8k (k >= 1): the same on at most 8k blocks (and at
+ * most the occupancy API's count).  Applies to the tiled configuration of the stable
+ * Neo-Hookean material (others launch per color).  A barrier whose blocks are not all resident
+ * gives up after a bounded wait; the next vbd_synchronize or download then fails with
  * VBD_ERR_HIP. */
 int vbd_set_persistent(vbd_solver* h, int32_t mode);
 

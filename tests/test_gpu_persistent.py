@@ -167,3 +167,20 @@ def test_set_persistent_arguments_and_other_configs():
             e.step(2)
             res.append(e.positions())
         assert np.array_equal(res[0], res[1])
+
+
+def test_auto_grid_on_a_chip_filling_workload_bitwise():
+    """Noisy cube 56^3 (185,193 vertices) at the library's default tiled configuration: more
+    tiles per color than the automatic grid (2 blocks per CU), so mode 1 runs on that grid."""
+    m = meshgen.noisy_cube(56, seed=0)
+    p = params(iterations=10, gravity=m.gravity, mu=m.mu, lam=m.lam)
+    d = upload(m, params=p)
+    i = d.info()
+    cfg = (i.lanes_per_vertex, i.block_size, i.pairs_per_lane)
+    print(f"[persistent auto] cfg={cfg} nV={m.num_verts}", flush=True)
+    d.set_persistent(1)
+    grid = d.info().persistent_grid
+    assert grid >= 8 and grid % 8 == 0
+    ref = frames_of(lambda: upload(m, params=p), cfg, 0, frames=2)
+    got = frames_of(lambda: upload(m, params=p), cfg, 1, frames=2)
+    assert_frames_equal(got, ref)

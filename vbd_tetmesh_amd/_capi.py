@@ -86,6 +86,7 @@ class Info(C.Structure):
         ("device_bytes", C.c_int64),
         ("cached_graphs", C.c_int32),
         ("full_records", C.c_int32),
+        ("persistent_grid", C.c_int32),
     ]
 
 

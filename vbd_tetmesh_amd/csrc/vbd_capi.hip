@@ -65,7 +65,7 @@ struct vbd_solver {
     // the current state (the FP32 arrays are its rounding); cleared by every entry point that
     // changes the FP32 state, after which the next FP64 call promotes the FP32 state
     int persistent = 0;          // 1 or a grid cap (8k): one persistent launch per sweep (tiled stable Neo-Hookean)
-    unsigned* d_bar = nullptr;   // its grid barrier: count, generation, timeout flag
+    unsigned* d_bar = nullptr;   // its grid barrier (PERSIST_BAR_WORDS words: counts, generation, timeout flag)
     int diffBits = 32;
     Dev64 d64;
     bool fp64Fresh = false;
@@ -297,26 +297,28 @@ hipError_t enqueue_sweep(vbd_solver* h, float sdt, float mu, float lambda, const
     return hipSuccess;
 }
 
-// Polls of a persistent sweep's grid barrier before it gives up (each poll sleeps about 50 ns,
-// and an atomic read crosses the device): a few milliseconds at the least, far above any
-// legitimate wait (a color's slowest block), and a bound on a launch whose blocks are not all
-// resident.
-constexpr unsigned PERSIST_SPIN = 200000;
+// Polls of a persistent sweep's grid barrier before it gives up (each poll sleeps about 50 ns
+// and reads across the device): half a second at the least, far above any legitimate wait (a
+// color's slowest block), and a bound on a launch whose blocks are not all resident.
+constexpr unsigned PERSIST_SPIN = 1u << 23;
 
 bool persistent_on(const vbd_solver* h) {
     return h->persistent && h->lpv == 0 && h->materialModel == VBD_MATERIAL_STABLE_NEO_HOOKEAN && h->d_bar;
 }
 
-// Grid of the persistent sweep: the resident blocks (occupancy x CUs), at most the largest
-// color's tile count and the caller's cap, a multiple of 8; 0 when not even 8 blocks fit.
-int persistent_grid(vbd_solver* h, const ContactArgs& ct, bool save_on) {
+// Grid of the persistent sweep, a multiple of 8 and at most the largest color's tile count:
+// mode 1 takes 2 blocks per CU (the fastest grid measured on the sphere, the batch and the cube;
+// a grid of the occupancy API's full count, 4 blocks of 512 threads per CU for this kernel, was
+// seen not to become co-resident on MI355X, its barrier timing out, where 3 per CU was); a cap
+// 8k takes at most 8k blocks and at most the occupancy API's count.  0 when not even 8 fit.
+int persistent_grid(const vbd_solver* h, const ContactArgs& ct, bool save_on) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) return 0;
     const int occ = persistent_occupancy(h->m, h->unroll, h->block, ct, save_on);
     int tiles = 0;
     for (int c : h->m.colorTileCount) tiles = std::max(tiles, c);
     int g = std::min(occ * cus, (tiles + 7) & ~7);
-    if (h->persistent >= 8) g = std::min(g, h->persistent);
+    g = std::min(g, h->persistent >= 8 ? h->persistent : 2 * cus);
     g &= ~7;
     return g >= 8 ? g : 0;
 }
@@ -475,10 +477,10 @@ int upload_vertex_array(vbd_solver* h, const float* src, f4* dst) {
 int check_persistent(vbd_solver* h) {
     if (!h->d_bar) return VBD_OK;
     unsigned flag = 0;
-    HIP_CHECK(h, hipMemcpyAsync(&flag, h->d_bar + 2, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipMemcpyAsync(&flag, h->d_bar + PERSIST_FLAG, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
     if (!flag) return VBD_OK;
-    HIP_CHECK(h, hipMemsetAsync(h->d_bar, 0, 3 * sizeof(unsigned), h->stream));
+    HIP_CHECK(h, hipMemsetAsync(h->d_bar, 0, PERSIST_BAR_WORDS * sizeof(unsigned), h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
     return fail(h, VBD_ERR_HIP, "persistent sweep: a grid barrier timed out (blocks not co-resident); the positions "
                                 "are not valid");
@@ -944,6 +946,10 @@ int vbd_get_info(const vbd_solver* h, vbd_info* info) {
     info->device_bytes = h->m.bytes;
     info->cached_graphs = (int32_t)h->blockGraphs.size() + (h->graphExec ? 1 : 0);
     info->full_records = h->m.pa != nullptr;
+    if (persistent_on(h) && h->paramsSet) {
+        DeviceGuard g(h->device);
+        info->persistent_grid = persistent_grid(h, contact_of(h->params), h->params.use_acceleration != 0);
+    }
     return VBD_OK;
 }
 
@@ -1187,8 +1193,8 @@ int vbd_set_persistent(vbd_solver* h, int32_t mode) {
     DeviceGuard g(h->device);
     drop_graph(h);
     if (mode && !h->d_bar) {
-        HIP_CHECK(h, hipMalloc(&h->d_bar, 3 * sizeof(unsigned)));
-        HIP_CHECK(h, hipMemsetAsync(h->d_bar, 0, 3 * sizeof(unsigned), h->stream));
+        HIP_CHECK(h, hipMalloc(&h->d_bar, PERSIST_BAR_WORDS * sizeof(unsigned)));
+        HIP_CHECK(h, hipMemsetAsync(h->d_bar, 0, PERSIST_BAR_WORDS * sizeof(unsigned), h->stream));
         HIP_CHECK(h, hipStreamSynchronize(h->stream));
     }
     h->persistent = mode;

@@ -135,7 +135,11 @@ hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t
 // One launch per sweep: every color through the tiled kernel's tile_solve with a grid barrier
 // between colors, then the Chebyshev blend when omega > 1 (pp: x at the start of the previous
 // sweep).  grid: a multiple of 8 not above the resident blocks (persistent_occupancy per CU x
-// CUs); bar: 3 zeroed words; spin: polls before a barrier gives up and raises bar[2].
+// CUs); bar: PERSIST_BAR_WORDS zeroed words; spin: polls before a barrier gives up and raises
+// bar[PERSIST_FLAG].
+constexpr int PERSIST_LINE = 32;                    // words per 128-byte line
+constexpr int PERSIST_BAR_WORDS = 11 * PERSIST_LINE;  // global count, generation, flag, 8 group counts
+constexpr int PERSIST_FLAG = 2 * PERSIST_LINE;
 int persistent_occupancy(const DevMesh& m, int per_thread, int block, const ContactArgs& ct, bool save_on);
 hipError_t launch_sweep_persistent(const DevMesh& m, int per_thread, int block, int grid, float sdt, float mu,
                                    float lambda, const ContactArgs& ct, f4* save, const f4* pp, float omega,

@@ -730,43 +730,66 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
 // grid), through the same tile_solve as k_color_tiled, so the results are bitwise those of the
 // per-color launches.
 //
-// Grid barrier: bar[0] counts arrivals, bar[1] is the generation, bar[2] the timeout flag.  Every
-// wave first releases its stores to the device (agent scope: its stores complete and the L2 of
-// its XCD is written back; a block barrier alone does not wait for outstanding stores), then the
-// block's first thread reads the generation and arrives; the last arrival resets the count and
-// bumps the generation; the others spin on the generation with s_sleep, at most `spin` polls.
-// After the block barrier every wave acquires (its CU's L1 and its XCD's L2 are invalidated).
-// The count is back to zero after every barrier, so graph replays need no reset.  A spin that
-// runs out (blocks that never became resident) raises bar[2] and the block returns, so every
-// barrier after it times out too and the launch ends; the host checks bar[2].
+// Grid barrier, hierarchical over the 8 XCDs (each with its own L2): blocks b with the same
+// b & 7 form a group (the hardware dispatches block b to XCD b mod 8).  Words of `bar`, each on
+// its own 128-byte line (PERSIST_LINE words apart): 0 the group arrivals, 1 the generation,
+// 2 the timeout flag, 3 + x the arrivals of group x.
+//  * every wave waits for its own stores (vmcnt 0): they are in its XCD's L2;
+//  * the block's first thread reads the generation and arrives at its group; a block running on
+//    another XCD than its group's (read from the XCC_ID register) writes its own L2 back first;
+//  * the last arrival of a group resets the group count, writes its XCD's L2 back (agent-scope
+//    release: one write-back per XCD instead of one per wave) and arrives globally; the last of
+//    the 8 resets the global count and bumps the generation; every other block spins on the
+//    generation with s_sleep, at most `spin` polls;
+//  * then the first thread acquires (its CU's L1 and its XCD's L2 are invalidated) and waits for
+//    the invalidation before the block barrier.
+// The counts are back to zero after every barrier, so graph replays need no reset.  A spin that
+// runs out raises the flag and the block returns; every barrier after it then times out too and
+// the launch ends; the host checks the flag.
 // ---------------------------------------------------------------------------------------
+#define VBD_WAIT_VMEM() __builtin_amdgcn_s_waitcnt(0x0F70)  // s_waitcnt vmcnt(0) (gfx9 encoding)
+
 __device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, unsigned spin) {
     __shared__ unsigned ok;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's positions, device-wide
+    VBD_WAIT_VMEM();  // this wave's stores have reached the L2
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned* const cnt = bar;
+        unsigned* const gen = bar + PERSIST_LINE;
+        unsigned* const flag = bar + 2 * PERSIST_LINE;
+        const unsigned x = blockIdx.x & 7;
+        unsigned* const xcnt = bar + (3 + x) * PERSIST_LINE;
+        const unsigned g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        VBD_WAIT_VMEM();  // the generation is read before this block arrives
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xf;  // HW_REG_XCC_ID
+        if (xcc != x) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // not in the group's L2
+        bool waiter = true;
+        if (__hip_atomic_fetch_add(xcnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (nblocks >> 3) - 1) {
+            __hip_atomic_store(xcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the group's positions, device-wide
+            if (__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 7u) {
+                __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                waiter = false;
+            }
+        }
         unsigned good = 1;
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        if (waiter) {
             unsigned n = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (++n > spin || __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                if (++n > spin || __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     good = 0;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other XCDs' positions
+        VBD_WAIT_VMEM();  // the invalidation is done before the block's other waves load
         ok = good;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other blocks' positions, in every wave
     return ok != 0;
 }
 
