@@ -442,7 +442,7 @@ def bench_hybrid(m, iterations: int, frames: int = 5):
     g.initFromCPU(t, (off, verts), dminv, vol)
     sdt = float(np.float32(1.0 / np.float64(np.float32(H) * np.float32(H))))
     dev = host = xfer = 0.0
-    for f in range(frames + 1):
+    for f in range(frames + 2):  # solveVBD's first call runs eagerly, the second captures its graph
         h0 = time.perf_counter()
         t.savePreviousPosition()
         t.applyGravity(m.gravity, H)
@@ -454,7 +454,7 @@ def bench_hybrid(m, iterations: int, frames: int = 5):
         h2 = time.perf_counter()
         t.updateVelocity(H)
         h3 = time.perf_counter()
-        if f > 0:
+        if f > 1:
             dev += h2 - h1
             host += (h1 - h0) + (h3 - h2)
     # the transfers alone: one inertia upload and one position download per frame
@@ -471,6 +471,48 @@ def bench_hybrid(m, iterations: int, frames: int = 5):
             "unit": "M vertex-iters/s", "frames": frames,
             "device_ms_per_frame": round(dev * 1e3 / frames, 3), "host_ms_per_frame": round(host * 1e3 / frames, 3),
             "transfer_ms_per_frame": round(xfer * 1e3 / frames, 3), "pcie_bytes_per_frame": 24 * m.num_verts}
+
+
+def bench_hybrid_cpp(m, iterations: int, frames: int = 5):
+    """The reference main.cpp:215-232 frame loop as a reference user compiles it: the C++ drop-in
+    (include/vbd/*.h; vbd_dropin_demo's hybrid mode) with host forwardStep and updateVelocity in
+    C++ (serial loops, as the reference's) and updateInertiaFromCPU, solveVBD (x <- y, Chebyshev
+    0.95) and copyPositionsToCPU on the device.  The same cube is written as TetGen files (rest
+    shape; the loader pins y <= min + 0.1% of the extent, the bottom face; gravity -y, h = 1/60)
+    and loaded by the drop-in's loader.  Reports the host and device parts per frame over the
+    frames after the first two (solveVBD's first call runs eagerly, the second captures its
+    graph) and the drop-in's set-up times."""
+    import shutil
+    import subprocess
+    import tempfile
+    from vbd_tetmesh_amd import meshgen
+    demo = os.path.join(ROOT, "vbd_tetmesh_amd", "vbd_dropin_demo")
+    if not os.path.exists(demo):
+        return {"error": "vbd_dropin_demo is not built"}
+    tmp = tempfile.mkdtemp(prefix="vbd_cube_")
+    try:
+        t0 = time.perf_counter()
+        meshgen.write_tetgen(os.path.join(tmp, "cube"), m.rest, m.tets)
+        write_s = time.perf_counter() - t0
+        cmd = [demo, os.path.join(tmp, "cube"), "hybrid", str(frames + 2), str(iterations), "0.95", "1", "0.001",
+               repr(float(m.mu)), repr(float(m.lam)), os.devnull]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("timing ")]
+    if r.returncode != 0 or not lines:
+        return {"error": (r.stderr or r.stdout)[-400:]}
+    tok = lines[-1].split()
+    kv = {k: float(v) for k, v in zip(tok[1::2], tok[2::2])}
+    dev, host = kv["device_ms_per_frame"], kv["host_ms_per_frame"]
+    vi = float(m.num_verts) * iterations
+    return {"workload": "same cube (rest shape, TetGen files), reference main.cpp frame loop through the C++ drop-in",
+            "device_value": round(vi / (dev / 1e3) / 1e6, 2), "loop_value": round(vi / ((dev + host) / 1e3) / 1e6, 2),
+            "unit": "M vertex-iters/s", "frames": int(kv["frames"]),
+            "device_ms_per_frame": round(dev, 3), "host_ms_per_frame": round(host, 3),
+            "setup_s": {"tetgen_load": kv["load_s"], "coloring": kv["coloring_s"], "tet_data": kv["tet_data_s"],
+                        "init_from_cpu": kv["init_from_cpu_s"]},
+            "tetgen_write_s": round(write_s, 2), "pcie_bytes_per_frame": 24 * m.num_verts}
 
 
 def bench_adjoint(n: int = 16, steps: int = 30):
@@ -647,6 +689,14 @@ def main():
         log(f"[bench] hybrid loop: device part {hy['device_value']:.1f} M vertex-iters/s "
             f"({hy['device_ms_per_frame']:.2f} ms/frame, transfers {hy['transfer_ms_per_frame']:.2f} ms), whole loop "
             f"{hy['loop_value']:.1f}")
+        hc = bench_hybrid_cpp(m, args.iterations)
+        out["hybrid_cpp"] = hc
+        if "error" in hc:
+            log(f"[bench] C++ drop-in loop: {hc['error']}")
+        else:
+            log(f"[bench] C++ drop-in loop: device part {hc['device_ms_per_frame']:.2f} ms/frame "
+                f"({hc['device_value']:.1f} M vertex-iters/s), host part {hc['host_ms_per_frame']:.2f} ms, whole loop "
+                f"{hc['loop_value']:.1f}; set-up {hc['setup_s']}")
     if not args.no_batch:
         d = None
         bb = bench_batch()

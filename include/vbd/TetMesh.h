@@ -16,6 +16,15 @@
 #include "../vbd_capi.h"
 #include "Types.h"
 
+// The per-vertex time-step helpers run their loops on OpenMP threads when the including
+// translation unit is compiled with OpenMP (each element's arithmetic is the same either way,
+// so the results are bitwise those of one thread); without it they are plain loops.
+#if defined(_OPENMP)
+#define VBD_PER_VERTEX_FOR _Pragma("omp parallel for schedule(static) if (numVerts >= 65536)")
+#else
+#define VBD_PER_VERTEX_FOR
+#endif
+
 namespace VBD {
 
 struct TetMesh {
@@ -94,6 +103,7 @@ struct TetMesh {
 
     // TetMesh.h:305-311
     void updateVelocity(FloatingType dt) {
+        VBD_PER_VERTEX_FOR
         for (size_t i = 0; i < numVerts; ++i)
             if (!isFixed[i]) {
                 auto v = mVelocity.col(i);
@@ -103,19 +113,32 @@ struct TetMesh {
 
     // TetMesh.h:319-325
     void applyGravity(const Vec3& gravity, FloatingType dt) {
+        const Vec3 dv = dt * gravity;
+        VBD_PER_VERTEX_FOR
         for (size_t i = 0; i < numVerts; ++i)
-            if (!isFixed[i]) mVelocity.col(i) += dt * gravity;
+            if (!isFixed[i]) mVelocity.col(i) += dv;
     }
 
     // TetMesh.h:332-341
     void computeInertia(FloatingType dt) {
+        VBD_PER_VERTEX_FOR
         for (size_t i = 0; i < numVerts; ++i) {
             auto y = mInertia.col(i);
             for (int c = 0; c < 3; ++c) y[c] = isFixed[i] ? mVertPos(c, i) : mVertPos(c, i) + dt * mVelocity(c, i);
         }
     }
 
-    void savePreviousPosition() { mVertPrevPos = mVertPos; }
+    void savePreviousPosition() {
+        if (mVertPrevPos.cols() != mVertPos.cols()) {
+            mVertPrevPos = mVertPos;
+            return;
+        }
+        FloatingType* d = mVertPrevPos.data();
+        const FloatingType* x = mVertPos.data();
+        VBD_PER_VERTEX_FOR
+        for (size_t i = 0; i < numVerts; ++i)
+            for (int c = 0; c < 3; ++c) d[3 * i + c] = x[3 * i + c];
+    }
 
     // Boundary faces: faces of the tets that occur exactly once, in ascending sorted-vertex
     // order, each with the winding of its (only) occurrence (TetMesh.h:80-141).

@@ -237,6 +237,49 @@ def test_cpp_host_solver_helpers_match_the_fp64_closed_form(tmp_path):
     assert eH <= 1e-5 and ef <= 1e-4
 
 
+@pytest.mark.parametrize("openmp", [False, True])
+def test_cpp_host_time_step_helpers_equal_the_float32_forms(tmp_path, openmp):
+    """include/vbd/TetMesh.h applyGravity, computeInertia, updateVelocity and
+    savePreviousPosition (TetMesh.h:305-348) on 100,003 vertices (above the size at which the
+    loops go to OpenMP threads), built without and with OpenMP: bitwise the float32 forms
+    v += h g, y = x + h v (pinned: y = x), v = ((x - x_prev) / h) * 0.999, x_prev = x on the free
+    vertices, pinned velocities untouched."""
+    n = 100003
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(n, 3)).astype(np.float32)
+    xp = (x + rng.normal(scale=1e-3, size=(n, 3))).astype(np.float32)
+    v = rng.normal(size=(n, 3)).astype(np.float32)
+    fixed = (rng.random(n) < 0.1).astype(np.uint8)
+    h = np.float32(1.0 / 60.0)
+    g = np.asarray([0.0, -9.8, 0.0], np.float32)
+    inp = tmp_path / "in.bin"
+    inp.write_bytes(np.int32(n).tobytes() + x.tobytes() + xp.tobytes() + v.tobytes() + fixed.tobytes() +
+                    h.tobytes() + g.tobytes())
+    exe = str(tmp_path / "host_steps_check")
+    lib = os.path.join(ROOT, "vbd_tetmesh_amd")
+    flags = ["-fopenmp"] if openmp else []
+    r = subprocess.run(["g++", "-O2", "-std=c++17", *flags, "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "host_steps_check.cpp"),
+                        "-L", lib, "-lvbd_hip", f"-Wl,-rpath,{lib}", "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "out.bin"
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=60, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.frombuffer(out.read_bytes(), np.float32).reshape(4, n, 3)
+    free = fixed == 0
+    v1 = v.copy()
+    v1[free] = v[free] + h * g
+    y = x.copy()
+    y[free] = x[free] + h * v1[free]
+    v2 = v1.copy()
+    v2[free] = ((x[free] - xp[free]) / h) * np.float32(0.999)
+    assert np.array_equal(got[0], v1)
+    assert np.array_equal(got[1], y)
+    assert np.array_equal(got[2], v2)
+    assert np.array_equal(got[3], x)
+
+
 def test_demo_binary_is_built():
     assert os.path.exists(DEMO), "vbd_dropin_demo not built (run __graft_entry__.build())"
 

@@ -27,8 +27,9 @@ DEMO_OUT = os.path.join(HERE, "vbd_dropin_demo")
 
 
 def build_demo(verbose: bool = False) -> str:
-    """The C++ drop-in driver (include/vbd/*.h over the C ABI), linked against the library."""
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), DEMO_SRC, "-L", HERE,
+    """The C++ drop-in driver (include/vbd/*.h over the C ABI), linked against the library;
+    with OpenMP, so the drop-in's per-vertex host loops run on the host's threads."""
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), DEMO_SRC, "-L", HERE,
            "-lvbd_hip", "-Wl,-rpath,$ORIGIN", "-o", DEMO_OUT + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)

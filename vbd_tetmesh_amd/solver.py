@@ -103,21 +103,27 @@ class TetMesh:
     def vertAdjacentTets(self) -> List[np.ndarray]:
         return [self.adjTets[self.adjOffsets[i]: self.adjOffsets[i + 1]] for i in range(self.numVerts)]
 
-    # time-step helpers (host), TetMesh.h:305-348
+    # time-step helpers (host), TetMesh.h:305-348; float32 arithmetic in the reference's order,
+    # evaluated in place over all rows and stored to the free rows only
     def updateVelocity(self, dt):
-        free = self.isFixed == 0
-        self.mVelocity[free] = ((self.mVertPos[free] - self.mVertPrevPos[free]) / np.float32(dt)) * np.float32(0.999)
+        free = (self.isFixed == 0)[:, None]
+        t = np.subtract(self.mVertPos, self.mVertPrevPos)
+        t /= np.float32(dt)
+        t *= np.float32(0.999)
+        np.copyto(self.mVelocity, t, where=free)
 
     def applyGravity(self, gravity, dt):
-        free = self.isFixed == 0
-        self.mVelocity[free] += np.float32(dt) * np.asarray(gravity, np.float32)
+        free = (self.isFixed == 0)[:, None]
+        np.add(self.mVelocity, np.float32(dt) * np.asarray(gravity, np.float32), out=self.mVelocity, where=free)
 
     def computeInertia(self, dt):
-        free = (self.isFixed == 0)[:, None]
-        self.mInertia[:] = np.where(free, self.mVertPos + np.float32(dt) * self.mVelocity, self.mVertPos)
+        fixed = (self.isFixed != 0)[:, None]
+        np.multiply(self.mVelocity, np.float32(dt), out=self.mInertia)
+        np.add(self.mVertPos, self.mInertia, out=self.mInertia)
+        np.copyto(self.mInertia, self.mVertPos, where=fixed)
 
     def savePreviousPosition(self):
-        self.mVertPrevPos[:] = self.mVertPos
+        np.copyto(self.mVertPrevPos, self.mVertPos)
 
     # surface output, TetMesh.h:45-186
     def extractSurfaceFaces(self) -> np.ndarray:
