@@ -227,7 +227,8 @@ int vbd_autotune(vbd_solver* h, int32_t iterations, int32_t* best_lanes, int32_t
                  float* best_ms);
 /* Runs `iterations` plain sweeps (+ blends if use_acceleration in params) with HIP events
  * around every launch on the handle's stream.  Outputs: total color-kernel milliseconds,
- * number of color launches, total blend milliseconds, number of blend launches. */
+ * number of color launches, total blend milliseconds, number of blend launches.  Moves the
+ * positions. */
 int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int32_t* color_launches,
                         double* blend_ms, int32_t* blend_launches);
 /* The color launches of `sweeps` sweeps (the step's own kernels: with Chebyshev in the
