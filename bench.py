@@ -140,7 +140,8 @@ def cpu_baseline(m, iterations: int, target_seconds: float = 10.0):
                                        f"({cores} threads requested, racy; {nfree} solves in {sr:.1f} s)"}
 
 
-def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, iterations: int = 20):
+def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, iterations: int = 20,
+                persistent: bool = True):
     """Batched workload: `count` independent beams of uneven size (nx ~ U{8..64}, ny = nz ~
     U{2..8}, per-beam mu, nu), per-tet materials, 20 iterations with Chebyshev 0.95, all meshes
     stepped by one hipGraph per frame, against the same meshes stepped one at a time with plain
@@ -193,7 +194,7 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             d.synchronize()
         return (time.perf_counter() - t0) / baseline_frames
 
-    pab = persistent_ab(b, np.concatenate([d["positions"] for d in descs]), frames)
+    pab = persistent_ab(b, np.concatenate([d["positions"] for d in descs]), frames) if persistent else None
     tp = per_mesh(False)
     tg = per_mesh(True)
     vi = float(nv) * iterations
@@ -301,8 +302,9 @@ def launch_roofline(d, sweeps: int, traffic_name=None):
            "traffic_gbs": round(traffic / sec / 1e9, 1) if traffic else None,
            "traffic_frac": round(traffic / sec / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
            "kernel": kernel, "alg_bytes_per_launch": round(lbytes / per_sweep),
-           "alg_bytes": "layout minimum: records as stored (32 B compact, +8 per-tet mu/lambda) + 72 B per free "
-                        "vertex (row 4, x 16 read + 16 written, inertia 16, mass 4, saved x 16)",
+           "alg_bytes": "layout minimum: records as stored (32 B compact, or 52 B full for the per-vertex kernels "
+                        "and the corotated material; +8 per-tet mu/lambda) + 72 B per free vertex (row 4, x 16 read "
+                        "+ 16 written, inertia 16, mass 4, saved x 16)",
            "avg_launch_ms": round(ms, 4), "launches_timed": n,
            "timing": "graph replay of the step's color launches, HIP events around the replay",
            "yardstick_rate": {"bytes_per_launch": round(ybytes / per_sweep), "gbs": round(ybytes / per_sweep / sec / 1e9, 1),
@@ -312,7 +314,7 @@ def launch_roofline(d, sweeps: int, traffic_name=None):
 
 
 def bench_sphere(n: int = 96, frames: int = 30, iterations: int = 30, slot_order: int = 0, plane: float = -1.2,
-                 roofline: bool = True):
+                 roofline: bool = True, persistent: bool = True):
     """Irregular workload as configured: the 96^3 voxel sphere (interior vertices jittered, ids
     shuffled), mu = 5e4, lambda = 2e5, gravity -z, 30 iterations/step, Chebyshev 0.95, dropped
     from rest onto the penalty plane z = `plane` (k = 1e4 N/m) for 30 frames.  `value` times the
@@ -361,7 +363,7 @@ def bench_sphere(n: int = 96, frames: int = 30, iterations: int = 30, slot_order
     free_ms = frame_ms[:first_contact] if first_contact is not None else frame_ms
     contact_ms = frame_ms[first_contact:] if first_contact is not None else []
     roof = launch_roofline(d, iterations, "traffic_color_tiled_sphere96") if roofline else None
-    pab = persistent_ab(d, m.x0, min(frames, 10)) if roofline else None
+    pab = persistent_ab(d, m.x0, min(frames, 10)) if (roofline and persistent) else None
     vi = float(info.num_verts) * iterations * frames
     per_frame_vi = float(info.num_verts) * iterations
     return {"workload": f"sphere {n}^3 voxels, {info.num_verts} verts, {info.num_tets} tets, {info.num_colors} colors, "
@@ -580,7 +582,9 @@ def main():
     ap.add_argument("--sphere-frames", type=int, default=30, help="frames of the sphere workload (30 as configured)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-corotated", action="store_true", help="skip the corotated-material run of the cube")
-    ap.add_argument("--no-hybrid", action="store_true", help="skip the host-buffer (PCIe-inclusive) frame loop")
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the host-buffer (PCIe-inclusive) frame loops")
+    ap.add_argument("--no-persistent-ab", action="store_true",
+                    help="skip the persistent-sweep A/B runs (for kernel traces of the default launches)")
     ap.add_argument("--only", choices=["batched", "sphere", "adjoint", "persistent"], default=None,
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
@@ -600,7 +604,9 @@ def main():
     from vbd_tetmesh_amd.solver import upload
 
     if args.only:
-        fn = {"batched": bench_batch, "sphere": lambda: bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order),
+        fn = {"batched": lambda: bench_batch(persistent=not args.no_persistent_ab),
+              "sphere": lambda: bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order,
+                                             persistent=not args.no_persistent_ab),
               "adjoint": bench_adjoint, "persistent": bench_persistent}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
@@ -671,10 +677,11 @@ def main():
                           "note": "whole timed step: layout bytes of the color launches and blends per "
                                   "vertex-iteration x the measured rate"},
     }
-    out["persistent_ab"] = persistent_ab(d, m.x0, args.steps)
-    pa = out["persistent_ab"]
-    log(f"[bench] persistent sweep: {pa['persistent_ms_per_step']} ms/step on {pa['persistent_grid']} blocks vs "
-        f"per-color {pa['per_color_ms_per_step']} ms/step, bitwise {pa['bitwise_equal']}")
+    if not args.no_persistent_ab:
+        out["persistent_ab"] = persistent_ab(d, m.x0, args.steps)
+        pa = out["persistent_ab"]
+        log(f"[bench] persistent sweep: {pa['persistent_ms_per_step']} ms/step on {pa['persistent_grid']} blocks vs "
+            f"per-color {pa['per_color_ms_per_step']} ms/step, bitwise {pa['bitwise_equal']}")
     out["energy"] = bench_energy(d)
     log(f"[bench] energies: " + ", ".join(f"{k} {v['ms_per_eval']:.3f} ms ({v['frac']})" for k, v in out["energy"].items()))
     if not args.no_corotated:
@@ -699,13 +706,13 @@ def main():
                 f"{hc['loop_value']:.1f}; set-up {hc['setup_s']}")
     if not args.no_batch:
         d = None
-        bb = bench_batch()
+        bb = bench_batch(persistent=not args.no_persistent_ab)
         out["batched"] = bb
         log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step; per-mesh plain "
             f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
             f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
     if not args.no_sphere:
-        sp = bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order)
+        sp = bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order, persistent=not args.no_persistent_ab)
         # the same drop onto a plane the sinking sphere reaches within the frames
         sc = bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order, plane=-1.02, roofline=False)
         sp["contact_run"] = {k: sc[k] for k in ("workload", "value", "ms_per_step", "first_frame_ending_in_contact",
