@@ -22,12 +22,19 @@ def test_dropin_headers_compile_cleanly(tmp_path):
 
 
 @pytest.mark.parametrize("one_based", [True, False])
-def test_tetgen_round_trip(tmp_path, one_based):
-    m = meshgen.beam(4, 2, 2)
-    prefix = str(tmp_path / "beam")
-    meshgen.write_tetgen(prefix, m.rest, m.tets, one_based=one_based)
+@pytest.mark.parametrize("chunk", [7, 65536])
+def test_tetgen_round_trip(tmp_path, one_based, chunk):
+    """write_tetgen in chunks of `chunk` rows (several, with a remainder, at 7) reads back
+    bitwise: noisy float32 coordinates, tets, one header and one row per node / tet."""
+    m = meshgen.noisy_cube(4, seed=3)
+    prefix = str(tmp_path / "cube")
+    meshgen.write_tetgen(prefix, m.x0, m.tets, one_based=one_based, chunk=chunk)
     pos, tets = meshgen.read_tetgen(prefix)
-    assert np.array_equal(pos, m.rest) and np.array_equal(tets, m.tets)
+    assert np.array_equal(pos, m.x0) and np.array_equal(tets, m.tets)
+    with open(prefix + ".node") as f:
+        assert len(f.read().splitlines()) == 1 + m.num_verts
+    with open(prefix + ".ele") as f:
+        assert len(f.read().splitlines()) == 1 + m.num_tets
 
 
 def _build_tetgen_check(tmp_path):
