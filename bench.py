@@ -174,6 +174,7 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
     xb = b.positions()
     if not np.all(np.isfinite(xb)):
         raise SystemExit("non-finite positions in the batched run")
+    roof = launch_roofline(b, iterations)
 
     def per_mesh(graph: bool):
         ds = []
@@ -204,7 +205,7 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
             "frames": frames,
             "per_mesh_plain_ms_per_step": round(tp * 1e3, 3), "per_mesh_graph_ms_per_step": round(tg * 1e3, 3),
             "speedup_vs_per_mesh_plain": round(tp / tb, 2), "speedup_vs_per_mesh_graph": round(tg / tb, 2),
-            "baseline_frames": baseline_frames, "persistent_ab": pab,
+            "baseline_frames": baseline_frames, "roofline": roof, "persistent_ab": pab,
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
 
@@ -708,7 +709,8 @@ def main():
         d = None
         bb = bench_batch(persistent=not args.no_persistent_ab)
         out["batched"] = bb
-        log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step; per-mesh plain "
+        log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step, color kernel frac "
+            f"{bb['roofline']['frac']}; per-mesh plain "
             f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
             f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
     if not args.no_sphere:
