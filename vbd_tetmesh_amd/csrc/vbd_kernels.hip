@@ -590,8 +590,18 @@ __device__ __forceinline__ void tile_solve(const int tile, const int* __restrict
     load_records<K, MAT, COROT>(R, src, 0, RT - 1, tid, T);
     const int v0 = tileStart[tile], v1 = tileStart[tile + 1];
     const int nv = v1 - v0;
+    f4 xs[K], ya[K], yb[K], yc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        xs[k] = gather_pos(prs, v0 + owner_offset(R.t[k]));  // owner: the tile-local vertex index
+        ya[k] = gather_pos(prs, R.t[k].x & SLOT_MASK);
+        yb[k] = gather_pos(prs, R.t[k].y & SLOT_MASK);
+        yc[k] = gather_pos(prs, R.t[k].z & SLOT_MASK);
+    }
     // the tile's vertex data, loaded by the first wave only (a tile has fewer than 64 vertices);
-    // clamped indices, lanes past nv load and ignore a copy
+    // clamped indices, lanes past nv load and ignore a copy.  Issued after the gathers, so that the
+    // first wave's gathers are in flight with them (vmcnt counts loads in issue order: loads issued
+    // before the gathers would hold the first wave's gathers back until they returned).
     const bool wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) == 0;
     int vrow = 0;
     f4 vx = {}, vy = {};
@@ -604,14 +614,6 @@ __device__ __forceinline__ void tile_solve(const int tile, const int* __restrict
         vm = __builtin_nontemporal_load(&mass[vs]);
     }
     const int vrow0 = row[v0];
-    f4 xs[K], ya[K], yb[K], yc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        xs[k] = gather_pos(prs, v0 + owner_offset(R.t[k]));  // owner: the tile-local vertex index
-        ya[k] = gather_pos(prs, R.t[k].x & SLOT_MASK);
-        yb[k] = gather_pos(prs, R.t[k].y & SLOT_MASK);
-        yc[k] = gather_pos(prs, R.t[k].z & SLOT_MASK);
-    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int q = tid + k * T;
