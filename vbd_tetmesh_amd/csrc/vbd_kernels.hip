@@ -483,10 +483,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const void* base, int
 // them, so the compiler's wait counts can keep the gathers in flight with the vertex loads.
 template <int K, bool MAT, bool COROT>
 __device__ __forceinline__ void load_records(RecRegs<K, MAT>& R, const RecSrc& src, int r0, int pmax, int tid, int T) {
+    // every record's corner slots first: the gathers need only those, and vmcnt counts loads in
+    // issue order, so a coefficient load issued between two slot loads would hold back the
+    // second record's gathers
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = min(r0 + tid + k * T, pmax);  // relative to the color's first record
         R.t[k] = __builtin_bit_cast(i4, __builtin_amdgcn_raw_buffer_load_b128(src.t, p << 4, 0, BUF_NT));
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int p = min(r0 + tid + k * T, pmax);
         if constexpr (COROT) {
             R.a[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.a, p << 4, 0, BUF_NT));
             R.b[k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(src.b, p << 4, 0, BUF_NT));
