@@ -557,11 +557,12 @@ __device__ __forceinline__ Acc compact_contrib(const bool corner0, const f4 qv, 
 
 // One tile of one color: records, gathers, contributions, the LDS reduction and the vertex
 // solves (see the block comment above).  `tileStart` points at the color's tile table and the
-// record pointers at the color's first padded record slot.  Uses every thread of the block; the
-// waves past the tile's last vertex group skip the reduction (no early return, so a persistent
-// caller can loop over tiles).
+// record pointers at the color's first padded record slot; T is the block size.  Uses every
+// thread of the block; the waves past the tile's last vertex group skip the reduction (no early
+// return, so a persistent caller can loop over tiles).
 template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
-__device__ __forceinline__ void tile_solve(const int tile, const int* __restrict__ tileStart, const int nslots,
+__device__ __forceinline__ void tile_solve(const int tile, const int T, const int* __restrict__ tileStart,
+                                           const int nslots,
                                            f4* __restrict__ pos, const f4* __restrict__ inert,
                                            const float* __restrict__ mass, const int* __restrict__ row,
                                            const i4* __restrict__ ptet, const f4* __restrict__ pq,
@@ -571,7 +572,6 @@ __device__ __forceinline__ void tile_solve(const int tile, const int* __restrict
                                            const ContactArgs& ct, float* red) {
     // LDS: 9 * RT contribution values, then VM = T/VDIV entries (a tile has at most VM - 1
     // vertices): row offsets (nv + 1 of them), x, y, m
-    const int T = (int)blockDim.x;
     const int RT = K * T;
     const int VM = T / VDIV;
     int* lrow = reinterpret_cast<int*>(red + 9 * RT);
@@ -699,20 +699,24 @@ __device__ __forceinline__ void tile_solve(const int tile, const int* __restrict
 // COROT: the corotated material of the reference GPU kernel on the full records (pa, pb, pc)
 // instead of the compact stable Neo-Hookean records (pq); same tiles, reduction and order.
 template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
-__global__ __launch_bounds__(512) void k_color_tiled(const int* __restrict__ tileStart, int ntiles, int nslots,
-                                                     f4* __restrict__ pos, const f4* __restrict__ inert,
-                                                     const float* __restrict__ mass, const int* __restrict__ row,
-                                                     const i4* __restrict__ ptet, const f4* __restrict__ pq,
-                                                     const f4* __restrict__ pa, const f4* __restrict__ pb,
-                                                     const float* __restrict__ pc, const f2* __restrict__ pmat,
+// The arguments a block needs before its first loads come first (the record streams, the tile
+// table, the positions, the tile and slot counts and the block size T, which the kernel takes as
+// an argument instead of reading the dispatch's workgroup size): built with kernel-argument
+// preloading they arrive in SGPRs, so the record loads issue without a scalar load before them.
+__global__ __launch_bounds__(512) void k_color_tiled(const i4* __restrict__ ptet, const f4* __restrict__ pq,
+                                                     const f2* __restrict__ pmat, const int* __restrict__ tileStart,
+                                                     f4* __restrict__ pos, int ntiles, int nslots, int T,
+                                                     const f4* __restrict__ inert, const float* __restrict__ mass,
+                                                     const int* __restrict__ row, const f4* __restrict__ pa,
+                                                     const f4* __restrict__ pb, const float* __restrict__ pc,
                                                      f4* __restrict__ save, float mu, float lambda, float sdt,
                                                      ContactArgs ct) {
     extern __shared__ float red[];
     const int xcd = (int)(blockIdx.x & 7);
     const int tile = (int)(((int64_t)ntiles * xcd) >> 3) + (int)(blockIdx.x >> 3);
     if (tile >= (int)(((int64_t)ntiles * (xcd + 1)) >> 3)) return;  // whole block
-    tile_solve<K, MAT, CONTACT, SAVE, COROT>(tile, tileStart, nslots, pos, inert, mass, row, ptet, pq, pa, pb, pc, pmat,
-                                             save, mu, lambda, sdt, ct, red);
+    tile_solve<K, MAT, CONTACT, SAVE, COROT>(tile, T, tileStart, nslots, pos, inert, mass, row, ptet, pq, pa, pb, pc,
+                                             pmat, save, mu, lambda, sdt, ct, red);
 }
 
 template <int K, bool MAT, bool CONTACT, bool SAVE, bool COROT>
@@ -726,9 +730,9 @@ static hipError_t launch_tiled_t(const DevMesh& m, int color, int block, float s
     if (COROT && !m.tpa) return hipErrorInvalidValue;  // full records not padded (build_tiles full)
     const size_t base = (size_t)m.colorTileBase[color] * m.tileRecords;
     hipLaunchKernelGGL((k_color_tiled<K, MAT, CONTACT, SAVE, COROT>), dim3(8 * per), dim3(block), lds, s,
-                       m.tileStart + m.colorTileBase[color], ntiles, m.nV, m.pos, m.inert, m.mass, m.row,
-                       m.tpt + base, m.tpq + base, COROT ? m.tpa + base : nullptr, COROT ? m.tpb + base : nullptr,
-                       COROT ? m.tpc + base : nullptr, MAT ? m.tpm + base : nullptr, save, mu, lambda, sdt, ct);
+                       m.tpt + base, m.tpq + base, MAT ? m.tpm + base : nullptr, m.tileStart + m.colorTileBase[color],
+                       m.pos, ntiles, m.nV, block, m.inert, m.mass, m.row, COROT ? m.tpa + base : nullptr,
+                       COROT ? m.tpb + base : nullptr, COROT ? m.tpc + base : nullptr, save, mu, lambda, sdt, ct);
     return hipGetLastError();
 }
 
@@ -828,7 +832,8 @@ __global__ __launch_bounds__(512) void k_sweep_persistent(int nColors, const int
             if (!first) lds_barrier();  // the previous tile's LDS reads are done
             first = false;
             const size_t rb = (size_t)cb * RT;
-            tile_solve<K, MAT, CONTACT, SAVE, false>(t, tileStart + cb, nslots, pos, inert, mass, row, ptet + rb,
+            tile_solve<K, MAT, CONTACT, SAVE, false>(t, (int)blockDim.x, tileStart + cb, nslots, pos, inert, mass, row,
+                                                     ptet + rb,
                                                      pq + rb, nullptr, nullptr, nullptr, MAT ? pmat + rb : nullptr,
                                                      save, mu, lambda, sdt, ct, red);
         }
