@@ -596,15 +596,20 @@ __device__ __forceinline__ void tile_solve(const int tile, const int T, const in
     RecRegs<K, MAT> R;
     load_records<K, MAT, COROT>(R, src, 0, RT - 1, tid, T);
     const int v0 = tileStart[tile], v1 = tileStart[tile + 1];
+    __builtin_amdgcn_sched_barrier(0);  // the tile-table load issues with the record loads
     const int nv = v1 - v0;
+    // the corners first: they need only the record's slots, while the record's own position
+    // needs the tile's first slot from the scalar tile-table load
     f4 xs[K], ya[K], yb[K], yc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        xs[k] = gather_pos(prs, v0 + owner_offset(R.t[k]));  // owner: the tile-local vertex index
         ya[k] = gather_pos(prs, R.t[k].x & SLOT_MASK);
         yb[k] = gather_pos(prs, R.t[k].y & SLOT_MASK);
         yc[k] = gather_pos(prs, R.t[k].z & SLOT_MASK);
     }
+    __builtin_amdgcn_sched_barrier(0);  // keeps the corner gathers ahead of the tile-table wait
+#pragma unroll
+    for (int k = 0; k < K; ++k) xs[k] = gather_pos(prs, v0 + owner_offset(R.t[k]));  // owner: tile-local index
     // the tile's vertex data, loaded by the first wave only (a tile has fewer than 64 vertices);
     // clamped indices, lanes past nv load and ignore a copy.  Issued after the gathers, so that the
     // first wave's gathers are in flight with them (vmcnt counts loads in issue order: loads issued
