@@ -671,9 +671,7 @@ __device__ __forceinline__ void tile_solve(const int tile, const int T, const in
         const bool ok = i < nv;
         float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
         if (ok) {
-            const int q1 = lrow[i + 1];
-            for (int q = lrow[i] + lane; q < q1; q += TG) {
-                const float* rq = red + 9 * q;
+            auto add9 = [&](const float* rq) __attribute__((always_inline)) {
                 s0 += rq[0];
                 s1 += rq[1];
                 s2 += rq[2];
@@ -683,7 +681,16 @@ __device__ __forceinline__ void tile_solve(const int tile, const int T, const in
                 s6 += rq[6];
                 s7 += rq[7];
                 s8 += rq[8];
+            };
+            // the lane's records q, q + TG, ... added in that order; two per step, so that the
+            // LDS reads of both are in flight together (the same sums, half the round trips)
+            const int q1 = lrow[i + 1];
+            int q = lrow[i] + lane;
+            for (; q + TG < q1; q += 2 * TG) {
+                add9(red + 9 * q);
+                add9(red + 9 * (q + TG));
             }
+            if (q < q1) add9(red + 9 * q);
         }
         s0 = gsum<TG>(s0); s1 = gsum<TG>(s1); s2 = gsum<TG>(s2);
         s3 = gsum<TG>(s3); s4 = gsum<TG>(s4); s5 = gsum<TG>(s5);
