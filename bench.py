@@ -141,11 +141,14 @@ def cpu_baseline(m, iterations: int, target_seconds: float = 10.0):
 
 
 def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, iterations: int = 20,
-                persistent: bool = True):
+                persistent: bool = True, launch=None):
     """Batched workload: `count` independent beams of uneven size (nx ~ U{8..64}, ny = nz ~
     U{2..8}, per-beam mu, nu), per-tet materials, 20 iterations with Chebyshev 0.95, all meshes
     stepped by one hipGraph per frame, against the same meshes stepped one at a time with plain
-    (non-graph) launches, and one at a time with a hipGraph per mesh."""
+    (non-graph) launches, and one at a time with a hipGraph per mesh (baseline_frames 0 skips
+    them).  `launch` = (lanes, block, pairs) fixes the launch configuration instead of
+    autotuning it.  The color kernel's measured HBM traffic is read from the profile committed
+    for the configuration that ran (profiles/traffic_color_tiled_batched<count>_<block>x<pairs>.txt)."""
     from vbd_tetmesh_amd import VBDParams, meshgen
     from vbd_tetmesh_amd.solver import DeviceMesh, prepare_host
     ms = meshgen.batch_beams(count, seed=0)
@@ -160,7 +163,11 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
     nt = sum(m.num_tets for m in ms)
     b = DeviceMesh().init_batch(descs)
     b.set_params(params)
-    tune = b.autotune(8)
+    if launch is None:
+        tune = b.autotune(8)
+    else:
+        b.set_launch_config(launch[0], launch[1], launch[2])
+        tune = (launch[0], launch[2], launch[1])
     b.set_positions(np.concatenate([d["positions"] for d in descs]))
     b.step(1)  # warm-up (captures the graph)
     b.synchronize()
@@ -174,7 +181,8 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
     xb = b.positions()
     if not np.all(np.isfinite(xb)):
         raise SystemExit("non-finite positions in the batched run")
-    roof = launch_roofline(b, iterations)
+    roof = launch_roofline(b, iterations,
+                           f"traffic_color_tiled_batched{count}_{tune[2]}x{tune[1]}" if tune[0] == 0 else None)
 
     def per_mesh(graph: bool):
         ds = []
@@ -196,15 +204,17 @@ def bench_batch(count: int = 256, frames: int = 60, baseline_frames: int = 3, it
         return (time.perf_counter() - t0) / baseline_frames
 
     pab = persistent_ab(b, np.concatenate([d["positions"] for d in descs]), frames) if persistent else None
-    tp = per_mesh(False)
-    tg = per_mesh(True)
+    tp = per_mesh(False) if baseline_frames > 0 else None
+    tg = per_mesh(True) if baseline_frames > 0 else None
     vi = float(nv) * iterations
     return {"workload": f"{count} beams, {nv} verts, {nt} tets, per-tet materials, {iterations} iters/step, "
                         f"Chebyshev 0.95, one hipGraph per frame over all meshes",
             "value": round(vi / tb / 1e6, 2), "unit": "M vertex-iters/s", "ms_per_step": round(tb * 1e3, 4),
             "frames": frames,
-            "per_mesh_plain_ms_per_step": round(tp * 1e3, 3), "per_mesh_graph_ms_per_step": round(tg * 1e3, 3),
-            "speedup_vs_per_mesh_plain": round(tp / tb, 2), "speedup_vs_per_mesh_graph": round(tg / tb, 2),
+            "per_mesh_plain_ms_per_step": round(tp * 1e3, 3) if tp else None,
+            "per_mesh_graph_ms_per_step": round(tg * 1e3, 3) if tg else None,
+            "speedup_vs_per_mesh_plain": round(tp / tb, 2) if tp else None,
+            "speedup_vs_per_mesh_graph": round(tg / tb, 2) if tg else None,
             "baseline_frames": baseline_frames, "roofline": roof, "persistent_ab": pab,
             "launch_config": {"lanes_per_vertex": tune[0], "pairs_per_lane": tune[1], "block_size": tune[2]}}
 
@@ -579,6 +589,11 @@ def main():
     ap.add_argument("--slot-order", type=int, default=0, help="vertex order inside a color: 0 auto, 1 vertex id, 2 Morton")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB; run only the copy probe")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-mesh comparison")
+    ap.add_argument("--batch-frames", type=int, default=60, help="timed frames of the batched workload")
+    ap.add_argument("--batch-baseline-frames", type=int, default=3,
+                    help="frames of the per-mesh baselines (0 skips them)")
+    ap.add_argument("--batch-launch", default=None,
+                    help="lanes,block,pairs: fixed launch configuration of the batch (default: autotuned)")
     ap.add_argument("--no-sphere", action="store_true", help="skip the irregular sphere workload")
     ap.add_argument("--sphere-frames", type=int, default=30, help="frames of the sphere workload (30 as configured)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
@@ -604,8 +619,11 @@ def main():
     from vbd_tetmesh_amd import VBDParams, meshgen
     from vbd_tetmesh_amd.solver import upload
 
+    batch_kw = dict(frames=args.batch_frames, baseline_frames=args.batch_baseline_frames,
+                    persistent=not args.no_persistent_ab,
+                    launch=tuple(int(v) for v in args.batch_launch.split(",")) if args.batch_launch else None)
     if args.only:
-        fn = {"batched": lambda: bench_batch(persistent=not args.no_persistent_ab),
+        fn = {"batched": lambda: bench_batch(**batch_kw),
               "sphere": lambda: bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order,
                                              persistent=not args.no_persistent_ab),
               "adjoint": bench_adjoint, "persistent": bench_persistent}[args.only]
@@ -707,7 +725,7 @@ def main():
                 f"{hc['loop_value']:.1f}; set-up {hc['setup_s']}")
     if not args.no_batch:
         d = None
-        bb = bench_batch(persistent=not args.no_persistent_ab)
+        bb = bench_batch(**batch_kw)
         out["batched"] = bb
         log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step, color kernel frac "
             f"{bb['roofline']['frac']}; per-mesh plain "
