@@ -405,10 +405,12 @@ def bench_energy(d, reps: int = 20):
     return res
 
 
-def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
+def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None, tag: str = "cube128"):
     """The reference GPU kernel's material (corotated, VBD_GPU_Wrapper.cu:114-213, with its
-    x1.000001 inertia term) on the same resident mesh: autotuned lane-group kernel, whole steps
-    replayed from a hipGraph, and the color kernel against the same byte yardstick."""
+    x1.000001 inertia term) on the same resident mesh: autotuned launch configuration, whole steps
+    replayed from a hipGraph, the color kernel against the layout's bytes, and its measured traffic
+    from the counter profile of the configuration that ran
+    (profiles/traffic_color_tiled_corot_<tag>_<block>x<pairs>.txt)."""
     from vbd_tetmesh_amd import _capi as A
     info0 = d.info()
     d.set_material_model(A.VBD_MATERIAL_COROTATED_GPU)
@@ -428,7 +430,9 @@ def bench_corotated(d, m, iterations: int, steps: int = 5, cfg=None):
     secs = time.perf_counter() - t0
     info = d.info()
     x = d.positions()
-    roof = launch_roofline(d, iterations)
+    roof = launch_roofline(d, iterations,
+                           f"traffic_color_tiled_corot_{tag}_{info.block_size}x{info.pairs_per_lane}"
+                           if info.lanes_per_vertex == 0 else None)
     roof["kernel"] = "k_color_tiled (corotated)" if info.lanes_per_vertex == 0 else "k_color_corot"
     roof["note"] = "VALU-bound (5 polar steps per record); HBM fraction on the 52 B full records"
     d.set_material_model(A.VBD_MATERIAL_STABLE_NEO_HOOKEAN)
@@ -705,7 +709,8 @@ def main():
     log(f"[bench] energies: " + ", ".join(f"{k} {v['ms_per_eval']:.3f} ms ({v['frac']})" for k, v in out["energy"].items()))
     if not args.no_corotated:
         cr = bench_corotated(d, m, args.iterations,
-                             cfg=None if args.lanes is None else (args.lanes, args.block, args.unroll))
+                             cfg=None if args.lanes is None else (args.lanes, args.block, args.unroll),
+                             tag=f"cube{args.n}")
         out["corotated"] = cr
         log(f"[bench] corotated: {cr['value']:.1f} M vertex-iters/s, color kernel frac {cr['roofline']['frac']}")
     if not args.no_hybrid:
