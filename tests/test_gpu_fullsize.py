@@ -5,10 +5,21 @@
 * The 96^3 voxel sphere (485,385 verts, 2,780,400 tets, 11 colors, shuffled ids), 30
   iterations, Chebyshev 0.95, with the penalty plane cutting its bottom cap so that contact is
   active from the first sweep, 2 whole frames.
+* The same cube with the reference GPU kernel's corotated material (VBD_GPU_Wrapper.cu:181,
+  VBD_DeviceMath.cuh:127-198; the oracle's material 1), 2 whole frames.
 
-Both run the oracle's closed-form backend in FP32 and FP64 on the host's cores (the host's CPU
+All run the oracle's closed-form backend in FP32 and FP64 on the host's cores (the host's CPU
 share: OMP_NUM_THREADS when set).  Tolerance: per frame max|x_gpu - x_oracle| / max|x_oracle|
-<= 1e-5 against both; the FP32 oracle's own distance from FP64 is printed beside it.
+<= 1e-5 against both for the stable Neo-Hookean material (the reference CPU solver's).
+
+The corotated material (the reference GPU kernel's) is further from FP64 in any FP32
+implementation on this mesh: its rotation comes from five Newton steps of the polar
+decomposition, which amplify rounding on the noisy cube's most distorted tets, and the FP32
+oracle itself lands about 2e-5 from the FP64 oracle after one frame (7e-6 on a 64^3 cube).  No
+FP32 implementation meets 1e-5 against either, so the bars are relative to that floor o (the
+FP32 oracle vs FP64, per frame): the device within 1.5 o + 1e-6 of FP64 (no further from exact
+arithmetic than the FP32 restatement of the reference) and within 2 o + 1e-6 of the FP32
+oracle (two FP32 roundings of the same step).
 """
 import os
 
@@ -17,6 +28,7 @@ import pytest
 
 from oracle import oracle as O
 from vbd_tetmesh_amd import VBDParams, meshgen
+from vbd_tetmesh_amd import _capi as A
 from vbd_tetmesh_amd.solver import upload
 
 pytestmark = pytest.mark.gpu
@@ -35,15 +47,16 @@ def relmax(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def whole_frames(m, iterations, frames, contact=None):
+def whole_frames(m, iterations, frames, contact=None, material=0):
     threads = host_threads()
     prep = O.prepare(m)
     p = O.make_params(dt=H, iterations=iterations, rho=0.95, mu=m.mu, lam=m.lam, gravity=m.gravity, contact=contact,
-                      threads=threads)
+                      threads=threads, material=material)
     _, _, of32, s32 = O.simulate(prep, p, m.x0, None, frames)
     _, _, of64, s64 = O.simulate(prep, p, m.x0, None, frames, f64=True, rest=m.rest, density=m.density)
     params = VBDParams(dt=H, numIterations=iterations, gravity=m.gravity, useAcceleration=True, accelerationRho=0.95,
-                       mu=m.mu, lambda_=m.lam, contact=contact)
+                       mu=m.mu, lambda_=m.lam, contact=contact,
+                       material=A.VBD_MATERIAL_COROTATED_GPU if material == 1 else A.VBD_MATERIAL_STABLE_NEO_HOOKEAN)
     d = upload(m, params=params)
     got = []
     for _ in range(frames):
@@ -80,3 +93,16 @@ def test_sphere96_whole_frames_with_contact_match_oracle():
         print(f"sphere96 frame {f + 1}: gpu vs fp32 oracle {r32:.2e}, vs fp64 {r64:.2e}, fp32 oracle vs fp64 {o:.2e} "
               f"({below} vertices still below the plane)", flush=True)
         assert r32 <= TOL and r64 <= TOL
+
+
+@pytest.mark.timeout(900)
+def test_cube128_corotated_whole_frames_match_oracle():
+    m = meshgen.noisy_cube(128, seed=0)
+    print(f"[cube128 corotated whole frames] nV={m.num_verts} nT={m.num_tets}", flush=True)
+    got, of32, of64 = whole_frames(m, 20, 2, material=1)
+    for f in range(2):
+        r32, r64, o = relmax(got[f], of32[f]), relmax(got[f], of64[f]), relmax(of32[f], of64[f])
+        disp = np.abs(of64[f] - m.x0).max()
+        print(f"cube128 corotated frame {f + 1}: gpu vs fp32 oracle {r32:.2e}, vs fp64 {r64:.2e}, fp32 oracle vs fp64 "
+              f"{o:.2e} (max displacement {disp:.3e} m)", flush=True)
+        assert r64 <= 1.5 * o + 1e-6 and r32 <= 2.0 * o + 1e-6
