@@ -135,3 +135,39 @@ def test_surface_extraction_is_closed_outward_and_complete(tmp_path):
     lines = path.read_text().splitlines()
     assert sum(1 for ln in lines if ln.startswith("v ")) == m.num_verts
     assert sum(1 for ln in lines if ln.startswith("f ")) == 704
+
+
+def test_python_wrappers_reject_arrays_of_the_wrong_size():
+    """The C ABI takes plain pointers and reads num_verts / num_tets entries from each array, so
+    the Python wrappers check the sizes first: a short array is a ValueError, never read past its
+    end (no GPU needed: the checks run before any ABI call)."""
+    from vbd_tetmesh_amd.solver import DeviceMesh, _mesh_arrays
+    m = meshgen.beam(3, 1, 1)
+    t, off, verts, dminv, vol = prepare_host(m)
+    good = dict(positions=t.mVertPos, mass=t.vertexMass, is_fixed=t.isFixed, tets=t.tets, dminv=dminv,
+                rest_volume=vol, color_off=off, color_verts=verts)
+    a = _mesh_arrays(**good)
+    assert a["positions"].shape == (m.num_verts, 3) and a["tets"].shape == (m.num_tets, 4)
+    nt = m.num_tets
+    for key, bad in (("mass", t.vertexMass[:-1]), ("is_fixed", t.isFixed[:-2]), ("color_verts", verts[:-1]),
+                     ("dminv", dminv[:-9]), ("rest_volume", vol[:-1]), ("color_off", np.zeros(0, np.int32))):
+        with pytest.raises(ValueError):
+            _mesh_arrays(**{**good, key: bad})
+    with pytest.raises(ValueError):
+        _mesh_arrays(**good, inertia=t.mVertPos[:-1])
+    with pytest.raises(ValueError):
+        _mesh_arrays(**good, tet_mu=np.ones(nt, np.float32))  # lambda missing
+    with pytest.raises(ValueError):
+        _mesh_arrays(**good, tet_mu=np.ones(nt, np.float32), tet_lambda=np.ones(nt - 1, np.float32))
+    # the state and differentiation wrappers of a handle (its counts set, no device call made)
+    d = object.__new__(DeviceMesh)
+    d.h, d.num_verts, d.num_tets = None, m.num_verts, nt
+    for call in (lambda: d.set_positions(np.zeros((m.num_verts - 1, 3))),
+                 lambda: d.set_state_f64(x=np.zeros((m.num_verts + 1, 3))),
+                 lambda: d.set_materials(np.ones(nt), np.ones(nt + 1)),
+                 lambda: d.hessian_vector(np.zeros((2, 3))),
+                 lambda: d.adjoint_solve(np.zeros((m.num_verts, 3)), w0=np.zeros((1, 3))),
+                 lambda: d.adjoint_solve_f64(np.zeros((3, 3))),
+                 lambda: d.parameter_gradients_f64(np.zeros((m.num_verts - 1, 3)))):
+        with pytest.raises(ValueError):
+            call()

@@ -176,6 +176,62 @@ def _colors_to_csr(colors) -> tuple:
     return off, np.ascontiguousarray(verts, np.int32)
 
 
+def _mesh_arrays(positions, mass, is_fixed, tets, dminv, rest_volume, color_off, color_verts, inertia=None,
+                 velocity=None, tet_mu=None, tet_lambda=None) -> dict:
+    """Contiguous arrays of one mesh description, checked against each other's sizes: the C ABI
+    takes plain pointers and reads num_verts / num_tets entries from each, so a short array
+    would be read past its end."""
+    a = dict(
+        positions=np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+        mass=np.ascontiguousarray(mass, np.float32).reshape(-1),
+        is_fixed=np.ascontiguousarray(is_fixed, np.uint8).reshape(-1),
+        tets=np.ascontiguousarray(tets, np.int32).reshape(-1, 4),
+        dminv=np.ascontiguousarray(dminv, np.float32).reshape(-1, 9),
+        rest_volume=np.ascontiguousarray(rest_volume, np.float32).reshape(-1),
+        color_off=np.ascontiguousarray(color_off, np.int32).reshape(-1),
+        color_verts=np.ascontiguousarray(color_verts, np.int32).reshape(-1),
+    )
+    nv, nt = a["positions"].shape[0], a["tets"].shape[0]
+    for k, n in (("mass", nv), ("is_fixed", nv), ("color_verts", nv), ("dminv", nt), ("rest_volume", nt)):
+        if a[k].shape[0] != n:
+            raise ValueError(f"{k} has {a[k].shape[0]} entries, expected {n}")
+    if a["color_off"].size < 1:
+        raise ValueError("color_off needs num_colors + 1 >= 1 entries")
+    for k, v in (("inertia", inertia), ("velocity", velocity)):
+        if v is not None:
+            a[k] = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+            if a[k].shape[0] != nv:
+                raise ValueError(f"{k} has {a[k].shape[0]} rows, expected {nv}")
+    if (tet_mu is None) != (tet_lambda is None):
+        raise ValueError("tet_mu and tet_lambda must both be given or both be None")
+    if tet_mu is not None:
+        a["tet_mu"] = np.ascontiguousarray(tet_mu, np.float32).reshape(-1)
+        a["tet_lambda"] = np.ascontiguousarray(tet_lambda, np.float32).reshape(-1)
+        for k in ("tet_mu", "tet_lambda"):
+            if a[k].shape[0] != nt:
+                raise ValueError(f"{k} has {a[k].shape[0]} entries, expected {nt}")
+    return a
+
+
+def _fill_desc(d, a: dict, slot_order: int):
+    d.num_verts = a["positions"].shape[0]
+    d.num_tets = a["tets"].shape[0]
+    d.positions = ptr(a["positions"], C.c_float)
+    d.inertia = ptr(a.get("inertia"), C.c_float)
+    d.velocity = ptr(a.get("velocity"), C.c_float)
+    d.mass = ptr(a["mass"], C.c_float)
+    d.is_fixed = ptr(a["is_fixed"], C.c_uint8)
+    d.tets = ptr(a["tets"], C.c_int32)
+    d.dminv = ptr(a["dminv"], C.c_float)
+    d.rest_volume = ptr(a["rest_volume"], C.c_float)
+    d.tet_mu = ptr(a.get("tet_mu"), C.c_float)
+    d.tet_lambda = ptr(a.get("tet_lambda"), C.c_float)
+    d.num_colors = a["color_off"].size - 1
+    d.color_offsets = ptr(a["color_off"], C.c_int32)
+    d.color_verts = ptr(a["color_verts"], C.c_int32)
+    d.slot_order = int(slot_order or 0)
+
+
 class _Handle:
     def __init__(self, device: int = 0):
         h = C.c_void_p()
@@ -200,91 +256,35 @@ class DeviceMesh(_Handle):
     def __init__(self, device: int = 0):
         super().__init__(device)
         self.num_verts = 0
+        self.num_tets = 0
         self._keep = None
 
     # -- upload
     def init(self, positions, mass, is_fixed, tets, dminv, rest_volume, color_off, color_verts, inertia=None,
              velocity=None, tet_mu=None, tet_lambda=None, slot_order: int = 0):
-        arrs = dict(
-            positions=np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
-            mass=np.ascontiguousarray(mass, np.float32),
-            is_fixed=np.ascontiguousarray(is_fixed, np.uint8),
-            tets=np.ascontiguousarray(tets, np.int32).reshape(-1, 4),
-            dminv=np.ascontiguousarray(dminv, np.float32).reshape(-1, 9),
-            rest_volume=np.ascontiguousarray(rest_volume, np.float32),
-            color_off=np.ascontiguousarray(color_off, np.int32),
-            color_verts=np.ascontiguousarray(color_verts, np.int32),
-        )
-        if inertia is not None:
-            arrs["inertia"] = np.ascontiguousarray(inertia, np.float32).reshape(-1, 3)
-        if velocity is not None:
-            arrs["velocity"] = np.ascontiguousarray(velocity, np.float32).reshape(-1, 3)
-        if tet_mu is not None:
-            arrs["tet_mu"] = np.ascontiguousarray(tet_mu, np.float32)
-            arrs["tet_lambda"] = np.ascontiguousarray(tet_lambda, np.float32)
+        arrs = _mesh_arrays(positions, mass, is_fixed, tets, dminv, rest_volume, color_off, color_verts, inertia,
+                            velocity, tet_mu, tet_lambda)
         d = A.MeshDesc()
-        d.num_verts = arrs["positions"].shape[0]
-        d.num_tets = arrs["tets"].shape[0]
-        d.positions = ptr(arrs["positions"], C.c_float)
-        d.inertia = ptr(arrs.get("inertia"), C.c_float)
-        d.velocity = ptr(arrs.get("velocity"), C.c_float)
-        d.mass = ptr(arrs["mass"], C.c_float)
-        d.is_fixed = ptr(arrs["is_fixed"], C.c_uint8)
-        d.tets = ptr(arrs["tets"], C.c_int32)
-        d.dminv = ptr(arrs["dminv"], C.c_float)
-        d.rest_volume = ptr(arrs["rest_volume"], C.c_float)
-        d.tet_mu = ptr(arrs.get("tet_mu"), C.c_float)
-        d.tet_lambda = ptr(arrs.get("tet_lambda"), C.c_float)
-        d.num_colors = arrs["color_off"].size - 1
-        d.color_offsets = ptr(arrs["color_off"], C.c_int32)
-        d.color_verts = ptr(arrs["color_verts"], C.c_int32)
-        d.slot_order = int(slot_order)
+        _fill_desc(d, arrs, slot_order)
         check(lib.vbd_init(self.h, C.byref(d)), self.h)
         self.num_verts = d.num_verts
+        self.num_tets = d.num_tets
         return self
 
     def init_batch(self, descs: Sequence[dict]):
         """descs: per-mesh dicts with the keyword arguments of init() (local numbering)."""
         keep = []
         arr = (A.MeshDesc * len(descs))()
-        total = 0
+        nv = nt = 0
         for i, kw in enumerate(descs):
-            a = {k: (np.ascontiguousarray(v) if v is not None else None) for k, v in kw.items() if k != "slot_order"}
-            pos = np.ascontiguousarray(a["positions"], np.float32).reshape(-1, 3)
-            tets = np.ascontiguousarray(a["tets"], np.int32).reshape(-1, 4)
-            fields = dict(positions=pos, mass=np.ascontiguousarray(a["mass"], np.float32),
-                          is_fixed=np.ascontiguousarray(a["is_fixed"], np.uint8), tets=tets,
-                          dminv=np.ascontiguousarray(a["dminv"], np.float32).reshape(-1, 9),
-                          rest_volume=np.ascontiguousarray(a["rest_volume"], np.float32),
-                          color_off=np.ascontiguousarray(a["color_off"], np.int32),
-                          color_verts=np.ascontiguousarray(a["color_verts"], np.int32))
-            for opt in ("inertia", "velocity"):
-                if a.get(opt) is not None:
-                    fields[opt] = np.ascontiguousarray(a[opt], np.float32).reshape(-1, 3)
-            for opt in ("tet_mu", "tet_lambda"):
-                if a.get(opt) is not None:
-                    fields[opt] = np.ascontiguousarray(a[opt], np.float32)
+            fields = _mesh_arrays(**{k: v for k, v in kw.items() if k != "slot_order"})
             keep.append(fields)
-            d = arr[i]
-            d.num_verts = pos.shape[0]
-            d.num_tets = tets.shape[0]
-            d.positions = ptr(fields["positions"], C.c_float)
-            d.inertia = ptr(fields.get("inertia"), C.c_float)
-            d.velocity = ptr(fields.get("velocity"), C.c_float)
-            d.mass = ptr(fields["mass"], C.c_float)
-            d.is_fixed = ptr(fields["is_fixed"], C.c_uint8)
-            d.tets = ptr(fields["tets"], C.c_int32)
-            d.dminv = ptr(fields["dminv"], C.c_float)
-            d.rest_volume = ptr(fields["rest_volume"], C.c_float)
-            d.tet_mu = ptr(fields.get("tet_mu"), C.c_float)
-            d.tet_lambda = ptr(fields.get("tet_lambda"), C.c_float)
-            d.num_colors = fields["color_off"].size - 1
-            d.color_offsets = ptr(fields["color_off"], C.c_int32)
-            d.color_verts = ptr(fields["color_verts"], C.c_int32)
-            d.slot_order = int(kw.get("slot_order", 0) or 0)
-            total += pos.shape[0]
+            _fill_desc(arr[i], fields, kw.get("slot_order", 0))
+            nv += arr[i].num_verts
+            nt += arr[i].num_tets
         check(lib.vbd_init_batch(self.h, len(descs), arr), self.h)
-        self.num_verts = total
+        self.num_verts = nv
+        self.num_tets = nt
         return self
 
     def info(self) -> A.Info:
@@ -301,10 +301,22 @@ class DeviceMesh(_Handle):
         check(fn(self.h, ptr(out, C.c_float)), self.h)
         return out
 
-    def _up(self, fn, a):
-        a = np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+    def _vec3(self, a, dtype=np.float32) -> np.ndarray:
+        """(num_verts, 3) contiguous array of dtype; other row counts are rejected (the C ABI reads
+        num_verts rows)."""
+        a = np.ascontiguousarray(a, dtype).reshape(-1, 3)
         if a.shape[0] != self.num_verts:
-            raise ValueError("vertex count mismatch")
+            raise ValueError(f"expected {self.num_verts} rows of 3, got {a.shape[0]}")
+        return a
+
+    def _per_tet(self, a, name: str) -> np.ndarray:
+        a = np.ascontiguousarray(a, np.float32).reshape(-1)
+        if a.shape[0] != self.num_tets:
+            raise ValueError(f"{name} has {a.shape[0]} entries, expected {self.num_tets}")
+        return a
+
+    def _up(self, fn, a):
+        a = self._vec3(a)
         check(fn(self.h, ptr(a, C.c_float)), self.h)
 
     def positions(self, out=None) -> np.ndarray:
@@ -464,13 +476,13 @@ class DeviceMesh(_Handle):
 
     def set_state_f64(self, x=None, v=None):
         """Sets the FP64 state's positions and / or velocities (None keeps them)."""
-        xa = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1, 3)
-        va = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        xa = None if x is None else self._vec3(x, np.float64)
+        va = None if v is None else self._vec3(v, np.float64)
         check(lib.vbd_upload_state_f64(self.h, ptr(xa, C.c_double), ptr(va, C.c_double)), self.h)
 
     def set_materials(self, tet_mu, tet_lambda):
-        mu = np.ascontiguousarray(tet_mu, np.float32)
-        lam = np.ascontiguousarray(tet_lambda, np.float32)
+        mu = self._per_tet(tet_mu, "tet_mu")
+        lam = self._per_tet(tet_lambda, "tet_lambda")
         check(lib.vbd_set_materials(self.h, ptr(mu, C.c_float), ptr(lam, C.c_float)), self.h)
 
     def residual_norm(self):
@@ -488,7 +500,7 @@ class DeviceMesh(_Handle):
         return it.value, nrm.value
 
     def hessian_vector(self, w) -> np.ndarray:
-        w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+        w = self._vec3(w)
         out = np.zeros_like(w)
         check(lib.vbd_hessian_vector(self.h, ptr(w, C.c_float), ptr(out, C.c_float)), self.h)
         return out
@@ -496,8 +508,8 @@ class DeviceMesh(_Handle):
     def adjoint_solve(self, rhs, w0=None, max_iterations: int = 5000, check_every: int = 10, rel_tol: float = 1e-6,
                       rho: float = 0.0, stall_checks: int = 0):
         """Solve (M/h^2 + K) w = rhs; returns (w, sweeps, relative residual)."""
-        rhs = np.ascontiguousarray(rhs, np.float32).reshape(-1, 3)
-        w = np.zeros_like(rhs) if w0 is None else np.ascontiguousarray(w0, np.float32).reshape(-1, 3).copy()
+        rhs = self._vec3(rhs)
+        w = np.zeros_like(rhs) if w0 is None else self._vec3(w0).copy()
         it, rr = C.c_int32(), C.c_double()
         check(lib.vbd_adjoint_solve(self.h, ptr(rhs, C.c_float), ptr(w, C.c_float), max_iterations, check_every,
                                     rel_tol, rho, stall_checks, C.byref(it), C.byref(rr)), self.h)
@@ -505,7 +517,7 @@ class DeviceMesh(_Handle):
 
     def hessian_vector_f64(self, w) -> np.ndarray:
         """A w with double vectors (precision 64)."""
-        w = np.ascontiguousarray(w, np.float64).reshape(-1, 3)
+        w = self._vec3(w, np.float64)
         out = np.zeros_like(w)
         check(lib.vbd_hessian_vector_f64(self.h, ptr(w, C.c_double), ptr(out, C.c_double)), self.h)
         return out
@@ -513,8 +525,8 @@ class DeviceMesh(_Handle):
     def adjoint_solve_f64(self, rhs, w0=None, max_iterations: int = 5000, check_every: int = 10,
                           rel_tol: float = 1e-10, rho: float = 0.0, stall_checks: int = 0):
         """adjoint_solve with double vectors (precision 64); returns (w, sweeps, relative residual)."""
-        rhs = np.ascontiguousarray(rhs, np.float64).reshape(-1, 3)
-        w = np.zeros_like(rhs) if w0 is None else np.ascontiguousarray(w0, np.float64).reshape(-1, 3).copy()
+        rhs = self._vec3(rhs, np.float64)
+        w = np.zeros_like(rhs) if w0 is None else self._vec3(w0, np.float64).copy()
         it, rr = C.c_int32(), C.c_double()
         check(lib.vbd_adjoint_solve_f64(self.h, ptr(rhs, C.c_double), ptr(w, C.c_double), max_iterations, check_every,
                                         rel_tol, rho, stall_checks, C.byref(it), C.byref(rr)), self.h)
@@ -522,7 +534,7 @@ class DeviceMesh(_Handle):
 
     def parameter_gradients_f64(self, w):
         """parameter_gradients with double w and outputs (precision 64)."""
-        w = np.ascontiguousarray(w, np.float64).reshape(-1, 3)
+        w = self._vec3(w, np.float64)
         nt = self.info().num_tets
         dmu = np.zeros(max(nt, 1), np.float64)
         dlam = np.zeros(max(nt, 1), np.float64)
@@ -531,7 +543,7 @@ class DeviceMesh(_Handle):
         return dmu[:nt], dlam[:nt]
 
     def parameter_gradients(self, w):
-        w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+        w = self._vec3(w)
         nt = self.info().num_tets
         dmu = np.zeros(max(nt, 1), np.float32)
         dlam = np.zeros(max(nt, 1), np.float32)
