@@ -117,6 +117,20 @@ def test_diverging_chebyshev_falls_back_to_gauss_seidel(small):
         assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b)
 
 
+def test_unconverged_forward_steps_are_counted_and_warned(small):
+    """Precision 64: a forward step that ends above abs_tol (here: an iteration cap far too low)
+    is counted in the stats and warned about, since the adjoint assumes converged steps; a
+    converged rollout counts none."""
+    m, _, _, v0, _, _, _ = small
+    sim = DiffSim(m, steps=3, max_iterations=10)
+    with pytest.warns(RuntimeWarning, match="forward steps ended"):
+        sim.rollout(m.tet_mu, m.tet_lambda, v0)
+    assert sim.stats.unconverged_steps == 3
+    ok = DiffSim(m, steps=3)
+    ok.rollout(m.tet_mu, m.tet_lambda, v0)
+    assert ok.stats.unconverged_steps == 0 and max(ok.stats.forward_residuals) <= 1e-6
+
+
 def test_converged_step_matches_newton(small):
     m, _, _, v0, P, sim, _ = small
     sim.dev.set_materials(m.tet_mu, m.tet_lambda)

@@ -23,6 +23,7 @@ device step is bit-deterministic, so the recomputed states equal the forward one
 from __future__ import annotations
 
 import time
+import warnings
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
@@ -39,6 +40,7 @@ class RolloutStats:
     adjoint_sweeps: List[int] = field(default_factory=list)
     adjoint_residuals: List[float] = field(default_factory=list)
     adjoint_fallbacks: int = 0  # solves rerun without Chebyshev after diverging
+    unconverged_steps: int = 0  # precision 64: forward steps that ended above abs_tol
     forward_seconds: float = 0.0
     backward_seconds: float = 0.0
 
@@ -110,6 +112,13 @@ class DiffSim:
             it, res = self._step()
             self.stats.forward_iterations.append(it)
             self.stats.forward_residuals.append(res)
+            if self.precision == 64 and self.abs_tol > 0 and not res <= self.abs_tol:
+                self.stats.unconverged_steps += 1
+        if self.stats.unconverged_steps:
+            # the adjoint assumes grad G = 0 at every step: its gradients are then those of the
+            # states reached, not of converged steps (raise max_iterations, or lower rho)
+            warnings.warn(f"{self.stats.unconverged_steps} of {self.steps} forward steps ended with |grad G| above "
+                          f"{self.abs_tol:g} (largest {max(self.stats.forward_residuals):.3g})", RuntimeWarning)
         x = self._state()[0]
         self.stats.forward_seconds = time.perf_counter() - t0
         return x, ck

@@ -2020,7 +2020,11 @@ static int adjoint_solve_core(vbd_solver* h, const float* rhs, float* w, const d
         if (blowup || (stall && accel && r > 10.0 * best)) {
             if (iterations) *iterations = it;
             if (rel_residual) *rel_residual = r / bn;
-            return fail(h, VBD_ERR_NUMERIC, "adjoint solve diverged (Chebyshev rho too large); w is unchanged");
+            return fail(h, VBD_ERR_NUMERIC,
+                        accel ? "adjoint solve diverged (Chebyshev rho too large); w is unchanged"
+                              : "adjoint solve diverged without acceleration (M/h^2 + K is not positive definite at "
+                                "this state: inverted or degenerate elements, or a step that did not converge); w is "
+                                "unchanged");
         }
         if (first == 0.0) first = r;
         if (r < 0.99 * best) {
