@@ -534,10 +534,11 @@ def bench_hybrid_cpp(m, iterations: int, frames: int = 5):
 
 def bench_adjoint(n: int = 16, steps: int = 30):
     """Differentiable workload: 16^3 cube, per-tet mu field, lambda = 4 mu, 30 converged steps;
-    forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation +
-    colored block-GS adjoint + parameter gradients) wall time on the device.  Precision 64 (the
-    default: every step converged to |grad G| < 1e-6) and precision 32 (FP32 state, steps end on
-    the stagnation test near 5e-3 |M g|)."""
+    forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation + adjoint
+    solves + parameter gradients) wall time on the device.  Precision 64 (the default: every step
+    converged to |grad G| < 1e-6, block-Jacobi preconditioned conjugate-gradient adjoint) and
+    precision 32 (FP32 state, steps end on the stagnation test near 5e-3 |M g|; colored block
+    Gauss-Seidel adjoint with Chebyshev)."""
     from vbd_tetmesh_amd import meshgen
     from vbd_tetmesh_amd.adjoint import DiffSim
     m, mu_t, lam_t, v0 = meshgen.diff_workload(n, seed=0)
@@ -554,8 +555,10 @@ def bench_adjoint(n: int = 16, steps: int = 30):
             "adjoint_over_forward": round(st.backward_seconds / st.forward_seconds, 2),
             "forward_iterations_mean": round(float(np.mean(st.forward_iterations)), 1),
             "grad_G_max": float(max(st.forward_residuals)),
-            "adjoint_sweeps_mean": round(float(np.mean(st.adjoint_sweeps)), 1),
-            "adjoint_rel_tol": sim.adj_rel_tol, "adjoint_chebyshev_rho": sim.adj_rho,
+            "adjoint_method": sim.adj_method,
+            "adjoint_iterations_mean": round(float(np.mean(st.adjoint_sweeps)), 1),
+            "adjoint_rel_tol": sim.adj_rel_tol,
+            "adjoint_chebyshev_rho": sim.adj_rho if sim.adj_method == "gs" else None,
             "adjoint_fallbacks": st.adjoint_fallbacks,
             "forward_value": round(vi / st.forward_seconds / 1e6, 2), "unit": "M vertex-iters/s"}
     return {"workload": f"cube {n}^3 ({m.num_verts} verts, {m.num_tets} tets), per-tet mu/lambda, {steps} converged "

@@ -350,6 +350,15 @@ int vbd_hessian_vector_f64(vbd_solver* h, const double* w, double* out);
 int vbd_adjoint_solve_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations, int32_t check_every,
                           float rel_tol, float rho, int32_t stall_checks, int32_t* iterations, double* rel_residual);
 int vbd_parameter_gradients_f64(vbd_solver* h, const double* w, double* dmu, double* dlambda);
+/* The adjoint system (M/h^2 + K(x)) w = rhs of vbd_adjoint_solve_f64, solved by conjugate
+ * gradients with the block-Jacobi preconditioner (the inverse 3x3 diagonal blocks) instead of
+ * colored block Gauss-Seidel: no counterpart in the reference (it has no adjoint).  w holds the
+ * starting guess on entry; the solve stops when |rhs - A w| <= rel_tol |rhs| (tested every
+ * check_every iterations) or after max_iterations.  VBD_ERR_NUMERIC, with w unchanged, when an
+ * iteration finds p.Ap or r.z not positive (A not positive definite at this state) or the
+ * residual grows 1e3-fold.  Precision 64 only. */
+int vbd_adjoint_solve_cg_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations,
+                             int32_t check_every, float rel_tol, int32_t* iterations, double* rel_residual);
 
 #ifdef __cplusplus
 }

@@ -108,8 +108,8 @@ def test_diverging_chebyshev_falls_back_to_gauss_seidel(small):
     still returns the plain Gauss-Seidel gradients."""
     m, mu_t, lam_t, v0, _, _, _ = small
     x_t = m.rest.copy()
-    ref = DiffSim(m, steps=4, adj_rho=0.0).loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
-    sim = DiffSim(m, steps=4, adj_rho=0.9999)
+    ref = DiffSim(m, steps=4, adj_rho=0.0, adj_method="gs").loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    sim = DiffSim(m, steps=4, adj_rho=0.9999, adj_method="gs")
     got = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
     print(f"fallbacks {sim.stats.adjoint_fallbacks} of {len(sim.stats.adjoint_sweeps)} solves", flush=True)
     assert sim.stats.adjoint_fallbacks > 0
@@ -328,3 +328,68 @@ def test_fp64_kernels_match_the_fp64_oracle(small):
           f"sweeps residual {rr:.1e} vs direct {e_w:.2e}", flush=True)
     assert e_hv <= 1e-12 and relmax(dm, rm) <= 1e-12 and relmax(dl, rl) <= 1e-12
     assert rr <= 1e-12 and e_w <= 1e-9
+
+
+def test_conjugate_gradient_adjoint_matches_direct_solve_and_gauss_seidel(small):
+    """vbd_adjoint_solve_cg_f64 (block-Jacobi preconditioned CG on the FP64 state) against a
+    float64 direct solve of the oracle's system on the device's inputs (FP32 DmInv, volumes,
+    masses, h): solution within 1e-9 and recomputed residual within 1e-12; fewer iterations than
+    the Chebyshev Gauss-Seidel solve has sweeps, which reaches the same w; a zero right-hand side
+    returns the start unchanged; precision 32 is a state error."""
+    from vbd_tetmesh_amd import _capi as A
+    from vbd_tetmesh_amd.solver import prepare_host
+    m, _, _, _, _, _, x = small
+    t, _, _, dminv32, vol32 = prepare_host(m)
+    P = D.make_problem(m, h=float(np.float32(1.0 / 60.0)))
+    P.dminv = dminv32.astype(np.float64).reshape(-1, 3, 3)
+    P.vol = vol32.astype(np.float64)
+    P.mass = t.vertexMass.astype(np.float64)
+    D._prepare(P)
+    sim = DiffSim(m, steps=1, precision=64)
+    sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+    xd = x.astype(np.float64)
+    sim.dev.set_state_f64(xd, None)
+    mu, lam = m.tet_mu.astype(np.float64), m.tet_lambda.astype(np.float64)
+    rng = np.random.default_rng(12)
+    rhs = rng.normal(size=x.shape)
+    rhs[m.fixed == 1] = 0
+    fr = m.fixed == 0
+    wc, its, rr = sim.dev.adjoint_solve_cg_f64(rhs, rel_tol=1e-12, check_every=5)
+    A_ = D.system_matrix(P, xd, mu, lam)
+    wd = D.solve_linear(P, A_, rhs.ravel(), mu, lam).reshape(-1, 3)
+    e_w = relmax(wc[fr], wd[fr])
+    wg, sweeps, rg = sim.dev.adjoint_solve_f64(rhs, rel_tol=1e-12, rho=0.95)
+    print(f"cg: {its} iterations, residual {rr:.1e}, vs direct {e_w:.2e}; chebyshev gs: {sweeps} sweeps, "
+          f"cg vs gs {relmax(wc[fr], wg[fr]):.2e}", flush=True)
+    assert rr <= 1e-12 and e_w <= 1e-9 and relmax(wc[fr], wg[fr]) <= 1e-9
+    assert its < sweeps and its % 5 == 0
+    assert np.all(wc[m.fixed == 1] == 0)
+    # a start that already meets the tolerance is returned without an iteration
+    w2, its2, rr2 = sim.dev.adjoint_solve_cg_f64(rhs, w0=wc, rel_tol=1e-10, check_every=5)
+    assert its2 == 0 and rr2 <= 1e-10 and np.array_equal(w2, wc)
+    start = rng.normal(size=x.shape)
+    w0, its0, rr0 = sim.dev.adjoint_solve_cg_f64(np.zeros_like(rhs), w0=start)
+    assert its0 == 0 and rr0 == 0.0
+    s32 = DiffSim(m, steps=1, precision=32)
+    with pytest.raises(A.VBDError) as ei:
+        s32.dev.adjoint_solve_cg_f64(rhs)
+    assert ei.value.code == A.VBD_ERR_STATE
+
+
+def test_rollout_gradients_conjugate_gradient_equal_gauss_seidel(small):
+    """The 4^3 rollout's gradients with the CG adjoint (the precision 64 default) equal those of
+    the Chebyshev Gauss-Seidel adjoint to 1e-8 of their size (both solve to 1e-10)."""
+    m, _, _, v0, _, _, _ = small
+    x_t = m.rest.copy()
+    cg = DiffSim(m, steps=6, precision=64)
+    gs = DiffSim(m, steps=6, precision=64, adj_method="gs")
+    assert cg.adj_method == "cg" and gs.adj_method == "gs"
+    a = cg.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    b = gs.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
+    print(f"cg {np.mean(cg.stats.adjoint_sweeps):.1f} iterations per solve, gs {np.mean(gs.stats.adjoint_sweeps):.1f} "
+          f"sweeps; fallbacks {cg.stats.adjoint_fallbacks}", flush=True)
+    assert a[0] == b[0] and cg.stats.adjoint_fallbacks == 0
+    for ga, gb in zip(a[1:], b[1:]):
+        assert np.linalg.norm(ga - gb) <= 1e-8 * np.linalg.norm(gb)
+    with pytest.raises(ValueError):
+        DiffSim(m, steps=1, precision=32, adj_method="cg")

@@ -151,6 +151,7 @@ SIGNATURES = [
     ("vbd_adjoint_solve_f64", C.c_int, [H, f64p, f64p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, i32p,
                                         f64p]),
     ("vbd_parameter_gradients_f64", C.c_int, [H, f64p, f64p, f64p]),
+    ("vbd_adjoint_solve_cg_f64", C.c_int, [H, f64p, f64p, C.c_int32, C.c_int32, C.c_float, i32p, f64p]),
 ]
 
 

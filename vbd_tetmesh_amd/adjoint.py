@@ -13,7 +13,9 @@ unit cube) and end on the stagnation test.
 Backward, per step n (x_{n+1} = argmin G, v_{n+1} = c (x_{n+1} - x_n)/h, c = 0.999; with
 precision 64 the vectors cross the ABI in double, since the update of a cancels most of its terms):
     a_hat = a + (c/h) b
-    A(x_{n+1}) w = a_hat           colored block Gauss-Seidel on the device (vbd_adjoint_solve)
+    A(x_{n+1}) w = a_hat           on the device: block-Jacobi preconditioned conjugate gradients
+                                   with precision 64 (vbd_adjoint_solve_cg_f64), colored block
+                                   Gauss-Seidel with Chebyshev otherwise (vbd_adjoint_solve)
     mu_bar  += -V (F - C) : dF(w),  lambda_bar += -V (J - 1) C : dF(w)   (vbd_parameter_gradients)
     a <- (M/h^2) w - (c/h) b,  b <- (M/h) w
 and v0_bar = b at the end.  Positions and velocities are checkpointed every `checkpoint_every`
@@ -39,7 +41,7 @@ class RolloutStats:
     forward_residuals: List[float] = field(default_factory=list)
     adjoint_sweeps: List[int] = field(default_factory=list)
     adjoint_residuals: List[float] = field(default_factory=list)
-    adjoint_fallbacks: int = 0  # solves rerun without Chebyshev after diverging
+    adjoint_fallbacks: int = 0  # solves rerun with the next method after breaking down or diverging
     unconverged_steps: int = 0  # precision 64: forward steps that ended above abs_tol
     forward_seconds: float = 0.0
     backward_seconds: float = 0.0
@@ -52,7 +54,7 @@ class DiffSim:
                  check_every: int = 10, rel_tol: float = 0.0, stall_checks: int = 10, adj_max_iterations: int = 8000,
                  adj_rel_tol: Optional[float] = None, adj_rho: float = 0.95, adj_stall_checks: int = 5,
                  checkpoint_every: int = 5, device: int = 0, damping: float = 0.999, precision: int = 64,
-                 abs_tol: float = 1e-6):
+                 abs_tol: float = 1e-6, adj_method: str = "auto"):
         if mesh.tet_mu is None:
             raise ValueError("the differentiable rollout needs per-tet materials (mesh.tet_mu / tet_lambda)")
         t, off, verts, dminv, vol = prepare_host(mesh)
@@ -71,6 +73,14 @@ class DiffSim:
         self.max_iterations, self.check_every, self.rel_tol, self.abs_tol = max_iterations, check_every, rel_tol, abs_tol
         self.stall_checks, self.adj_stall_checks = stall_checks, adj_stall_checks
         self.adj_max_iterations, self.adj_rel_tol, self.adj_rho = adj_max_iterations, adj_rel_tol, adj_rho
+        # "cg": block-Jacobi preconditioned conjugate gradients (precision 64; about a third of the
+        # Chebyshev Gauss-Seidel solve's launches on the 16^3 workload); "gs": colored block
+        # Gauss-Seidel with Chebyshev; "auto": cg with precision 64, else gs
+        if adj_method not in ("auto", "cg", "gs"):
+            raise ValueError("adj_method must be 'auto', 'cg' or 'gs'")
+        if adj_method == "cg" and self.precision != 64:
+            raise ValueError("the conjugate-gradient adjoint solve needs precision 64")
+        self.adj_method = ("cg" if self.precision == 64 else "gs") if adj_method == "auto" else adj_method
         self.k = max(1, checkpoint_every)
         self.x0 = t.mVertPos.copy()
         self.mass = t.vertexMass.astype(np.float64)
@@ -172,7 +182,17 @@ class DiffSim:
         return gmu, glam, b
 
     def _adjoint_solve(self, rhs, w0):
-        """Chebyshev-accelerated solve; plain Gauss-Seidel (rho = 0) if the accelerated one diverges."""
+        """Conjugate gradients (adj_method "cg"), falling back to the Chebyshev-accelerated Gauss-
+        Seidel solve if they break down; that solve falls back to plain Gauss-Seidel (rho = 0) if it
+        diverges."""
+        if self.adj_method == "cg":
+            try:
+                return self.dev.adjoint_solve_cg_f64(rhs, w0=w0, max_iterations=self.adj_max_iterations,
+                                                     check_every=self.check_every, rel_tol=self.adj_rel_tol)
+            except A.VBDError as e:
+                if e.code != A.VBD_ERR_NUMERIC:
+                    raise
+                self.stats.adjoint_fallbacks += 1
         kw = dict(max_iterations=self.adj_max_iterations, check_every=self.check_every, rel_tol=self.adj_rel_tol,
                   stall_checks=self.adj_stall_checks)
         solve = self.dev.adjoint_solve_f64 if self.precision == 64 else self.dev.adjoint_solve

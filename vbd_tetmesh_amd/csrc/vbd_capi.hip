@@ -2054,6 +2054,96 @@ int vbd_adjoint_solve_f64(vbd_solver* h, const double* rhs, double* w, int32_t m
                               iterations, rel_residual);
 }
 
+// Conjugate gradients with the block-Jacobi preconditioner on the FP64 state (the adjoint system
+// is symmetric positive definite at a converged step).  Per iteration: A p with p.Ap per slot, its
+// fixed-order sum, the x / r / z update with r.z and r.r per slot, their sums, the new direction;
+// blocks of check_every iterations replay one cached graph (keyed by the block length and the
+// parity of its first iteration, which selects the r.z slots), and the host reads r.r and the
+// breakdown flag after each block.  The reported residual is recomputed from rhs - A w.
+int vbd_adjoint_solve_cg_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations,
+                             int32_t check_every, float rel_tol, int32_t* iterations, double* rel_residual) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!fp64(h)) return fail(h, VBD_ERR_STATE, "the FP64 vectors need vbd_set_differentiation_precision(h, 64)");
+    if (h->materialModel != VBD_MATERIAL_STABLE_NEO_HOOKEAN)
+        return fail(h, VBD_ERR_STATE, "differentiation supports the stable Neo-Hookean material only");
+    if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
+    if ((!w || !rhs) && h->m.nV) return fail(h, VBD_ERR_ARG, "null vector");
+    if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
+    if (!(rel_tol >= 0.f)) return fail(h, VBD_ERR_ARG, "rel_tol must be >= 0");
+    DeviceGuard g(h->device);
+    HIP_CHECK(h, ensure_fp64(h));
+    HIP_CHECK(h, alloc_cg64(h->d64, h->m.nV));
+    const ContactArgs ct = contact_of(h->params);
+    const float mu = step_mu(h), lam = step_lambda(h);
+    const double sdt64 = step_sdt64(h);
+    const int n = h->m.nFree;
+    Dev64& d = h->d64;
+    HIP_CHECK(h, upload_vec64d(h, rhs, d.rhs));
+    HIP_CHECK(h, upload_vec64d(h, w, d.w));
+    double b2 = 0.0;
+    HIP_CHECK(h, launch_norm2_d4(n, d.rhs, h->d_r2, h->stream));
+    HIP_CHECK(h, sum_r2(h, n, &b2));
+    const double bn = std::sqrt(b2);
+    int it = 0;
+    if (bn > 0 && n > 0) {
+        // the products A p read p at pinned neighbours: zero there (the iterations write free slots only)
+        if (h->m.nV > n)
+            HIP_CHECK(h, hipMemsetAsync(d.cg_p + n, 0, sizeof(d4) * (size_t)(h->m.nV - n), h->stream));
+        // D^-1, r = rhs - A w, z = D^-1 r, p = z, r.z
+        HIP_CHECK(h, launch_pass64(5, h->m, d, 0, n, nullptr, nullptr, d.cg_da, nullptr, d.cg_db, mu, lam, sdt64, ct,
+                                   h->stream));
+        HIP_CHECK(h, launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.cg_r, h->d_r2, nullptr, mu, lam, sdt64, ct,
+                                   h->stream));
+        double r02 = 0.0;
+        HIP_CHECK(h, sum_r2(h, n, &r02));
+        const double first = std::sqrt(r02);  // a start that already meets the tolerance is returned as is
+        HIP_CHECK(h, launch_cg_init64(n, d, h->d_r2, h->stream));
+        HIP_CHECK(h, launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_RZ, h->stream, r2_stage(h)));
+        while (it < max_iterations && first > rel_tol * bn) {
+            const int nb = std::min(check_every - it % check_every, max_iterations - it);
+            const int it0 = it;
+            HIP_CHECK(h, run_block(h, block_key('G', it0, std::vector<float>(nb, 1.0f), 0.f), [&]() -> hipError_t {
+                hipError_t e = hipSuccess;
+                for (int j = 0; j < nb && e == hipSuccess; ++j) {
+                    const int k = it0 + j, rzOld = CG_RZ + (k & 1), rzNew = CG_RZ + ((k + 1) & 1);
+                    e = launch_pass64(4, h->m, d, 0, n, d.cg_p, nullptr, d.out, h->d_r2, nullptr, mu, lam, sdt64, ct,
+                                      h->stream);
+                    if (e == hipSuccess) e = launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_PQ, h->stream, r2_stage(h));
+                    if (e == hipSuccess) e = launch_cg_step64(n, d, d.w, d.out, rzOld, h->d_r2, d.cg_part, h->stream);
+                    if (e == hipSuccess) e = launch_sum_fixed(n, h->d_r2, d.cg_scal + rzNew, h->stream, r2_stage(h));
+                    if (e == hipSuccess) e = launch_sum_fixed(n, d.cg_part, d.cg_scal + CG_RR, h->stream, d.cg_part + n);
+                    if (e == hipSuccess) e = launch_cg_dir64(n, d, rzOld, rzNew, h->stream);
+                }
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(h->h_res, d.cg_scal + CG_RR, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                       h->stream);
+                return e;
+            }));
+            it += nb;
+            const double rr = h->h_res[0], bad = h->h_res[1];  // CG_BAD follows CG_RR
+            const double r = std::sqrt(rr);
+            if (bad != 0.0 || !std::isfinite(r) || r > 1e3 * std::max(bn, first)) {
+                if (iterations) *iterations = it;
+                if (rel_residual) *rel_residual = r / bn;
+                return fail(h, VBD_ERR_NUMERIC,
+                            "conjugate gradients broke down (A is not positive definite at this state); w is unchanged");
+            }
+            if (r <= rel_tol * bn) break;
+        }
+    }
+    // the residual of the returned w, recomputed (the recurrence's r drifts from it in rounding)
+    double r2 = 0.0;
+    if (bn > 0 && n > 0) {
+        HIP_CHECK(h, launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.out, h->d_r2, nullptr, mu, lam, sdt64, ct, h->stream));
+        HIP_CHECK(h, sum_r2(h, n, &r2));
+    }
+    HIP_CHECK(h, download_vec64d(h, d.w, w));
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = bn > 0 ? std::sqrt(r2) / bn : 0.0;
+    return VBD_OK;
+}
+
 int vbd_parameter_gradients(vbd_solver* h, const float* w, float* dmu, float* dlambda) {
     int rc = check_ready(h);
     if (rc) return rc;

@@ -123,14 +123,18 @@ __device__ __forceinline__ double gsum64(double v) {
     return v;
 }
 
-enum Pass64 { P64_FWD_UPDATE = 0, P64_FWD_RESID = 1, P64_ADJ_SOLVE = 2, P64_ADJ_RESID = 3, P64_HVP = 4 };
+enum Pass64 {
+    P64_FWD_UPDATE = 0, P64_FWD_RESID = 1, P64_ADJ_SOLVE = 2, P64_ADJ_RESID = 3, P64_HVP = 4, P64_DIAG_INV = 5
+};
 
 // One pass over free slots [slot0, slot0 + count), LPV64 lanes per vertex.
 //   P64_FWD_UPDATE: one Gauss-Seidel vertex update of x (a color's launch); SAVE keeps x before it
 //   P64_FWD_RESID:  out = grad G = m/h^2 (x - y) + sum g - contact force, r2 = |out|^2
 //   P64_ADJ_SOLVE:  w_s += D_ss^-1 (rhs - A w)_s (a color's launch); SAVE keeps w before it
 //   P64_ADJ_RESID:  out = rhs - A w, r2 = |out|^2
-//   P64_HVP:        out = A w
+//   P64_HVP:        out = A w; with r2, r2 = w_s . (A w)_s
+//   P64_DIAG_INV:   D_ss^-1 of the diagonal block (the ADJ_SOLVE block), symmetric, as
+//                   out = (i00, i01, i02, i11), save = (i12, i22, 0, 0); zero when |det D| <= 1e-8
 template <int MODE, bool MAT, bool CONTACT, bool SAVE>
 __global__ __launch_bounds__(256) void k_pass64(int slot0, int count, d4* __restrict__ pos, const d4* __restrict__ inert,
                                                 const float* __restrict__ mass, const int* __restrict__ row,
@@ -141,7 +145,7 @@ __global__ __launch_bounds__(256) void k_pass64(int slot0, int count, d4* __rest
                                                 double* __restrict__ r2, d4* __restrict__ save, double mu, double lambda,
                                                 double sdt, ContactArgs ct) {
     constexpr bool NEED_W = MODE == P64_ADJ_SOLVE || MODE == P64_ADJ_RESID || MODE == P64_HVP;
-    constexpr bool NEED_BLOCK = MODE == P64_FWD_UPDATE || MODE == P64_ADJ_SOLVE;
+    constexpr bool NEED_BLOCK = MODE == P64_FWD_UPDATE || MODE == P64_ADJ_SOLVE || MODE == P64_DIAG_INV;
     constexpr bool NEED_FORCE = MODE == P64_FWD_UPDATE || MODE == P64_FWD_RESID;
     const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x) / LPV64;
     const int lane = (int)threadIdx.x % LPV64;
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(256) void k_pass64(int slot0, int count, d4* __rest
             const bool c0 = t.x < 0;
             const double V = (double)__int_as_float(t.w);
             const Kin64 k = kin64(c0, a, b, c, xs, pos[sa], pos[sb], pos[sc]);
-            if constexpr (NEED_FORCE) {
+            if constexpr (NEED_FORCE || MODE == P64_DIAG_INV) {
                 accumulate64(k, V, tmu, tlam, A);
             } else {
                 if constexpr (MODE == P64_ADJ_SOLVE) {
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(256) void k_pass64(int slot0, int count, d4* __rest
     double f0 = 0, f1 = 0, f2v = 0;
     if constexpr (NEED_FORCE) {
         f0 = gsum64(A.r0); f1 = gsum64(A.r1); f2v = gsum64(A.r2);
-    } else {
+    } else if constexpr (MODE != P64_DIAG_INV) {
         kw[0] = gsum64(kw[0]); kw[1] = gsum64(kw[1]); kw[2] = gsum64(kw[2]);
     }
     if (!valid || lane != LPV64 - 1) return;
@@ -209,8 +213,20 @@ __global__ __launch_bounds__(256) void k_pass64(int slot0, int count, d4* __rest
         if (cd > 0.0) ck = ct.k;
         else cd = 0.0;
     }
-    double e0, e1, e2;  // the right-hand side of the local system
-    if constexpr (NEED_FORCE) {
+    double e0 = 0, e1 = 0, e2 = 0;  // the right-hand side of the local system
+    if constexpr (MODE == P64_DIAG_INV) {
+        h00 += ms; h11 += ms; h22 += ms;
+        if (cax == 0) h00 += ck;
+        else if (cax == 1) h11 += ck;
+        else h22 += ck;
+        const double a00 = h11 * h22 - h12 * h12, a01 = h02 * h12 - h01 * h22, a02 = h01 * h12 - h02 * h11;
+        const double a11 = h00 * h22 - h02 * h02, a12 = h02 * h01 - h00 * h12, a22 = h00 * h11 - h01 * h01;
+        const double det = h00 * a00 + h01 * a01 + h02 * a02;
+        const double inv = fabs(det) > 1e-8 ? 1.0 / det : 0.0;  // skipped blocks as in ADJ_SOLVE
+        out[s] = d4{inv * a00, inv * a01, inv * a02, inv * a11};
+        save[s] = d4{inv * a12, inv * a22, 0.0, 0.0};
+        return;
+    } else if constexpr (NEED_FORCE) {
         const d4 ys = inert[s];
         e0 = ms * (ys.x - xs.x) + f0;
         e1 = ms * (ys.y - xs.y) + f1;
@@ -232,6 +248,7 @@ __global__ __launch_bounds__(256) void k_pass64(int slot0, int count, d4* __rest
         if constexpr (MODE == P64_HVP) {
             d4 o = {aw[0], aw[1], aw[2], 0.0};
             out[s] = o;
+            if (r2) r2[s] = ws.x * aw[0] + ws.y * aw[1] + ws.z * aw[2];
             return;
         }
         const d4 b4 = rhs[s];
@@ -293,6 +310,18 @@ static hipError_t pass64_m(const DevMesh& m, const Dev64& d, int slot0, int coun
     return hipErrorInvalidValue;
 }
 
+static hipError_t pass64_t_diag(const DevMesh& m, const Dev64& d, int slot0, int count, d4* out, d4* save2, double mu,
+                                double lambda, double sdt, const ContactArgs& ct, hipStream_t s) {
+    const bool mat = m.perTetMat, con = ct.on != 0;
+#define VBD_P64D(MT, CT) \
+    if (mat == MT && con == CT) \
+        return pass64_t<P64_DIAG_INV, MT, CT, false>(m, d, slot0, count, nullptr, nullptr, out, nullptr, save2, mu, \
+                                                     lambda, sdt, ct, s);
+    VBD_P64D(false, false) VBD_P64D(false, true) VBD_P64D(true, false) VBD_P64D(true, true)
+#undef VBD_P64D
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_pass64(int mode, const DevMesh& m, const Dev64& d, int slot0, int count, d4* w, const d4* rhs, d4* out,
                          double* r2, d4* save, double mu, double lambda, double sdt, const ContactArgs& ct,
                          hipStream_t s) {
@@ -304,6 +333,9 @@ hipError_t launch_pass64(int mode, const DevMesh& m, const Dev64& d, int slot0, 
         case P64_ADJ_SOLVE: return pass64_m<P64_ADJ_SOLVE>(m, d, slot0, count, w, rhs, out, r2, save, mu, lambda, sdt, ct, s);
         case P64_ADJ_RESID: return pass64_m<P64_ADJ_RESID>(m, d, slot0, count, w, rhs, out, r2, save, mu, lambda, sdt, ct, s);
         case P64_HVP: return pass64_m<P64_HVP>(m, d, slot0, count, w, rhs, out, r2, save, mu, lambda, sdt, ct, s);
+        case P64_DIAG_INV:
+            if (!out || !save) return hipErrorInvalidValue;
+            return pass64_t_diag(m, d, slot0, count, out, save, mu, lambda, sdt, ct, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -461,7 +493,93 @@ __global__ __launch_bounds__(256) void k_param_grad64(int nT, const i4* __restri
     dlam[t] = (T)(-(double)vol[t] * (J - 1.0) * cg);
 }
 
+// ---------------------------------------------------------------------------------------
+// Conjugate gradients on A = M/h^2 + K (+ contact) with the block-Jacobi preconditioner
+// D^-1 (the inverses of the 3x3 diagonal blocks, P64_DIAG_INV).  The scalars live on the device
+// (scal[CG_PQ] = p.Ap, scal[CG_RZ + 0/1] = r.z of alternate iterations, scal[CG_RR] = r.r,
+// scal[CG_BAD] set when p.Ap or r.z is not positive), so a block of iterations has no host
+// round trip; every dot product is a per-vertex product followed by the fixed-order sum.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ d4 dinv_apply(const d4 a, const d4 b, const d4 r) {
+    // [[a.x a.y a.z] [a.y a.w b.x] [a.z b.x b.y]] r
+    return d4{a.x * r.x + a.y * r.y + a.z * r.z, a.y * r.x + a.w * r.y + b.x * r.z,
+              a.z * r.x + b.x * r.y + b.y * r.z, 0.0};
+}
+
+__device__ __forceinline__ double dot3(const d4 a, const d4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// z = D^-1 r, p = z, part = r.z
+__global__ __launch_bounds__(256) void k_cg_init64(int n, const d4* __restrict__ r, d4* __restrict__ z,
+                                                   d4* __restrict__ p, const d4* __restrict__ da,
+                                                   const d4* __restrict__ db, double* __restrict__ part,
+                                                   double* __restrict__ scal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) scal[CG_BAD] = 0.0;
+    if (i >= n) return;
+    const d4 ri = r[i];
+    const d4 zi = dinv_apply(da[i], db[i], ri);
+    z[i] = zi;
+    p[i] = zi;
+    part[i] = dot3(ri, zi);
+}
+
+// alpha = r.z / p.Ap; x += alpha p; r -= alpha q; z = D^-1 r; part_rz = r.z, part_rr = r.r
+__global__ __launch_bounds__(256) void k_cg_step64(int n, d4* __restrict__ x, d4* __restrict__ r,
+                                                   const d4* __restrict__ p, const d4* __restrict__ q,
+                                                   d4* __restrict__ z, const d4* __restrict__ da,
+                                                   const d4* __restrict__ db, double* __restrict__ scal, int rzOld,
+                                                   double* __restrict__ partRz, double* __restrict__ partRr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double pq = scal[CG_PQ], rz = scal[rzOld];
+    if (i == 0 && !(pq > 0.0 && rz > 0.0)) scal[CG_BAD] = 1.0;
+    if (i >= n) return;
+    const double alpha = rz / pq;
+    const d4 pi = p[i], qi = q[i];
+    d4 xi = x[i], ri = r[i];
+    xi.x += alpha * pi.x; xi.y += alpha * pi.y; xi.z += alpha * pi.z;
+    ri.x -= alpha * qi.x; ri.y -= alpha * qi.y; ri.z -= alpha * qi.z;
+    x[i] = xi;
+    r[i] = ri;
+    const d4 zi = dinv_apply(da[i], db[i], ri);
+    z[i] = zi;
+    partRz[i] = dot3(ri, zi);
+    partRr[i] = dot3(ri, ri);
+}
+
+// beta = (r.z)_new / (r.z)_old; p = z + beta p
+__global__ __launch_bounds__(256) void k_cg_dir64(int n, d4* __restrict__ p, const d4* __restrict__ z,
+                                                  const double* __restrict__ scal, int rzOld, int rzNew) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double beta = scal[rzNew] / scal[rzOld];
+    const d4 zi = z[i];
+    d4 pi = p[i];
+    pi.x = zi.x + beta * pi.x;
+    pi.y = zi.y + beta * pi.y;
+    pi.z = zi.z + beta * pi.z;
+    p[i] = pi;
+}
+
 static inline dim3 grid256(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t launch_cg_init64(int n, const Dev64& d, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_init64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, d.cg_r, d.cg_z, d.cg_p, d.cg_da,
+                       d.cg_db, part, d.cg_scal);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_step64(int n, const Dev64& d, d4* x, const d4* q, int rzOld, double* partRz, double* partRr,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_step64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, x, d.cg_r, d.cg_p, q, d.cg_z, d.cg_da,
+                       d.cg_db, d.cg_scal, rzOld, partRz, partRr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_dir64(int n, const Dev64& d, int rzOld, int rzNew, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cg_dir64, grid256(n), dim3(256), 0, s, n, d.cg_p, d.cg_z, d.cg_scal, rzOld, rzNew);
+    return hipGetLastError();
+}
 
 hipError_t launch_predict64(const DevMesh& m, const Dev64& d, double h, const float g[3], hipStream_t s) {
     if (m.nV == 0) return hipSuccess;
@@ -545,6 +663,20 @@ hipError_t launch_param_grad64(const DevMesh& m, const Dev64& d, const d4* w, do
     return hipGetLastError();
 }
 
+hipError_t alloc_cg64(Dev64& d, int nV) {
+    const size_t n = (size_t)(nV > 0 ? nV : 1);
+    d4** arrs[] = {&d.cg_r, &d.cg_z, &d.cg_p, &d.cg_da, &d.cg_db};
+    for (d4** a : arrs) {
+        if (*a) continue;
+        hipError_t e = hipMalloc((void**)a, sizeof(d4) * n);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipSuccess;
+    if (!d.cg_part && (e = hipMalloc((void**)&d.cg_part, sizeof(double) * (n + SUM_STAGE))) != hipSuccess) return e;
+    if (!d.cg_scal && (e = hipMalloc((void**)&d.cg_scal, sizeof(double) * CG_SCALARS)) != hipSuccess) return e;
+    return e;
+}
+
 hipError_t alloc_dev64(Dev64& d, int nV) {
     const size_t n = (size_t)(nV > 0 ? nV : 1);
     d4** arrs[] = {&d.pos, &d.inert, &d.vel, &d.xprev, &d.save[0], &d.save[1],
@@ -558,9 +690,12 @@ hipError_t alloc_dev64(Dev64& d, int nV) {
 }
 
 void free_dev64(Dev64& d) {
-    d4* arrs[] = {d.pos, d.inert, d.vel, d.xprev, d.save[0], d.save[1], d.w, d.rhs, d.out, d.wsave[0], d.wsave[1]};
+    d4* arrs[] = {d.pos, d.inert, d.vel, d.xprev, d.save[0], d.save[1], d.w, d.rhs, d.out, d.wsave[0], d.wsave[1],
+                  d.cg_r, d.cg_z, d.cg_p, d.cg_da, d.cg_db};
     for (d4* a : arrs)
         if (a) (void)hipFree(a);
+    if (d.cg_part) (void)hipFree(d.cg_part);
+    if (d.cg_scal) (void)hipFree(d.cg_scal);
     d = Dev64();
 }
 

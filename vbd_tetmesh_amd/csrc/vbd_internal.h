@@ -205,11 +205,28 @@ struct Dev64 {
     d4* rhs = nullptr;
     d4* out = nullptr;
     d4* wsave[2] = {nullptr, nullptr};
+    // conjugate-gradient adjoint solve (allocated on first use): residual, preconditioned residual,
+    // direction, the block-Jacobi inverse (two double4 per slot), per-slot products, scalars
+    d4* cg_r = nullptr;
+    d4* cg_z = nullptr;
+    d4* cg_p = nullptr;
+    d4* cg_da = nullptr;
+    d4* cg_db = nullptr;
+    double* cg_part = nullptr;
+    double* cg_scal = nullptr;
 };
+// slots of Dev64::cg_scal
+constexpr int CG_PQ = 0, CG_RZ = 1, CG_RR = 3, CG_BAD = 4, CG_SCALARS = 8;
 hipError_t alloc_dev64(Dev64& d, int nV);
+hipError_t alloc_cg64(Dev64& d, int nV);
+hipError_t launch_cg_init64(int n, const Dev64& d, double* part, hipStream_t s);
+hipError_t launch_cg_step64(int n, const Dev64& d, d4* x, const d4* q, int rzOld, double* partRz, double* partRr,
+                            hipStream_t s);
+hipError_t launch_cg_dir64(int n, const Dev64& d, int rzOld, int rzNew, hipStream_t s);
 void free_dev64(Dev64& d);
 // mode: 0 Gauss-Seidel position update (one color), 1 forward residual grad G, 2 adjoint
-// block-GS update (one color), 3 adjoint residual rhs - A w, 4 A w; on free slots
+// block-GS update (one color), 3 adjoint residual rhs - A w, 4 A w (with r2: w . A w per slot),
+// 5 the inverse diagonal blocks (out, save: two double4 per slot); on free slots
 // [slot0, slot0 + count), reading the full pair records
 hipError_t launch_pass64(int mode, const DevMesh& m, const Dev64& d, int slot0, int count, d4* w, const d4* rhs, d4* out,
                          double* r2, d4* save, double mu, double lambda, double sdt, const ContactArgs& ct,

@@ -532,6 +532,17 @@ class DeviceMesh(_Handle):
                                         rel_tol, rho, stall_checks, C.byref(it), C.byref(rr)), self.h)
         return w, it.value, rr.value
 
+    def adjoint_solve_cg_f64(self, rhs, w0=None, max_iterations: int = 5000, check_every: int = 10,
+                             rel_tol: float = 1e-10):
+        """(M/h^2 + K) w = rhs by block-Jacobi preconditioned conjugate gradients on the FP64 state
+        (precision 64); returns (w, iterations, relative residual of the returned w)."""
+        rhs = self._vec3(rhs, np.float64)
+        w = np.zeros_like(rhs) if w0 is None else self._vec3(w0, np.float64).copy()
+        it, rr = C.c_int32(), C.c_double()
+        check(lib.vbd_adjoint_solve_cg_f64(self.h, ptr(rhs, C.c_double), ptr(w, C.c_double), max_iterations,
+                                           check_every, rel_tol, C.byref(it), C.byref(rr)), self.h)
+        return w, it.value, rr.value
+
     def parameter_gradients_f64(self, w):
         """parameter_gradients with double w and outputs (precision 64)."""
         w = self._vec3(w, np.float64)
