@@ -116,10 +116,25 @@ __device__ __forceinline__ void hvp64(const Kin64& k, const bool corner0, const 
     }
 }
 
-// sum over the aligned group of LPV64 lanes (a fixed butterfly: every lane of the group holds it)
+// Sum over the aligned group of LPV64 = 32 lanes, valid in the group's last lane (the only lane
+// that reads it): DPP suffix sums inside each row of 16 (row_shr 1, 2, 4, 8), then row 0's total
+// carried into row 1 and row 2's into row 3 (row_bcast:15), the two 32-bit halves of a double
+// moved separately.  Lanes whose DPP source lies outside the row receive 0.  A fixed order, and
+// no LDS round trip per step as a cross-lane shuffle takes.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_take64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double gsum64(double v) {
-#pragma unroll
-    for (int o = LPV64 / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    static_assert(LPV64 == 32, "the DPP tree sums groups of 32 lanes");
+    v += dpp_take64<0x111, 0xF>(v);  // row_shr:1
+    v += dpp_take64<0x112, 0xF>(v);  // row_shr:2
+    v += dpp_take64<0x114, 0xF>(v);  // row_shr:4
+    v += dpp_take64<0x118, 0xF>(v);  // row_shr:8
+    v += dpp_take64<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
     return v;
 }
 
