@@ -374,6 +374,18 @@ def test_conjugate_gradient_adjoint_matches_direct_solve_and_gauss_seidel(small)
     with pytest.raises(A.VBDError) as ei:
         s32.dev.adjoint_solve_cg_f64(rhs)
     assert ei.value.code == A.VBD_ERR_STATE
+    # with the penalty plane cutting the state (its k e e^T enters A's diagonal blocks and A p)
+    # both solves still reach the same w
+    import dataclasses
+    plane = float(np.quantile(xd[:, 1], 0.3))
+    sim.dev.set_params(dataclasses.replace(sim.params, contact=(1, plane, 1e5)))
+    sim.dev.set_state_f64(xd, None)
+    assert (xd[fr][:, 1] < plane).sum() > 0
+    wc2, its2c, rr2c = sim.dev.adjoint_solve_cg_f64(rhs, rel_tol=1e-12, check_every=5)
+    wg2, _, _ = sim.dev.adjoint_solve_f64(rhs, rel_tol=1e-12, rho=0.95)
+    print(f"cg with contact: {its2c} iterations, residual {rr2c:.1e}, vs gs {relmax(wc2[fr], wg2[fr]):.2e}, "
+          f"vs the contact-free w {relmax(wc2[fr], wc[fr]):.2e}", flush=True)
+    assert rr2c <= 1e-12 and relmax(wc2[fr], wg2[fr]) <= 1e-9 and relmax(wc2[fr], wc[fr]) > 1e-6
 
 
 def test_rollout_gradients_conjugate_gradient_equal_gauss_seidel(small):
