@@ -536,21 +536,24 @@ def bench_adjoint(n: int = 16, steps: int = 30):
     """Differentiable workload: 16^3 cube, per-tet mu field, lambda = 4 mu, 30 converged steps;
     forward (rollout with checkpoints every 5 steps) vs backward (segment recomputation + adjoint
     solves + parameter gradients) wall time on the device.  Precision 64 (the default: every step
-    converged to |grad G| < 1e-6, block-Jacobi preconditioned conjugate-gradient adjoint) and
+    converged to |grad G| < 1e-6, block-Jacobi preconditioned conjugate-gradient adjoint),
     precision 32 (FP32 state, steps end on the stagnation test near 5e-3 |M g|; colored block
-    Gauss-Seidel adjoint with Chebyshev)."""
+    Gauss-Seidel adjoint with Chebyshev), and precision 64 with the steps converged by Newton
+    steps with conjugate-gradient solves instead of VBD sweeps (fp64_newton; forward iterations
+    are then Newton steps, with the CG iterations inside them beside)."""
     from vbd_tetmesh_amd import meshgen
     from vbd_tetmesh_amd.adjoint import DiffSim
     m, mu_t, lam_t, v0 = meshgen.diff_workload(n, seed=0)
     x_t = m.rest.copy()
     out = {}
-    for prec in (64, 32):
-        sim = DiffSim(m, steps=steps, precision=prec)
+    for prec, fwd in ((64, "vbd"), (32, "vbd"), (64, "newton")):
+        sim = DiffSim(m, steps=steps, precision=prec, forward_method=fwd)
         sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)  # warm-up
         sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, x_t)
         st = sim.stats
         vi = float(m.num_verts) * sum(st.forward_iterations)
-        out[f"fp{prec}"] = {
+        out[f"fp{prec}" + ("_newton" if fwd == "newton" else "")] = {
+            "forward_method": fwd,
             "forward_s": round(st.forward_seconds, 4), "backward_s": round(st.backward_seconds, 4),
             "adjoint_over_forward": round(st.backward_seconds / st.forward_seconds, 2),
             "forward_iterations_mean": round(float(np.mean(st.forward_iterations)), 1),
@@ -560,7 +563,10 @@ def bench_adjoint(n: int = 16, steps: int = 30):
             "adjoint_rel_tol": sim.adj_rel_tol,
             "adjoint_chebyshev_rho": sim.adj_rho if sim.adj_method == "gs" else None,
             "adjoint_fallbacks": st.adjoint_fallbacks,
-            "forward_value": round(vi / st.forward_seconds / 1e6, 2), "unit": "M vertex-iters/s"}
+            "forward_inner_iterations_mean": (round(float(np.mean(st.forward_inner_iterations)), 1)
+                                              if st.forward_inner_iterations else None),
+            "forward_value": round(vi / st.forward_seconds / 1e6, 2) if fwd == "vbd" else None,
+            "unit": "M vertex-iters/s"}
     return {"workload": f"cube {n}^3 ({m.num_verts} verts, {m.num_tets} tets), per-tet mu/lambda, {steps} converged "
                         f"steps, checkpoint every 5, gradients w.r.t. mu, lambda, v0", **out}
 

@@ -360,6 +360,21 @@ int vbd_parameter_gradients_f64(vbd_solver* h, const double* w, double* dmu, dou
 int vbd_adjoint_solve_cg_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations,
                              int32_t check_every, float rel_tol, int32_t* iterations, double* rel_residual);
 
+/* Method of vbd_step_converged (no counterpart in the reference, whose step runs a fixed number
+ * of VBD iterations): VBD_CONVERGED_VBD (default) iterates VBD sweeps with Chebyshev;
+ * VBD_CONVERGED_NEWTON (precision 64 only) takes Newton steps on the incremental potential G,
+ * each solving (M/h^2 + K) dx = -grad G by block-Jacobi preconditioned conjugate gradients to
+ * 1e-2 |grad G| and halving the step (at most 4 times) until |grad G| falls; a breakdown, a
+ * failed line search or 20 steps hand the state to the VBD sweeps.  The converged state is the
+ * same minimiser of G to the step's tolerance.  With Newton, vbd_step_converged's iterations
+ * output counts Newton steps plus any VBD sweeps. */
+#define VBD_CONVERGED_VBD 0
+#define VBD_CONVERGED_NEWTON 1
+int vbd_set_converged_method(vbd_solver* h, int32_t method);
+/* Work of the last vbd_step_converged: Newton steps, conjugate-gradient iterations inside them,
+ * VBD sweeps (any of the pointers may be NULL). */
+int vbd_last_step_stats(const vbd_solver* h, int32_t* newton_steps, int32_t* inner_iterations, int32_t* sweeps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -148,12 +148,53 @@ def test_converged_step_matches_newton(small):
     assert np.abs(sim.dev.velocities() - v64).max() <= 1e-3 * np.abs(v64).max() + 1e-6
 
 
-def test_checkpoint_recomputation_is_bitwise(small):
-    """A segment recomputed from its checkpoint reproduces the forward states exactly."""
+def test_newton_converged_step_matches_vbd_and_the_fp64_newton_oracle(small):
+    """vbd_step_converged with the Newton method (precision 64) reaches the same minimiser of G as
+    the VBD sweeps (both to |grad G| < 1e-6) and as the FP64 oracle's Newton step; the stats
+    count its Newton steps and CG iterations; precision 32 refuses the method."""
+    from vbd_tetmesh_amd import _capi as A
+    m, _, _, v0, P, _, _ = small
+    out = {}
+    for method in (A.VBD_CONVERGED_VBD, A.VBD_CONVERGED_NEWTON):
+        sim = DiffSim(m, steps=1, precision=64)
+        sim.dev.set_converged_method(method)
+        sim.dev.set_materials(m.tet_mu, m.tet_lambda)
+        sim.dev.set_positions(m.rest)
+        sim.dev.set_velocities(v0)
+        it, res = sim._step()
+        x, _ = sim.dev.state_f64()
+        out[method] = (x, it, res, sim.dev.last_step_stats())
+    x64, _, _ = D.solve_step(P, m.rest.astype(np.float64), v0.astype(np.float64), m.tet_mu.astype(np.float64),
+                             m.tet_lambda.astype(np.float64))
+    disp = np.abs(x64 - m.rest).max()
+    xv, itv, resv, sv = out[A.VBD_CONVERGED_VBD]
+    xn, itn, resn, sn = out[A.VBD_CONVERGED_NEWTON]
+    print(f"converged step: vbd {itv} sweeps |grad G| {resv:.2e}; newton {sn[0]} steps, {sn[1]} CG iterations, "
+          f"{sn[2]} sweeps, |grad G| {resn:.2e}; newton vs vbd {np.abs(xn - xv).max() / disp:.2e}, vs the FP64 oracle "
+          f"{np.abs(xn - x64).max() / disp:.2e} of max displacement", flush=True)
+    assert resv < 1e-6 and resn < 1e-6
+    assert sv == (0, 0, itv) and sn[0] >= 1 and sn[1] > 0 and sn[2] == 0 and itn == sn[0]
+    assert np.abs(xn - xv).max() <= 1e-6 * disp and np.abs(xn - x64).max() <= 1e-5 * disp
+    s32 = DiffSim(m, steps=1, precision=32)
+    s32.dev.set_converged_method(A.VBD_CONVERGED_NEWTON)
+    with pytest.raises(A.VBDError) as ei:
+        s32._step()
+    assert ei.value.code == A.VBD_ERR_STATE
+    with pytest.raises(A.VBDError) as ei:
+        s32.dev.set_converged_method(2)
+    assert ei.value.code == A.VBD_ERR_ARG
+    with pytest.raises(ValueError):
+        DiffSim(m, steps=1, precision=32, forward_method="newton")
+
+
+@pytest.mark.parametrize("forward", ["vbd", "newton"])
+def test_checkpoint_recomputation_is_bitwise(small, forward):
+    """A segment recomputed from its checkpoint reproduces the forward states exactly (both
+    converged-step methods: every reduction has a fixed order)."""
     m, _, _, v0, _, _, _ = small
-    sim = DiffSim(m, steps=6, checkpoint_every=3)
+    sim = DiffSim(m, steps=6, checkpoint_every=3, forward_method=forward)
     x, ck = sim.rollout(m.tet_mu, m.tet_lambda, v0)
-    every = DiffSim(m, steps=6, checkpoint_every=1)
+    every = DiffSim(m, steps=6, checkpoint_every=1, forward_method=forward)
     _, ck1 = every.rollout(m.tet_mu, m.tet_lambda, v0)
     xs = sim._segment_states(3, 6, ck)
     for j, n in enumerate(range(3, 6)):
@@ -207,8 +248,8 @@ def test_torch_autograd_function(small):
 
 @pytest.mark.skipif(not os.path.exists(GOLDEN), reason="FP64 golden fixture not generated")
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("precision", [64, 32])
-def test_diffcube16_gradients_match_fp64_and_finite_differences(precision):
+@pytest.mark.parametrize("precision,forward", [(64, "vbd"), (32, "vbd"), (64, "newton")])
+def test_diffcube16_gradients_match_fp64_and_finite_differences(precision, forward):
     """Differentiable workload: 16^3 cube (4913 verts, 24576 tets), per-tet mu field, lambda = 4 mu,
     30 converged steps, loss |x_30 - x_target|^2.  GPU adjoint vs the committed FP64 CPU adjoint
     and its central finite differences (step 1e-3 of the parameter, of max |v0| for v0) on 32 seeded
@@ -218,10 +259,12 @@ def test_diffcube16_gradients_match_fp64_and_finite_differences(precision):
     |g_gpu - g_fd| / |g_fd| <= 1e-3 and |g_gpu - g_adjoint| / |g_adjoint| <= 1e-5.
     FP32 state (steps end on the stagnation test near 5e-3 |M g|): per parameter
     |g_gpu - g_ref| <= 1e-3 * max|g_ref| of that parameter family.
-    Both: loss within 1e-3 and each whole gradient vector within 1e-2, relative."""
+    Both: loss within 1e-3 and each whole gradient vector within 1e-2, relative.  The FP64 state
+    runs twice: converged by VBD sweeps and by Newton steps with conjugate-gradient solves (the
+    same tolerances)."""
     z = np.load(GOLDEN)
     m, mu_t, lam_t, v0 = meshgen.diff_workload(int(z["n"]), seed=0)
-    sim = DiffSim(m, steps=int(z["steps"]), precision=precision)
+    sim = DiffSim(m, steps=int(z["steps"]), precision=precision, forward_method=forward)
     L, gmu, glam, gv0 = sim.loss_and_grads(m.tet_mu, m.tet_lambda, v0, z["x_target"])
     fwd, bwd = sim.stats.forward_seconds, sim.stats.backward_seconds
     ref = {"mu": z["grad_mu"].astype(np.float64), "lambda": z["grad_lambda"].astype(np.float64),
@@ -250,7 +293,11 @@ def test_diffcube16_gradients_match_fp64_and_finite_differences(precision):
     ar = np.asarray(sim.stats.adjoint_residuals)
     print(f"  adjoint solves: relative residual max {ar.max():.2e} median {np.median(ar):.2e}, sweeps "
           f"{sim.stats.adjoint_sweeps[:5]}..., fallbacks {sim.stats.adjoint_fallbacks}", flush=True)
-    print(f"diffcube16, precision {precision}: loss {L:.6e} vs {float(z['loss']):.6e}; vector rel errors {vec}; "
+    if forward == "newton":
+        inner = np.asarray(sim.stats.forward_inner_iterations)
+        print(f"  newton: {np.mean(sim.stats.forward_iterations):.2f} steps and {inner.mean():.1f} CG iterations per "
+              f"converged step", flush=True)
+    print(f"diffcube16, precision {precision}, {forward}: loss {L:.6e} vs {float(z['loss']):.6e}; vector rel errors {vec}; "
           f"32 params: max |gpu - fd| {worst_fd:.2e}, |gpu - fp64 adjoint| {worst_adj:.2e} (of family max); "
           f"per parameter max |gpu - fd| / |fd| {worst_rel_fd:.2e}, |gpu - adjoint| / |adjoint| {worst_rel_adj:.2e}; "
           f"forward {fwd:.2f}s backward {bwd:.2f}s ratio {bwd / fwd:.2f}", flush=True)

@@ -50,6 +50,8 @@ class MeshDesc(C.Structure):
 
 VBD_MATERIAL_STABLE_NEO_HOOKEAN = 0
 VBD_MATERIAL_COROTATED_GPU = 1
+VBD_CONVERGED_VBD = 0
+VBD_CONVERGED_NEWTON = 1
 
 
 class StepParams(C.Structure):
@@ -152,6 +154,8 @@ SIGNATURES = [
                                         f64p]),
     ("vbd_parameter_gradients_f64", C.c_int, [H, f64p, f64p, f64p]),
     ("vbd_adjoint_solve_cg_f64", C.c_int, [H, f64p, f64p, C.c_int32, C.c_int32, C.c_float, i32p, f64p]),
+    ("vbd_set_converged_method", C.c_int, [H, C.c_int32]),
+    ("vbd_last_step_stats", C.c_int, [H, i32p, i32p, i32p]),
 ]
 
 

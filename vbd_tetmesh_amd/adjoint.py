@@ -1,7 +1,8 @@
 """Differentiable rollouts of converged VBD steps (implicit differentiation on the device).
 
 Forward: `steps` implicit-Euler steps, each iterated on the GPU until
-|grad G| <= max(abs_tol, rel_tol |M g|) (vbd_step_converged; by default |grad G| < 1e-6), from
+|grad G| <= max(abs_tol, rel_tol |M g|) (vbd_step_converged; by default |grad G| < 1e-6) by VBD
+sweeps, or (forward_method="newton", precision 64) by Newton steps with conjugate-gradient solves, from
 x_0 = rest with initial velocity v_0 and per-tet materials (mu, lambda).  The reference has no
 differentiation; the step itself follows VBDsolver::step (VBD.cpp:60-101, TetMesh.h:305-348).
 
@@ -43,6 +44,7 @@ class RolloutStats:
     adjoint_residuals: List[float] = field(default_factory=list)
     adjoint_fallbacks: int = 0  # solves rerun with the next method after breaking down or diverging
     unconverged_steps: int = 0  # precision 64: forward steps that ended above abs_tol
+    forward_inner_iterations: List[int] = field(default_factory=list)  # Newton: CG iterations per step
     forward_seconds: float = 0.0
     backward_seconds: float = 0.0
 
@@ -54,7 +56,7 @@ class DiffSim:
                  check_every: int = 10, rel_tol: float = 0.0, stall_checks: int = 10, adj_max_iterations: int = 8000,
                  adj_rel_tol: Optional[float] = None, adj_rho: float = 0.95, adj_stall_checks: int = 5,
                  checkpoint_every: int = 5, device: int = 0, damping: float = 0.999, precision: int = 64,
-                 abs_tol: float = 1e-6, adj_method: str = "auto"):
+                 abs_tol: float = 1e-6, adj_method: str = "auto", forward_method: str = "vbd"):
         if mesh.tet_mu is None:
             raise ValueError("the differentiable rollout needs per-tet materials (mesh.tet_mu / tet_lambda)")
         t, off, verts, dminv, vol = prepare_host(mesh)
@@ -81,6 +83,15 @@ class DiffSim:
         if adj_method == "cg" and self.precision != 64:
             raise ValueError("the conjugate-gradient adjoint solve needs precision 64")
         self.adj_method = ("cg" if self.precision == 64 else "gs") if adj_method == "auto" else adj_method
+        # converged steps by VBD sweeps ("vbd") or by Newton steps with conjugate-gradient solves
+        # ("newton", precision 64: the same minimiser of G, about a third of the launches on the
+        # 16^3 workload)
+        if forward_method not in ("vbd", "newton"):
+            raise ValueError("forward_method must be 'vbd' or 'newton'")
+        if forward_method == "newton" and self.precision != 64:
+            raise ValueError("the Newton forward needs precision 64")
+        self.forward_method = forward_method
+        self.dev.set_converged_method(A.VBD_CONVERGED_NEWTON if forward_method == "newton" else A.VBD_CONVERGED_VBD)
         self.k = max(1, checkpoint_every)
         self.x0 = t.mVertPos.copy()
         self.mass = t.vertexMass.astype(np.float64)
@@ -122,6 +133,8 @@ class DiffSim:
             it, res = self._step()
             self.stats.forward_iterations.append(it)
             self.stats.forward_residuals.append(res)
+            if self.forward_method == "newton":
+                self.stats.forward_inner_iterations.append(self.dev.last_step_stats()[1])
             if self.precision == 64 and self.abs_tol > 0 and not res <= self.abs_tol:
                 self.stats.unconverged_steps += 1
         if self.stats.unconverged_steps:

@@ -67,6 +67,8 @@ struct vbd_solver {
     int persistent = 0;          // 1 or a grid cap (8k): one persistent launch per sweep (tiled stable Neo-Hookean)
     unsigned* d_bar = nullptr;   // its grid barrier (PERSIST_BAR_WORDS words: counts, generation, timeout flag)
     int diffBits = 32;
+    int convergedMethod = VBD_CONVERGED_VBD;  // vbd_set_converged_method
+    int lastNewton = 0, lastInner = 0, lastSweeps = 0;  // the last converged step's work (vbd_last_step_stats)
     Dev64 d64;
     bool fp64Fresh = false;
     double* d_x3d = nullptr;   // 3 * nV staging of FP64 state transfers
@@ -1817,6 +1819,15 @@ int vbd_residual_norm(vbd_solver* h, double* norm, double* gravity_load) {
     return VBD_OK;
 }
 
+static hipError_t cg64_run(vbd_solver* h, double bn, int max_it, int check_every, double tol, int* iters,
+                           bool* broke);
+
+// Newton method of the converged step: CG tolerance relative to |grad G| (a CPU prototype on the
+// 16^3 workload: 1e-1 -> 7.7 Newton steps and 190 CG iterations per step, 1e-2 -> 4.1 and 196,
+// 1e-3 -> 3.0 and 208), and the steps before it hands the state to VBD sweeps.
+constexpr double NEWTON_ETA = 1e-2;
+constexpr int NEWTON_MAX_STEPS = 20;
+
 int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_every, float abs_tol, float rel_tol,
                        int32_t stall_checks, int32_t* iterations, double* final_norm) {
     int rc = check_ready(h);
@@ -1826,6 +1837,8 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
     if (!h->paramsSet) return fail(h, VBD_ERR_STATE, "vbd_set_params has not been called");
     if (max_iterations < 1 || check_every < 1) return fail(h, VBD_ERR_ARG, "max_iterations and check_every must be >= 1");
     if (h->params.substeps != 1) return fail(h, VBD_ERR_ARG, "converged steps support substeps == 1");
+    if (h->convergedMethod == VBD_CONVERGED_NEWTON && !fp64(h))
+        return fail(h, VBD_ERR_STATE, "the Newton method of the converged step needs precision 64");
     DeviceGuard g(h->device);
     const bool f64 = fp64(h);
     HIP_CHECK(h, f64 ? ensure_fp64(h) : ensure_scratch(h));
@@ -1846,12 +1859,74 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
         HIP_CHECK(h, launch_predict(h->m, hstep, p.gravity, h->stream));
         if (p.start_from_inertia) HIP_CHECK(h, launch_copy_inertia(h->m, h->stream));
     }
+    double r2 = 0;
+    int newton = 0, inner = 0;
+    bool sweeps = true;
+    if (h->convergedMethod == VBD_CONVERGED_NEWTON) {
+        // Newton on G: (M/h^2 + K(x)) dx = -grad G by block-Jacobi CG to NEWTON_ETA |grad G|, then
+        // x += alpha dx with alpha halved (at most 4 times) while |grad G| does not fall.  A CG
+        // breakdown, a failed line search or NEWTON_MAX_STEPS steps hand the state to the VBD
+        // sweeps below.
+        const int n = h->m.nFree;
+        Dev64& d = h->d64;
+        auto residual = [&](double* r) -> hipError_t {
+            hipError_t e = launch_pass64(1, h->m, d, 0, n, nullptr, nullptr, d.out, h->d_r2, nullptr, p.mu, p.lambda,
+                                         sdt64, ct, h->stream);
+            double q = 0.0;
+            if (e == hipSuccess) e = sum_r2(h, n, &q);
+            *r = std::sqrt(q);
+            return e;
+        };
+        double r = 0.0;
+        HIP_CHECK(h, residual(&r));
+        sweeps = false;
+        while (r > tol && std::isfinite(r)) {
+            if (newton >= NEWTON_MAX_STEPS) {
+                sweeps = true;
+                break;
+            }
+            HIP_CHECK(h, launch_scale_d4(n, d.out, d.rhs, -1.0, h->stream));
+            HIP_CHECK(h, hipMemsetAsync(d.w, 0, sizeof(d4) * (size_t)h->m.nV, h->stream));
+            int its = 0;
+            bool broke = false;
+            HIP_CHECK(h, cg64_run(h, r, max_iterations, check_every, NEWTON_ETA * r, &its, &broke));
+            inner += its;
+            ++newton;
+            if (broke) {
+                sweeps = true;
+                break;
+            }
+            HIP_CHECK(h, launch_copy_d4(n, d.pos, d.save[0], h->stream));
+            double alpha = 1.0, rn = 0.0;
+            bool ok = false;
+            for (int ls = 0; ls < 5 && !ok; ++ls, alpha *= 0.5) {
+                HIP_CHECK(h, launch_newton_move64(n, d.pos, d.save[0], d.w, alpha, h->stream));
+                HIP_CHECK(h, residual(&rn));
+                ok = std::isfinite(rn) && rn < r;
+            }
+            if (!ok) {  // back to the start of this Newton step
+                HIP_CHECK(h, launch_copy_d4(n, d.save[0], d.pos, h->stream));
+                sweeps = true;
+                break;
+            }
+            r = rn;
+        }
+        if (!std::isfinite(r)) {
+            h->lastNewton = newton;
+            h->lastInner = inner;
+            h->lastSweeps = 0;
+            if (iterations) *iterations = newton;
+            if (final_norm) *final_norm = r;
+            return fail(h, VBD_ERR_NUMERIC, "converged step: the residual became non-finite");
+        }
+        r2 = r * r;
+    }
     float omega = 1.0f;
     int it = 0, stalled = 0;
-    double r2 = 0, best = 1e300;
+    double best = 1e300;
     const bool accel = p.use_acceleration != 0;
     std::vector<float> om;
-    while (it < max_iterations) {
+    while (sweeps && it < max_iterations) {
         // one block: the iterations up to the next check, then the residual
         const int n = std::min(check_every - it % check_every, max_iterations - it);
         om.assign(n, 1.0f);
@@ -1906,8 +1981,27 @@ int vbd_step_converged(vbd_solver* h, int32_t max_iterations, int32_t check_ever
         HIP_CHECK(h, launch_velocity(h->m, hstep, p.velocity_damping, h->stream));
     }
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
-    if (iterations) *iterations = it;
+    h->lastNewton = newton;
+    h->lastInner = inner;
+    h->lastSweeps = it;
+    if (iterations) *iterations = newton + it;
     if (final_norm) *final_norm = std::sqrt(r2);
+    return VBD_OK;
+}
+
+int vbd_set_converged_method(vbd_solver* h, int32_t method) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (method != VBD_CONVERGED_VBD && method != VBD_CONVERGED_NEWTON)
+        return fail(h, VBD_ERR_ARG, "method must be VBD_CONVERGED_VBD or VBD_CONVERGED_NEWTON");
+    h->convergedMethod = method;
+    return VBD_OK;
+}
+
+int vbd_last_step_stats(const vbd_solver* h, int32_t* newton_steps, int32_t* inner_iterations, int32_t* sweeps) {
+    if (!h) return fail(nullptr, VBD_ERR_ARG, "null handle");
+    if (newton_steps) *newton_steps = h->lastNewton;
+    if (inner_iterations) *inner_iterations = h->lastInner;
+    if (sweeps) *sweeps = h->lastSweeps;
     return VBD_OK;
 }
 
@@ -2054,12 +2148,75 @@ int vbd_adjoint_solve_f64(vbd_solver* h, const double* rhs, double* w, int32_t m
                               iterations, rel_residual);
 }
 
-// Conjugate gradients with the block-Jacobi preconditioner on the FP64 state (the adjoint system
-// is symmetric positive definite at a converged step).  Per iteration: A p with p.Ap per slot, its
-// fixed-order sum, the x / r / z update with r.z and r.r per slot, their sums, the new direction;
-// blocks of check_every iterations replay one cached graph (keyed by the block length and the
-// parity of its first iteration, which selects the r.z slots), and the host reads r.r and the
-// breakdown flag after each block.  The reported residual is recomputed from rhs - A w.
+// Conjugate gradients with the block-Jacobi preconditioner on A = M/h^2 + K (+ contact) at the
+// FP64 state: rhs in d64.rhs (norm bn), x in d64.w (its start on entry).  Per iteration: A p with
+// p.Ap per slot, its fixed-order sum, the x / r / z update with r.z and r.r per slot, their sums,
+// the new direction; blocks of check_every iterations replay one cached graph (keyed by the block
+// length and the parity of its first iteration, which selects the r.z slots), and the host reads
+// r.r and the breakdown flag after each block.  Stops when |r| <= tol or after max_it iterations;
+// *broke when p.Ap or r.z was not positive (A not positive definite) or |r| grew 1e3-fold.
+static hipError_t cg64_run(vbd_solver* h, double bn, int max_it, int check_every, double tol, int* iters,
+                           bool* broke) {
+    const ContactArgs ct = contact_of(h->params);
+    const float mu = step_mu(h), lam = step_lambda(h);
+    const double sdt64 = step_sdt64(h);
+    const int n = h->m.nFree;
+    Dev64& d = h->d64;
+    *iters = 0;
+    *broke = false;
+    if (!(bn > 0) || n == 0) return hipSuccess;
+    hipError_t e = alloc_cg64(d, h->m.nV);
+    // the products A p read p at pinned neighbours: zero there (the iterations write free slots only)
+    if (e == hipSuccess && h->m.nV > n)
+        e = hipMemsetAsync(d.cg_p + n, 0, sizeof(d4) * (size_t)(h->m.nV - n), h->stream);
+    // D^-1, r = rhs - A x, z = D^-1 r, p = z, r.z
+    if (e == hipSuccess)
+        e = launch_pass64(5, h->m, d, 0, n, nullptr, nullptr, d.cg_da, nullptr, d.cg_db, mu, lam, sdt64, ct, h->stream);
+    if (e == hipSuccess)
+        e = launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.cg_r, h->d_r2, nullptr, mu, lam, sdt64, ct, h->stream);
+    double r02 = 0.0;
+    if (e == hipSuccess) e = sum_r2(h, n, &r02);
+    if (e != hipSuccess) return e;
+    const double first = std::sqrt(r02);  // a start that already meets the tolerance is returned as is
+    if (first <= tol) return hipSuccess;
+    e = launch_cg_init64(n, d, h->d_r2, h->stream);
+    if (e == hipSuccess) e = launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_RZ, h->stream, r2_stage(h));
+    if (e != hipSuccess) return e;
+    int it = 0;
+    while (it < max_it) {
+        const int nb = std::min(check_every - it % check_every, max_it - it);
+        const int it0 = it;
+        e = run_block(h, block_key('G', it0, std::vector<float>(nb, 1.0f), 0.f), [&]() -> hipError_t {
+            hipError_t f = hipSuccess;
+            for (int j = 0; j < nb && f == hipSuccess; ++j) {
+                const int k = it0 + j, rzOld = CG_RZ + (k & 1), rzNew = CG_RZ + ((k + 1) & 1);
+                f = launch_pass64(4, h->m, d, 0, n, d.cg_p, nullptr, d.out, h->d_r2, nullptr, mu, lam, sdt64, ct,
+                                  h->stream);
+                if (f == hipSuccess) f = launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_PQ, h->stream, r2_stage(h));
+                if (f == hipSuccess) f = launch_cg_step64(n, d, d.w, d.out, rzOld, h->d_r2, d.cg_part, h->stream);
+                if (f == hipSuccess) f = launch_sum_fixed(n, h->d_r2, d.cg_scal + rzNew, h->stream, r2_stage(h));
+                if (f == hipSuccess) f = launch_sum_fixed(n, d.cg_part, d.cg_scal + CG_RR, h->stream, d.cg_part + n);
+                if (f == hipSuccess) f = launch_cg_dir64(n, d, rzOld, rzNew, h->stream);
+            }
+            if (f == hipSuccess)
+                f = hipMemcpyAsync(h->h_res, d.cg_scal + CG_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+            return f;
+        });
+        if (e != hipSuccess) return e;
+        it += nb;
+        *iters = it;
+        const double r = std::sqrt(h->h_res[0]), bad = h->h_res[1];  // CG_BAD follows CG_RR
+        if (bad != 0.0 || !std::isfinite(r) || r > 1e3 * std::max(bn, first)) {
+            *broke = true;
+            return hipSuccess;
+        }
+        if (r <= tol) break;
+    }
+    return hipSuccess;
+}
+
+// The adjoint system solved by cg64_run from the caller's rhs and start; the reported residual
+// is recomputed from rhs - A w (the recurrence's r drifts from it in rounding).
 int vbd_adjoint_solve_cg_f64(vbd_solver* h, const double* rhs, double* w, int32_t max_iterations,
                              int32_t check_every, float rel_tol, int32_t* iterations, double* rel_residual) {
     int rc = check_ready(h);
@@ -2073,10 +2230,6 @@ int vbd_adjoint_solve_cg_f64(vbd_solver* h, const double* rhs, double* w, int32_
     if (!(rel_tol >= 0.f)) return fail(h, VBD_ERR_ARG, "rel_tol must be >= 0");
     DeviceGuard g(h->device);
     HIP_CHECK(h, ensure_fp64(h));
-    HIP_CHECK(h, alloc_cg64(h->d64, h->m.nV));
-    const ContactArgs ct = contact_of(h->params);
-    const float mu = step_mu(h), lam = step_lambda(h);
-    const double sdt64 = step_sdt64(h);
     const int n = h->m.nFree;
     Dev64& d = h->d64;
     HIP_CHECK(h, upload_vec64d(h, rhs, d.rhs));
@@ -2086,56 +2239,18 @@ int vbd_adjoint_solve_cg_f64(vbd_solver* h, const double* rhs, double* w, int32_
     HIP_CHECK(h, sum_r2(h, n, &b2));
     const double bn = std::sqrt(b2);
     int it = 0;
-    if (bn > 0 && n > 0) {
-        // the products A p read p at pinned neighbours: zero there (the iterations write free slots only)
-        if (h->m.nV > n)
-            HIP_CHECK(h, hipMemsetAsync(d.cg_p + n, 0, sizeof(d4) * (size_t)(h->m.nV - n), h->stream));
-        // D^-1, r = rhs - A w, z = D^-1 r, p = z, r.z
-        HIP_CHECK(h, launch_pass64(5, h->m, d, 0, n, nullptr, nullptr, d.cg_da, nullptr, d.cg_db, mu, lam, sdt64, ct,
-                                   h->stream));
-        HIP_CHECK(h, launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.cg_r, h->d_r2, nullptr, mu, lam, sdt64, ct,
-                                   h->stream));
-        double r02 = 0.0;
-        HIP_CHECK(h, sum_r2(h, n, &r02));
-        const double first = std::sqrt(r02);  // a start that already meets the tolerance is returned as is
-        HIP_CHECK(h, launch_cg_init64(n, d, h->d_r2, h->stream));
-        HIP_CHECK(h, launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_RZ, h->stream, r2_stage(h)));
-        while (it < max_iterations && first > rel_tol * bn) {
-            const int nb = std::min(check_every - it % check_every, max_iterations - it);
-            const int it0 = it;
-            HIP_CHECK(h, run_block(h, block_key('G', it0, std::vector<float>(nb, 1.0f), 0.f), [&]() -> hipError_t {
-                hipError_t e = hipSuccess;
-                for (int j = 0; j < nb && e == hipSuccess; ++j) {
-                    const int k = it0 + j, rzOld = CG_RZ + (k & 1), rzNew = CG_RZ + ((k + 1) & 1);
-                    e = launch_pass64(4, h->m, d, 0, n, d.cg_p, nullptr, d.out, h->d_r2, nullptr, mu, lam, sdt64, ct,
-                                      h->stream);
-                    if (e == hipSuccess) e = launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_PQ, h->stream, r2_stage(h));
-                    if (e == hipSuccess) e = launch_cg_step64(n, d, d.w, d.out, rzOld, h->d_r2, d.cg_part, h->stream);
-                    if (e == hipSuccess) e = launch_sum_fixed(n, h->d_r2, d.cg_scal + rzNew, h->stream, r2_stage(h));
-                    if (e == hipSuccess) e = launch_sum_fixed(n, d.cg_part, d.cg_scal + CG_RR, h->stream, d.cg_part + n);
-                    if (e == hipSuccess) e = launch_cg_dir64(n, d, rzOld, rzNew, h->stream);
-                }
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(h->h_res, d.cg_scal + CG_RR, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                                       h->stream);
-                return e;
-            }));
-            it += nb;
-            const double rr = h->h_res[0], bad = h->h_res[1];  // CG_BAD follows CG_RR
-            const double r = std::sqrt(rr);
-            if (bad != 0.0 || !std::isfinite(r) || r > 1e3 * std::max(bn, first)) {
-                if (iterations) *iterations = it;
-                if (rel_residual) *rel_residual = r / bn;
-                return fail(h, VBD_ERR_NUMERIC,
-                            "conjugate gradients broke down (A is not positive definite at this state); w is unchanged");
-            }
-            if (r <= rel_tol * bn) break;
-        }
+    bool broke = false;
+    HIP_CHECK(h, cg64_run(h, bn, max_iterations, check_every, (double)rel_tol * bn, &it, &broke));
+    if (broke) {
+        if (iterations) *iterations = it;
+        if (rel_residual) *rel_residual = std::sqrt(h->h_res[0]) / bn;
+        return fail(h, VBD_ERR_NUMERIC,
+                    "conjugate gradients broke down (A is not positive definite at this state); w is unchanged");
     }
-    // the residual of the returned w, recomputed (the recurrence's r drifts from it in rounding)
     double r2 = 0.0;
     if (bn > 0 && n > 0) {
-        HIP_CHECK(h, launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.out, h->d_r2, nullptr, mu, lam, sdt64, ct, h->stream));
+        HIP_CHECK(h, launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.out, h->d_r2, nullptr, step_mu(h), step_lambda(h),
+                                   step_sdt64(h), contact_of(h->params), h->stream));
         HIP_CHECK(h, sum_r2(h, n, &r2));
     }
     HIP_CHECK(h, download_vec64d(h, d.w, w));

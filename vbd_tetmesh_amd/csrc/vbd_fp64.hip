@@ -575,7 +575,40 @@ __global__ __launch_bounds__(256) void k_cg_dir64(int n, d4* __restrict__ p, con
     p[i] = pi;
 }
 
+// dst = a src (the Newton right-hand side -grad G)
+__global__ __launch_bounds__(256) void k_scale_d4(int n, const d4* __restrict__ src, d4* __restrict__ dst, double a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const d4 v = src[i];
+    dst[i] = d4{a * v.x, a * v.y, a * v.z, 0.0};
+}
+
+// pos = base + alpha dx (a Newton step of length alpha from base)
+__global__ __launch_bounds__(256) void k_newton_move64(int n, d4* __restrict__ pos, const d4* __restrict__ base,
+                                                       const d4* __restrict__ dx, double alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const d4 b = base[i], d = dx[i];
+    d4 x = pos[i];
+    x.x = b.x + alpha * d.x;
+    x.y = b.y + alpha * d.y;
+    x.z = b.z + alpha * d.z;
+    pos[i] = x;
+}
+
 static inline dim3 grid256(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t launch_scale_d4(int n, const d4* src, d4* dst, double a, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_d4, grid256(n), dim3(256), 0, s, n, src, dst, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton_move64(int n, d4* pos, const d4* base, const d4* dx, double alpha, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_newton_move64, grid256(n), dim3(256), 0, s, n, pos, base, dx, alpha);
+    return hipGetLastError();
+}
 
 hipError_t launch_cg_init64(int n, const Dev64& d, double* part, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_init64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, d.cg_r, d.cg_z, d.cg_p, d.cg_da,

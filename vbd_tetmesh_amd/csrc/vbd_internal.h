@@ -223,6 +223,8 @@ hipError_t launch_cg_init64(int n, const Dev64& d, double* part, hipStream_t s);
 hipError_t launch_cg_step64(int n, const Dev64& d, d4* x, const d4* q, int rzOld, double* partRz, double* partRr,
                             hipStream_t s);
 hipError_t launch_cg_dir64(int n, const Dev64& d, int rzOld, int rzNew, hipStream_t s);
+hipError_t launch_scale_d4(int n, const d4* src, d4* dst, double a, hipStream_t s);
+hipError_t launch_newton_move64(int n, d4* pos, const d4* base, const d4* dx, double alpha, hipStream_t s);
 void free_dev64(Dev64& d);
 // mode: 0 Gauss-Seidel position update (one color), 1 forward residual grad G, 2 adjoint
 // block-GS update (one color), 3 adjoint residual rhs - A w, 4 A w (with r2: w . A w per slot),

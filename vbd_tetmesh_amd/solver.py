@@ -485,6 +485,17 @@ class DeviceMesh(_Handle):
         lam = self._per_tet(tet_lambda, "tet_lambda")
         check(lib.vbd_set_materials(self.h, ptr(mu, C.c_float), ptr(lam, C.c_float)), self.h)
 
+    def set_converged_method(self, method: int):
+        """VBD_CONVERGED_VBD (VBD sweeps, the default) or VBD_CONVERGED_NEWTON (Newton steps with
+        conjugate-gradient solves, precision 64) for step_converged."""
+        check(lib.vbd_set_converged_method(self.h, int(method)), self.h)
+
+    def last_step_stats(self) -> Tuple[int, int, int]:
+        """(Newton steps, CG iterations inside them, VBD sweeps) of the last step_converged."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib.vbd_last_step_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), self.h)
+        return a.value, b.value, c.value
+
     def residual_norm(self):
         """(|grad G| at the current state, |M g|)."""
         r, l = C.c_double(), C.c_double()
