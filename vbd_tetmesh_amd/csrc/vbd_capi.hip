@@ -2149,12 +2149,14 @@ int vbd_adjoint_solve_f64(vbd_solver* h, const double* rhs, double* w, int32_t m
 }
 
 // Conjugate gradients with the block-Jacobi preconditioner on A = M/h^2 + K (+ contact) at the
-// FP64 state: rhs in d64.rhs (norm bn), x in d64.w (its start on entry).  Per iteration: A p with
-// p.Ap per slot, its fixed-order sum, the x / r / z update with r.z and r.r per slot, their sums,
-// the new direction; blocks of check_every iterations replay one cached graph (keyed by the block
-// length and the parity of its first iteration, which selects the r.z slots), and the host reads
-// r.r and the breakdown flag after each block.  Stops when |r| <= tol or after max_it iterations;
-// *broke when p.Ap or r.z was not positive (A not positive definite) or |r| grew 1e3-fold.
+// FP64 state, in the single-reduction form (the kernels' comment in vbd_fp64.hip): rhs in
+// d64.rhs (norm bn), x in d64.w (its start on entry).  Per iteration three launches: the update
+// of the directions, x, r and z with r.z and r.r per slot; w = A z with w.z per slot; one launch
+// summing the three (fixed order).  Blocks of check_every iterations replay one cached graph,
+// keyed by the block length, the parity of its first iteration (which selects the scalar slots)
+// and whether it starts the solve; the host reads r.r and the breakdown flag after each block.
+// Stops when |r| <= tol or after max_it iterations; *broke when gamma or the alpha denominator
+// was not positive (A not positive definite) or |r| grew 1e3-fold.
 static hipError_t cg64_run(vbd_solver* h, double bn, int max_it, int check_every, double tol, int* iters,
                            bool* broke) {
     const ContactArgs ct = contact_of(h->params);
@@ -2166,37 +2168,51 @@ static hipError_t cg64_run(vbd_solver* h, double bn, int max_it, int check_every
     *broke = false;
     if (!(bn > 0) || n == 0) return hipSuccess;
     hipError_t e = alloc_cg64(d, h->m.nV);
-    // the products A p read p at pinned neighbours: zero there (the iterations write free slots only)
-    if (e == hipSuccess && h->m.nV > n)
-        e = hipMemsetAsync(d.cg_p + n, 0, sizeof(d4) * (size_t)(h->m.nV - n), h->stream);
-    // D^-1, r = rhs - A x, z = D^-1 r, p = z, r.z
+    if (e != hipSuccess) return e;
+    // per-slot products: r.z, w.z, r.r (each with SUM_STAGE doubles of two-stage scratch after it)
+    const size_t stride = (size_t)h->m.nV + SUM_STAGE;
+    double* pRz = h->d_r2;
+    double* pWz = d.cg_part;
+    double* pRr = d.cg_part + stride;
+    auto sums = [&](int gammaSlot) -> hipError_t {
+        if (n <= SUM_TWO_STAGE)
+            return launch_sum3_fixed(n, pRz, d.cg_scal + gammaSlot, pWz, d.cg_scal + CG_D, pRr, d.cg_scal + CG_RR,
+                                     h->stream);
+        hipError_t f = launch_sum_fixed(n, pRz, d.cg_scal + gammaSlot, h->stream, pRz + n);
+        if (f == hipSuccess) f = launch_sum_fixed(n, pWz, d.cg_scal + CG_D, h->stream, pWz + n);
+        if (f == hipSuccess) f = launch_sum_fixed(n, pRr, d.cg_scal + CG_RR, h->stream, pRr + n);
+        return f;
+    };
+    // the products A z read z at pinned neighbours: zero there (the iterations write free slots only)
+    if (h->m.nV > n) e = hipMemsetAsync(d.cg_z + n, 0, sizeof(d4) * (size_t)(h->m.nV - n), h->stream);
+    // D^-1, r = rhs - A x, z = D^-1 r, p = s = 0, w = A z, gamma_0, delta_0
     if (e == hipSuccess)
         e = launch_pass64(5, h->m, d, 0, n, nullptr, nullptr, d.cg_da, nullptr, d.cg_db, mu, lam, sdt64, ct, h->stream);
     if (e == hipSuccess)
-        e = launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.cg_r, h->d_r2, nullptr, mu, lam, sdt64, ct, h->stream);
+        e = launch_pass64(3, h->m, d, 0, n, d.w, d.rhs, d.cg_r, pRz, nullptr, mu, lam, sdt64, ct, h->stream);
     double r02 = 0.0;
     if (e == hipSuccess) e = sum_r2(h, n, &r02);
     if (e != hipSuccess) return e;
     const double first = std::sqrt(r02);  // a start that already meets the tolerance is returned as is
     if (first <= tol) return hipSuccess;
-    e = launch_cg_init64(n, d, h->d_r2, h->stream);
-    if (e == hipSuccess) e = launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_RZ, h->stream, r2_stage(h));
+    e = launch_cg_init64(n, d, pRz, pRr, h->stream);
+    if (e == hipSuccess)
+        e = launch_pass64(4, h->m, d, 0, n, d.cg_z, nullptr, d.out, pWz, nullptr, mu, lam, sdt64, ct, h->stream);
+    if (e == hipSuccess) e = sums(CG_G);
     if (e != hipSuccess) return e;
     int it = 0;
     while (it < max_it) {
         const int nb = std::min(check_every - it % check_every, max_it - it);
         const int it0 = it;
-        e = run_block(h, block_key('G', it0, std::vector<float>(nb, 1.0f), 0.f), [&]() -> hipError_t {
+        e = run_block(h, block_key(it0 == 0 ? 'H' : 'G', it0, std::vector<float>(nb, 1.0f), 0.f), [&]() -> hipError_t {
             hipError_t f = hipSuccess;
             for (int j = 0; j < nb && f == hipSuccess; ++j) {
-                const int k = it0 + j, rzOld = CG_RZ + (k & 1), rzNew = CG_RZ + ((k + 1) & 1);
-                f = launch_pass64(4, h->m, d, 0, n, d.cg_p, nullptr, d.out, h->d_r2, nullptr, mu, lam, sdt64, ct,
-                                  h->stream);
-                if (f == hipSuccess) f = launch_sum_fixed(n, h->d_r2, d.cg_scal + CG_PQ, h->stream, r2_stage(h));
-                if (f == hipSuccess) f = launch_cg_step64(n, d, d.w, d.out, rzOld, h->d_r2, d.cg_part, h->stream);
-                if (f == hipSuccess) f = launch_sum_fixed(n, h->d_r2, d.cg_scal + rzNew, h->stream, r2_stage(h));
-                if (f == hipSuccess) f = launch_sum_fixed(n, d.cg_part, d.cg_scal + CG_RR, h->stream, d.cg_part + n);
-                if (f == hipSuccess) f = launch_cg_dir64(n, d, rzOld, rzNew, h->stream);
+                const int k = it0 + j;
+                f = launch_cg_update64(n, k, d, d.w, d.out, pRz, pRr, h->stream);
+                if (f == hipSuccess)
+                    f = launch_pass64(4, h->m, d, 0, n, d.cg_z, nullptr, d.out, pWz, nullptr, mu, lam, sdt64, ct,
+                                      h->stream);
+                if (f == hipSuccess) f = sums(CG_G + ((k + 1) & 1));
             }
             if (f == hipSuccess)
                 f = hipMemcpyAsync(h->h_res, d.cg_scal + CG_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream);

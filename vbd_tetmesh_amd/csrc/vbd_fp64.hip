@@ -509,11 +509,16 @@ __global__ __launch_bounds__(256) void k_param_grad64(int nT, const i4* __restri
 }
 
 // ---------------------------------------------------------------------------------------
-// Conjugate gradients on A = M/h^2 + K (+ contact) with the block-Jacobi preconditioner
-// D^-1 (the inverses of the 3x3 diagonal blocks, P64_DIAG_INV).  The scalars live on the device
-// (scal[CG_PQ] = p.Ap, scal[CG_RZ + 0/1] = r.z of alternate iterations, scal[CG_RR] = r.r,
-// scal[CG_BAD] set when p.Ap or r.z is not positive), so a block of iterations has no host
-// round trip; every dot product is a per-vertex product followed by the fixed-order sum.
+// Conjugate gradients on A = M/h^2 + K (+ contact) with the block-Jacobi preconditioner D^-1
+// (the inverses of the 3x3 diagonal blocks, P64_DIAG_INV), in the single-reduction form of
+// Chronopoulos and Gear: with s = A p carried by the recurrence s = w + beta s (w = A z), the
+// two dot products of an iteration, gamma = r.z and delta = w.z, come from one reduction, so an
+// iteration is three launches: the update below, the A z pass (with w.z per slot) and one
+// launch summing gamma, delta and r.r.  Iteration k:
+//   beta_k = gamma_k / gamma_{k-1} (0 for k = 0), alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//   p = z + beta p, s = w + beta s, x += alpha p, r -= alpha s, z = D^-1 r  (then w = A z)
+// The scalars stay on the device: cg_scal[CG_G + k&1] = gamma, [CG_D] = delta, [CG_A + k&1] =
+// alpha, [CG_RR] = r.r, [CG_BAD] set when gamma or the alpha denominator is not positive.
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ d4 dinv_apply(const d4 a, const d4 b, const d4 r) {
     // [[a.x a.y a.z] [a.y a.w b.x] [a.z b.x b.y]] r
@@ -523,56 +528,76 @@ __device__ __forceinline__ d4 dinv_apply(const d4 a, const d4 b, const d4 r) {
 
 __device__ __forceinline__ double dot3(const d4 a, const d4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-// z = D^-1 r, p = z, part = r.z
+// z = D^-1 r, p = s = 0, part_rz = r.z, part_rr = r.r (the start: r = rhs - A x)
 __global__ __launch_bounds__(256) void k_cg_init64(int n, const d4* __restrict__ r, d4* __restrict__ z,
-                                                   d4* __restrict__ p, const d4* __restrict__ da,
-                                                   const d4* __restrict__ db, double* __restrict__ part,
-                                                   double* __restrict__ scal) {
+                                                   d4* __restrict__ p, d4* __restrict__ sv, const d4* __restrict__ da,
+                                                   const d4* __restrict__ db, double* __restrict__ partRz,
+                                                   double* __restrict__ partRr, double* __restrict__ scal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) scal[CG_BAD] = 0.0;
     if (i >= n) return;
     const d4 ri = r[i];
     const d4 zi = dinv_apply(da[i], db[i], ri);
     z[i] = zi;
-    p[i] = zi;
-    part[i] = dot3(ri, zi);
-}
-
-// alpha = r.z / p.Ap; x += alpha p; r -= alpha q; z = D^-1 r; part_rz = r.z, part_rr = r.r
-__global__ __launch_bounds__(256) void k_cg_step64(int n, d4* __restrict__ x, d4* __restrict__ r,
-                                                   const d4* __restrict__ p, const d4* __restrict__ q,
-                                                   d4* __restrict__ z, const d4* __restrict__ da,
-                                                   const d4* __restrict__ db, double* __restrict__ scal, int rzOld,
-                                                   double* __restrict__ partRz, double* __restrict__ partRr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const double pq = scal[CG_PQ], rz = scal[rzOld];
-    if (i == 0 && !(pq > 0.0 && rz > 0.0)) scal[CG_BAD] = 1.0;
-    if (i >= n) return;
-    const double alpha = rz / pq;
-    const d4 pi = p[i], qi = q[i];
-    d4 xi = x[i], ri = r[i];
-    xi.x += alpha * pi.x; xi.y += alpha * pi.y; xi.z += alpha * pi.z;
-    ri.x -= alpha * qi.x; ri.y -= alpha * qi.y; ri.z -= alpha * qi.z;
-    x[i] = xi;
-    r[i] = ri;
-    const d4 zi = dinv_apply(da[i], db[i], ri);
-    z[i] = zi;
+    p[i] = d4{0.0, 0.0, 0.0, 0.0};
+    sv[i] = d4{0.0, 0.0, 0.0, 0.0};
     partRz[i] = dot3(ri, zi);
     partRr[i] = dot3(ri, ri);
 }
 
-// beta = (r.z)_new / (r.z)_old; p = z + beta p
-__global__ __launch_bounds__(256) void k_cg_dir64(int n, d4* __restrict__ p, const d4* __restrict__ z,
-                                                  const double* __restrict__ scal, int rzOld, int rzNew) {
+// iteration k (see above; par = k & 1, first = (k == 0)): directions, x and r, the new z, and
+// r.z, r.r per slot
+__global__ __launch_bounds__(256) void k_cg_update64(int n, int par, int first, d4* __restrict__ x, d4* __restrict__ r,
+                                                     d4* __restrict__ z, d4* __restrict__ p, d4* __restrict__ sv,
+                                                     const d4* __restrict__ w, const d4* __restrict__ da,
+                                                     const d4* __restrict__ db, double* __restrict__ scal,
+                                                     double* __restrict__ partRz, double* __restrict__ partRr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double gamma = scal[CG_G + par], delta = scal[CG_D];
+    double beta = 0.0, denom = delta;
+    if (!first) {
+        beta = gamma / scal[CG_G + (par ^ 1)];
+        denom = delta - beta * gamma / scal[CG_A + (par ^ 1)];
+    }
+    const double alpha = gamma / denom;
+    if (i == 0) {
+        scal[CG_A + par] = alpha;
+        if (!(gamma > 0.0 && denom > 0.0)) scal[CG_BAD] = 1.0;
+    }
     if (i >= n) return;
-    const double beta = scal[rzNew] / scal[rzOld];
-    const d4 zi = z[i];
-    d4 pi = p[i];
-    pi.x = zi.x + beta * pi.x;
-    pi.y = zi.y + beta * pi.y;
-    pi.z = zi.z + beta * pi.z;
+    const d4 zi = z[i], wi = w[i];
+    d4 pi = p[i], si = sv[i], xi = x[i], ri = r[i];
+    pi.x = zi.x + beta * pi.x; pi.y = zi.y + beta * pi.y; pi.z = zi.z + beta * pi.z;
+    si.x = wi.x + beta * si.x; si.y = wi.y + beta * si.y; si.z = wi.z + beta * si.z;
+    xi.x += alpha * pi.x; xi.y += alpha * pi.y; xi.z += alpha * pi.z;
+    ri.x -= alpha * si.x; ri.y -= alpha * si.y; ri.z -= alpha * si.z;
     p[i] = pi;
+    sv[i] = si;
+    x[i] = xi;
+    r[i] = ri;
+    const d4 zn = dinv_apply(da[i], db[i], ri);
+    z[i] = zn;
+    partRz[i] = dot3(ri, zn);
+    partRr[i] = dot3(ri, ri);
+}
+
+// Fixed-order sums of three arrays in one launch, block b summing array b exactly as
+// k_sum_fixed does (the same order, so the same bits)
+__global__ __launch_bounds__(1024) void k_sum3_fixed(int n, const double* __restrict__ v0, double* __restrict__ o0,
+                                                     const double* __restrict__ v1, double* __restrict__ o1,
+                                                     const double* __restrict__ v2, double* __restrict__ o2) {
+    __shared__ double sh[1024];
+    const double* v = blockIdx.x == 0 ? v0 : (blockIdx.x == 1 ? v1 : v2);
+    double* out = blockIdx.x == 0 ? o0 : (blockIdx.x == 1 ? o1 : o2);
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) acc += v[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // dst = a src (the Newton right-hand side -grad G)
@@ -610,22 +635,24 @@ hipError_t launch_newton_move64(int n, d4* pos, const d4* base, const d4* dx, do
     return hipGetLastError();
 }
 
-hipError_t launch_cg_init64(int n, const Dev64& d, double* part, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_init64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, d.cg_r, d.cg_z, d.cg_p, d.cg_da,
-                       d.cg_db, part, d.cg_scal);
+hipError_t launch_cg_init64(int n, const Dev64& d, double* partRz, double* partRr, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_init64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, d.cg_r, d.cg_z, d.cg_p, d.cg_s, d.cg_da,
+                       d.cg_db, partRz, partRr, d.cg_scal);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_step64(int n, const Dev64& d, d4* x, const d4* q, int rzOld, double* partRz, double* partRr,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_step64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, x, d.cg_r, d.cg_p, q, d.cg_z, d.cg_da,
-                       d.cg_db, d.cg_scal, rzOld, partRz, partRr);
+hipError_t launch_cg_update64(int n, int k, const Dev64& d, d4* x, const d4* w, double* partRz, double* partRr,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_update64, grid256(n > 0 ? n : 1), dim3(256), 0, s, n, k & 1, k == 0 ? 1 : 0, x, d.cg_r,
+                       d.cg_z, d.cg_p, d.cg_s,
+                       w, d.cg_da, d.cg_db, d.cg_scal, partRz, partRr);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_dir64(int n, const Dev64& d, int rzOld, int rzNew, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cg_dir64, grid256(n), dim3(256), 0, s, n, d.cg_p, d.cg_z, d.cg_scal, rzOld, rzNew);
+hipError_t launch_sum3_fixed(int n, const double* v0, double* o0, const double* v1, double* o1, const double* v2,
+                             double* o2, hipStream_t s) {
+    if (n > SUM_TWO_STAGE) return hipErrorInvalidValue;  // the callers sum larger arrays in two stages
+    hipLaunchKernelGGL(k_sum3_fixed, dim3(3), dim3(1024), 0, s, n, v0, o0, v1, o1, v2, o2);
     return hipGetLastError();
 }
 
@@ -713,14 +740,15 @@ hipError_t launch_param_grad64(const DevMesh& m, const Dev64& d, const d4* w, do
 
 hipError_t alloc_cg64(Dev64& d, int nV) {
     const size_t n = (size_t)(nV > 0 ? nV : 1);
-    d4** arrs[] = {&d.cg_r, &d.cg_z, &d.cg_p, &d.cg_da, &d.cg_db};
+    d4** arrs[] = {&d.cg_r, &d.cg_z, &d.cg_p, &d.cg_s, &d.cg_da, &d.cg_db};
     for (d4** a : arrs) {
         if (*a) continue;
         hipError_t e = hipMalloc((void**)a, sizeof(d4) * n);
         if (e != hipSuccess) return e;
     }
     hipError_t e = hipSuccess;
-    if (!d.cg_part && (e = hipMalloc((void**)&d.cg_part, sizeof(double) * (n + SUM_STAGE))) != hipSuccess) return e;
+    if (!d.cg_part && (e = hipMalloc((void**)&d.cg_part, sizeof(double) * 2 * (n + SUM_STAGE))) != hipSuccess)
+        return e;
     if (!d.cg_scal && (e = hipMalloc((void**)&d.cg_scal, sizeof(double) * CG_SCALARS)) != hipSuccess) return e;
     return e;
 }
@@ -739,7 +767,7 @@ hipError_t alloc_dev64(Dev64& d, int nV) {
 
 void free_dev64(Dev64& d) {
     d4* arrs[] = {d.pos, d.inert, d.vel, d.xprev, d.save[0], d.save[1], d.w, d.rhs, d.out, d.wsave[0], d.wsave[1],
-                  d.cg_r, d.cg_z, d.cg_p, d.cg_da, d.cg_db};
+                  d.cg_r, d.cg_z, d.cg_p, d.cg_s, d.cg_da, d.cg_db};
     for (d4* a : arrs)
         if (a) (void)hipFree(a);
     if (d.cg_part) (void)hipFree(d.cg_part);

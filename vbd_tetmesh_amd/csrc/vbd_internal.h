@@ -205,24 +205,27 @@ struct Dev64 {
     d4* rhs = nullptr;
     d4* out = nullptr;
     d4* wsave[2] = {nullptr, nullptr};
-    // conjugate-gradient adjoint solve (allocated on first use): residual, preconditioned residual,
-    // direction, the block-Jacobi inverse (two double4 per slot), per-slot products, scalars
+    // conjugate gradients (allocated on first use): residual, preconditioned residual, direction,
+    // its product with A, the block-Jacobi inverse (two double4 per slot), two arrays of per-slot
+    // products (2 (nV + SUM_STAGE) doubles), scalars
     d4* cg_r = nullptr;
     d4* cg_z = nullptr;
     d4* cg_p = nullptr;
+    d4* cg_s = nullptr;
     d4* cg_da = nullptr;
     d4* cg_db = nullptr;
     double* cg_part = nullptr;
     double* cg_scal = nullptr;
 };
-// slots of Dev64::cg_scal
-constexpr int CG_PQ = 0, CG_RZ = 1, CG_RR = 3, CG_BAD = 4, CG_SCALARS = 8;
+// slots of Dev64::cg_scal: gamma (two, alternate iterations), delta, alpha (two), r.r, breakdown flag
+constexpr int CG_G = 0, CG_D = 2, CG_A = 3, CG_RR = 5, CG_BAD = 6, CG_SCALARS = 8;
 hipError_t alloc_dev64(Dev64& d, int nV);
 hipError_t alloc_cg64(Dev64& d, int nV);
-hipError_t launch_cg_init64(int n, const Dev64& d, double* part, hipStream_t s);
-hipError_t launch_cg_step64(int n, const Dev64& d, d4* x, const d4* q, int rzOld, double* partRz, double* partRr,
-                            hipStream_t s);
-hipError_t launch_cg_dir64(int n, const Dev64& d, int rzOld, int rzNew, hipStream_t s);
+hipError_t launch_cg_init64(int n, const Dev64& d, double* partRz, double* partRr, hipStream_t s);
+hipError_t launch_cg_update64(int n, int k, const Dev64& d, d4* x, const d4* w, double* partRz, double* partRr,
+                              hipStream_t s);
+hipError_t launch_sum3_fixed(int n, const double* v0, double* o0, const double* v1, double* o1, const double* v2,
+                             double* o2, hipStream_t s);
 hipError_t launch_scale_d4(int n, const d4* src, d4* dst, double a, hipStream_t s);
 hipError_t launch_newton_move64(int n, d4* pos, const d4* base, const d4* dx, double alpha, hipStream_t s);
 void free_dev64(Dev64& d);
