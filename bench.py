@@ -741,10 +741,12 @@ def main():
         d = None
         bb = bench_batch(**batch_kw)
         out["batched"] = bb
+        base = (f"; per-mesh plain {bb['per_mesh_plain_ms_per_step']:.2f} ms/step "
+                f"({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph {bb['per_mesh_graph_ms_per_step']:.2f} "
+                f"ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)" if bb['per_mesh_plain_ms_per_step'] else
+                "; per-mesh baselines skipped")
         log(f"[bench] batched: {bb['value']:.1f} M vertex-iters/s, {bb['ms_per_step']:.3f} ms/step, color kernel frac "
-            f"{bb['roofline']['frac']}; per-mesh plain "
-            f"{bb['per_mesh_plain_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_plain']:.1f}x), per-mesh graph "
-            f"{bb['per_mesh_graph_ms_per_step']:.2f} ms/step ({bb['speedup_vs_per_mesh_graph']:.1f}x)")
+            f"{bb['roofline']['frac']}{base}")
     if not args.no_sphere:
         sp = bench_sphere(frames=args.sphere_frames, slot_order=args.slot_order, persistent=not args.no_persistent_ab)
         # the same drop onto a plane the sinking sphere reaches within the frames
