@@ -242,9 +242,10 @@ int vbd_time_color_launches(vbd_solver* h, int32_t sweeps, double* ms_per_launch
  * (no reference counterpart): per (free vertex, incident tet) record the record as stored (32 B
  * compact for the tiled stable Neo-Hookean kernel, 52 B full for the per-vertex kernels and the
  * corotated material; +8 B with per-tet mu, lambda), per free vertex 56 B (row offset 4, x read
- * 16, inertia 16, mass 4, x write 16) plus 16 B of saved position when the parameters enable
- * Chebyshev; the blend launches 48 B per free vertex (x and x_pp read, x written).  Tile padding
- * and neighbour-position gathers (cache-resident) are not counted. */
+ * 16, inertia 16, mass 4, x write 16); the blend launches 64 B per free vertex (x and x_pp read,
+ * x written, and written again as the next sweep's start).  That is a sweep of a Chebyshev step
+ * after its first two, whose color launches also save the sweep's start (16 B per free vertex).
+ * Tile padding and neighbour-position gathers (cache-resident) are not counted. */
 int vbd_layout_bytes(const vbd_solver* h, double* color_bytes_per_sweep, double* blend_bytes_per_sweep);
 /* Algorithmic bytes of one sweep, a fixed yardstick independent of the device layout:
  * color launches 45 B per free vertex (x read 12, x write 12, inertia 12, mass 4, CSR row 4,

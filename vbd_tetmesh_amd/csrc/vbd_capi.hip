@@ -326,17 +326,20 @@ int persistent_grid(const vbd_solver* h, const ContactArgs& ct, bool save_on) {
 }
 
 // Iteration loop of VBDsolver::step (VBD.cpp:75-96) / VBD_GPU_Solver::solveVBD (.cu:338-361):
-// the color kernels save each vertex's start-of-sweep position into save[it & 1]; after the
-// sweep, omega > 1 blends against save[(it + 1) & 1], the start of the previous sweep, which
-// is the prevPrevPos of the reference (the two buffers alternate instead of being copied).
+// save[it & 1] holds each vertex's start-of-sweep position; after the sweep, omega > 1 blends
+// against save[(it + 1) & 1], the start of the previous sweep, which is the prevPrevPos of the
+// reference (the two buffers alternate instead of being copied).  The blend also writes its
+// result into save[(it + 1) & 1], the next sweep's start, so only the sweeps that no blend
+// precedes (the first two) save from their color launches.
 // With `elog`, the diagnostic energy (VBD.cpp:92-94) after iterations 0, every, 2 every, ...
 // goes to elog[it / every] (mu, lambda: the energy's material, the step parameters).
 hipError_t enqueue_iterations(vbd_solver* h, int iters, float sdt, float mu, float lambda, bool accel, float rho,
                               const ContactArgs& ct, double* elog = nullptr, int every = 0) {
     float omega = 1.0f;
     const int pgrid = persistent_on(h) ? persistent_grid(h, ct, accel) : 0;
+    bool kept = false;  // the previous blend wrote this sweep's start into save[it & 1]
     for (int it = 0; it < iters; ++it) {
-        f4* save = accel ? h->m.save[it & 1] : nullptr;
+        f4* save = accel && !kept ? h->m.save[it & 1] : nullptr;
         if (pgrid > 0) {  // one launch: every color, then the blend
             if (accel) omega = omega_ref(it + 1, rho, omega);
             hipError_t e = launch_sweep_persistent(h->m, h->unroll, h->block, pgrid, sdt, mu, lambda, ct, save,
@@ -351,8 +354,9 @@ hipError_t enqueue_iterations(vbd_solver* h, int iters, float sdt, float mu, flo
         if (e != hipSuccess) return e;
         if (accel) {
             omega = omega_ref(it + 1, rho, omega);
-            if (omega > 1.0f) {
-                e = launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream);
+            kept = omega > 1.0f;
+            if (kept) {
+                e = launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream, true);
                 if (e != hipSuccess) return e;
             }
         }
@@ -1282,8 +1286,9 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
     };
     hipError_t e = hipSuccess;
     float omega = 1.0f;
+    bool kept = false;  // as in the step: a blend writes the next sweep's start
     for (int it = 0; it < iterations && e == hipSuccess; ++it) {
-        f4* save = accel ? h->m.save[it & 1] : nullptr;
+        f4* save = accel && !kept ? h->m.save[it & 1] : nullptr;
         for (int c = 0; c < h->m.nColors && e == hipSuccess; ++c) {
             if (h->m.colorSlotOff[c + 1] == h->m.colorSlotOff[c]) continue;
             if ((e = mk(false)) != hipSuccess) break;
@@ -1293,10 +1298,11 @@ int vbd_time_iterations(vbd_solver* h, int32_t iterations, double* color_ms, int
         }
         if (accel && e == hipSuccess) {
             omega = omega_ref(it + 1, p.acceleration_rho, omega);
-            if (omega > 1.0f) {
+            kept = omega > 1.0f;
+            if (kept) {
                 if ((e = mk(true)) != hipSuccess) break;
                 (void)hipEventRecord(evs.back().a, h->stream);
-                e = launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream);
+                e = launch_blend(h->m, omega, h->m.save[(it + 1) & 1], h->stream, true);
                 (void)hipEventRecord(evs.back().b, h->stream);
             }
         }
@@ -1501,8 +1507,9 @@ int vbd_time_color_launches(vbd_solver* h, int32_t sweeps, double* ms_per_launch
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         hipError_t ee = hipSuccess;
-        for (int it = 0; it < sweeps && ee == hipSuccess; ++it)
-            ee = enqueue_sweep(h, sdt, mu, lam, ct, accel ? h->m.save[it & 1] : nullptr);
+        // the step's color launches after its first two sweeps: the blends write each sweep's
+        // start, so the color launches save nothing
+        for (int it = 0; it < sweeps && ee == hipSuccess; ++it) ee = enqueue_sweep(h, sdt, mu, lam, ct, nullptr);
         e = hipStreamEndCapture(h->stream, &graph);
         if (ee != hipSuccess) e = ee;
     }
@@ -1532,10 +1539,8 @@ int vbd_layout_bytes(const vbd_solver* h, double* color_bytes, double* blend_byt
     if (!h->inited) return fail(const_cast<vbd_solver*>(h), VBD_ERR_STATE, "not initialised");
     const bool compact = h->lpv == 0 && h->materialModel == VBD_MATERIAL_STABLE_NEO_HOOKEAN;
     const double per_rec = (compact ? 32.0 : 52.0) + (h->m.perTetMat ? 8.0 : 0.0);
-    const bool accel = h->paramsSet && h->params.use_acceleration;
-    const double per_vert = 56.0 + (accel ? 16.0 : 0.0);
-    if (color_bytes) *color_bytes = per_vert * h->m.nFree + per_rec * (double)h->m.nPairs;
-    if (blend_bytes) *blend_bytes = 48.0 * h->m.nFree;
+    if (color_bytes) *color_bytes = 56.0 * h->m.nFree + per_rec * (double)h->m.nPairs;
+    if (blend_bytes) *blend_bytes = 64.0 * h->m.nFree;
     return VBD_OK;
 }
 

@@ -131,7 +131,7 @@ struct TileCut {
 void detach_tiles(DevMesh& m, TileCut& out);                  // m is left without a cut
 void attach_tiles(DevMesh& m, TileCut& in, hipStream_t s);     // frees m's cut, moves `in` back
 void free_tile_cut(TileCut& t);
-hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s);
+hipError_t launch_blend(const DevMesh& m, float omega, f4* pp, hipStream_t s, bool keep = false);
 // One launch per sweep: every color through the tiled kernel's tile_solve with a grid barrier
 // between colors, then the Chebyshev blend when omega > 1 (pp: x at the start of the previous
 // sweep).  grid: a multiple of 8 not above the resident blocks (persistent_occupancy per CU x

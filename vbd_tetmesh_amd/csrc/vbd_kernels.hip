@@ -1802,7 +1802,12 @@ hipError_t launch_blend_vec(int n, f4* v, const f4* pp, float omega, hipStream_t
 // ---------------------------------------------------------------------------------------
 // The Chebyshev step (VBD.cpp:290-300): omega (x - x_pp) + x_pp, rounded after the product and
 // after the sum as the reference's CPU code is (no contraction).
-__global__ __launch_bounds__(256) void k_blend(int n, f4* __restrict__ pos, const f4* __restrict__ pp, float omega) {
+// x = omega (x - x_pp) + x_pp over the free slots (VBD.cpp:290-300).  With keep, the blended x
+// also replaces x_pp: it is the next sweep's starting position, which that sweep's blend needs two
+// iterations later, so the next sweep's color launches need not save it (their per-vertex stores
+// come from one lane in four; here they are one coalesced store per vertex).
+__global__ __launch_bounds__(256) void k_blend(int n, f4* __restrict__ pos, f4* __restrict__ pp, float omega,
+                                               int keep) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1813,11 +1818,12 @@ __global__ __launch_bounds__(256) void k_blend(int n, f4* __restrict__ pos, cons
     r.y = omega * (x.y - q.y) + q.y;
     r.z = omega * (x.z - q.z) + q.z;
     pos[i] = r;
+    if (keep) pp[i] = r;
 }
 
-hipError_t launch_blend(const DevMesh& m, float omega, const f4* pp, hipStream_t s) {
+hipError_t launch_blend(const DevMesh& m, float omega, f4* pp, hipStream_t s, bool keep) {
     if (m.nFree == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blend, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.pos, pp, omega);
+    hipLaunchKernelGGL(k_blend, dim3((m.nFree + 255) / 256), dim3(256), 0, s, m.nFree, m.pos, pp, omega, keep ? 1 : 0);
     return hipGetLastError();
 }
 
