@@ -1493,7 +1493,6 @@ int vbd_time_color_launches(vbd_solver* h, int32_t sweeps, double* ms_per_launch
     DeviceGuard g(h->device);
     h->fp64Fresh = false;  // the FP32 state changes
     const vbd_step_params p = h->params;
-    const bool accel = h->paramsSet && p.use_acceleration;
     const float hstep = h->paramsSet ? p.dt / (float)p.substeps : 1.0f / 60.0f;
     const float sdt = inv_h2_ref(hstep);
     const float mu = h->paramsSet ? p.mu : 1e5f, lam = h->paramsSet ? p.lambda : 4e5f;
